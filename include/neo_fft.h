@@ -1,0 +1,179 @@
+/*
+ * neo_fft.h -- C-ABI of the MI355X (gfx950) blockwise spectral-filter core of
+ * neo-fft's DFTTest and FFT3D.
+ *
+ * This boundary replaces the call surface of the reference's Plan class
+ * (src/algorithms/plan.hpp:49-140): the host layer (parameter parsing,
+ * temporal slot selection, Kalman checkpoint policy; reference
+ * src/plugin/filter.hpp) calls these functions instead of Plan's methods.
+ * Plain C types only; every pointer to sample memory is device memory
+ * (hipMalloc or torch CUDA tensors) unless stated otherwise.
+ *
+ * Errors: every function returns nf_status. NF_EINVAL carries the same texts
+ * as the reference's std::invalid_argument (src/base/checked.hpp:13-20);
+ * NF_ENONFINITE is the reference's frame error "non-finite sample or
+ * intermediate" (checked.hpp:37-41). nf_last_error() returns the message of
+ * the calling thread's last failure. A frame that fails must not be used:
+ * the destination plane may hold partial results.
+ *
+ * Threading: plans are immutable after creation and may be used from several
+ * host threads at once (one stream per thread, or the default stream).
+ */
+#ifndef NEO_FFT_H
+#define NEO_FFT_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum nf_status {
+  NF_OK = 0,
+  NF_EINVAL = 1,       /* invalid parameter / shape (reference std::invalid_argument) */
+  NF_EUNSUPPORTED = 2, /* valid for the reference, not implemented on this device path */
+  NF_ENONFINITE = 3,   /* non-finite sample or intermediate (frame error) */
+  NF_ENOMEM = 4,
+  NF_EDEVICE = 5       /* HIP runtime error */
+} nf_status;
+
+/* Error bits OR-ed by the kernels into a caller-supplied device word. */
+#define NF_FLAG_NONFINITE 1u
+
+/* SampleFormat (reference src/algorithms/plan.hpp:20-23). bits 8..16 integer
+ * (u8 storage for 8, u16 otherwise) or 32 with floating=1. chroma selects the
+ * FFT3D integer midpoint (docs/knowledge/en/shared/sample-domain.md). */
+typedef struct nf_sample_format {
+  int bits;
+  int floating;
+  int chroma;
+} nf_sample_format;
+
+/* DFTTest parameters: names, meaning and defaults of the public function
+ * (reference src/plugin/descriptors.hpp:200-235; DFTConfig plan.hpp:37-46).
+ * Curves are (position, value) pairs; n_* counts floats. nlocation holds
+ * (frame, plane, y, x) quadruples; n_nlocation counts ints. */
+typedef struct nf_dfttest_config {
+  int ftype;
+  float sigma, sigma2, pmin, pmax;
+  int sbsize, smode, sosize, tbsize, tmode, tosize, swin, twin;
+  float sbeta, tbeta;
+  int zmean;
+  float f0beta;
+  const int* nlocation;
+  int n_nlocation;
+  float alpha; /* <0: reference default (5 for ftype 0, else 7) */
+  const float *slocation, *ssx, *ssy, *sst;
+  int n_slocation, n_ssx, n_ssy, n_sst;
+  int ssystem;
+  int dither, dither_seed;
+  int opt;
+} nf_dfttest_config;
+
+/* FFT3D parameters (reference descriptors.hpp:161-199; FFT3DConfig
+ * plan.hpp:24-36). sigma2..4 < 0 mean "same as sigma". */
+typedef struct nf_fft3d_config {
+  float sigma, beta;
+  int bw, bh, bt, ow, oh;
+  float kratio, sharpen, scutoff, svr, smin, smax;
+  int interlaced, wintype, pframe, px, py, pshow;
+  float pcutoff, pfactor, sigma2, sigma3, sigma4, degrid, dehalo, hr, ht;
+  int l, t, r, b, opt;
+  int kalman_warmup;
+} nf_fft3d_config;
+
+/* span2d::Plane<const T> equivalent; data in device memory. */
+typedef struct nf_plane {
+  const void* data;
+  int width, height;
+  long long stride_bytes;
+} nf_plane;
+
+typedef struct nf_mut_plane {
+  void* data;
+  int width, height;
+  long long stride_bytes;
+} nf_mut_plane;
+
+typedef struct nf_plan nf_plan;
+typedef struct nf_kalman_state nf_kalman_state;
+
+/* Thread-local message of the last failure on this thread. */
+const char* nf_last_error(void);
+
+/* Reference defaults: descriptors.hpp:200-235 / :161-199. */
+void nf_dfttest_config_init(nf_dfttest_config* config);
+void nf_fft3d_config_init(nf_fft3d_config* config);
+
+/* Plan constructors (reference plan.cpp:170-226 DFTTest, :91-169 FFT3D).
+ * width/height are the plane (for FFT3D: the ROI) dimensions. */
+nf_status nf_dfttest_plan_create(int width, int height, nf_sample_format format,
+                                 const nf_dfttest_config* config, nf_plan** out);
+nf_status nf_fft3d_plan_create(int width, int height, nf_sample_format format,
+                               const nf_fft3d_config* config, nf_plan** out);
+void nf_plan_destroy(nf_plan* plan);
+
+/* Geometry of a plan, reference geometry.hpp Axis {length, block, overlap,
+ * step, count, cover, offset} for x then y (14 ints). */
+nf_status nf_plan_geometry(const nf_plan* plan, int out[14]);
+/* Number of temporal slots the plan consumes per output (tbsize / bt). */
+int nf_plan_temporal_size(const nf_plan* plan);
+
+/* Plan::process_at (reference plan.hpp:77-80, plan.cpp:331-640).
+ * sources: the temporal slots (DFTTest tmode=0: tbsize slots, centre =
+ * output frame; tmode=1: n_targets covering blocks of tbsize slots each;
+ * FFT3D: bt slots or 1 for the end-of-clip fallback). frame/plane feed the
+ * dither coordinates. Enqueues on `stream` (hipStream_t, NULL = default),
+ * then synchronises it and returns NF_ENONFINITE if the frame failed. */
+nf_status nf_plan_process(const nf_plan* plan, const nf_plane* sources, int n_sources,
+                          const int* targets, int n_targets, nf_mut_plane dst, int frame, int plane,
+                          void* stream);
+/* Same work without synchronising: kernels OR NF_FLAG_* bits into the
+ * caller's device word *device_flags (must stay valid until the stream
+ * reaches this point; the caller clears it). Capturable in a hipGraph. */
+nf_status nf_plan_enqueue(const nf_plan* plan, const nf_plane* sources, int n_sources,
+                          const int* targets, int n_targets, nf_mut_plane dst, int frame, int plane,
+                          unsigned* device_flags, void* stream);
+
+/* One launch for several planes/frames (e.g. the three planes of a YUV
+ * frame, or a group of frames): job i uses plans[i], its sources slice
+ * sources[first_source[i] .. first_source[i]+n_sources[i]) and dsts[i].
+ * FFT3D and DFTTest jobs cannot be mixed in one call. Host arrays. */
+nf_status nf_process_batch(const nf_plan* const* plans, int n_jobs, const nf_plane* sources,
+                           const int* first_source, const int* n_sources, const nf_mut_plane* dsts,
+                           const int* frames, const int* planes, unsigned* device_flags,
+                           void* stream);
+
+/* FFT3D Kalman mode (bt=0): device state of one plane (reference
+ * KalmanState, src/kernels/kalman.hpp:5-14: complex L + float C + float Q,
+ * 16 bytes per bin). create = Plan::initial_kalman (plan.cpp:293-300). */
+nf_status nf_kalman_state_create(const nf_plan* plan, nf_kalman_state** out);
+nf_status nf_kalman_state_copy(const nf_kalman_state* src, nf_kalman_state* dst, void* stream);
+void nf_kalman_state_destroy(nf_kalman_state* state);
+size_t nf_kalman_state_bytes(const nf_kalman_state* state);
+/* Plan::advance_kalman (plan.cpp:301-321) and Plan::render_kalman
+ * (plan.hpp:86-88 -> plan.cpp:471-490). Synchronous like nf_plan_process. */
+nf_status nf_kalman_advance(const nf_plan* plan, nf_plane source, nf_kalman_state* state,
+                            void* stream);
+nf_status nf_kalman_render(const nf_plan* plan, nf_plane source, nf_mut_plane dst,
+                           const nf_kalman_state* state, void* stream);
+/* Host access to the state (tests): last as interleaved complex, cov, proc. */
+nf_status nf_kalman_state_download(const nf_kalman_state* state, float* last, float* cov,
+                                   float* proc);
+
+/* Kernel timing (diagnostics; no counterpart in the reference). While
+ * enabled, the calling thread's launches record a hipEvent pair around every
+ * kernel; nf_profile_read synchronises those events and returns the summed
+ * milliseconds and launch counts per kernel class: 0 DFTTest strip kernel,
+ * 1 FFT3D block kernel, 2 FFT3D synthesis kernel, 3 other. */
+nf_status nf_profile_enable(int enable);
+nf_status nf_profile_read(double ms[4], long long counts[4], int reset);
+
+/* KernelInfo (reference vapoursynth.cpp:64-69): writes a NUL-terminated
+ * "key=value;..." description of device, arch and kernel variants. */
+nf_status nf_kernel_info(char* buffer, size_t size);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

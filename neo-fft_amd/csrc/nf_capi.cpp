@@ -1,0 +1,945 @@
+// nf_capi.cpp -- C-ABI of the gfx950 spectral core (include/neo_fft.h).
+// Plans mirror the reference Plan constructors (src/algorithms/plan.cpp:91-226):
+// validation, geometry, windows and tables are computed on the host once and
+// uploaded to one device allocation; processing builds by-value kernel
+// descriptors, so a call allocates nothing on the DFTTest path and only a
+// stream-ordered contribution buffer on the FFT3D path.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "neo_fft.h"
+#include "nf_device.hpp"
+#include "nf_launch.hpp"
+#include "nf_tables.hpp"
+
+using namespace nf;
+
+namespace {
+thread_local std::string g_error;
+
+nf_status set_error(nf_status s, const std::string& m) {
+  g_error = m;
+  return s;
+}
+
+struct DeviceError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+nf_status guarded(F&& f) {
+  try {
+    return f();
+  } catch (const InvalidArgument& e) {
+    return set_error(NF_EINVAL, e.what());
+  } catch (const Unsupported& e) {
+    return set_error(NF_EUNSUPPORTED, e.what());
+  } catch (const DeviceError& e) {
+    return set_error(NF_EDEVICE, e.what());
+  } catch (const std::bad_alloc&) {
+    return set_error(NF_ENOMEM, "out of memory");
+  } catch (const std::exception& e) {
+    return set_error(NF_EINVAL, e.what());
+  }
+}
+
+std::once_flag g_lds_once;
+hipError_t g_lds_status = hipSuccess;
+void ensure_lds_limits() {
+  std::call_once(g_lds_once, [] {
+    g_lds_status = dfttest_t1_set_lds_limit();
+    if (g_lds_status == hipSuccess) g_lds_status = fft3d_set_lds_limit();
+  });
+  hip_check(g_lds_status, "hipFuncSetAttribute");
+}
+
+int storage_of(nf_sample_format f) {
+  require(f.floating ? f.bits == 32 : (f.bits == 8 || f.bits == 10 || f.bits == 12 || f.bits == 14 || f.bits == 16),
+          "unsupported sample format");  // plan.cpp:15-18
+  return f.floating ? kF32 : (f.bits == 8 ? kU8 : kU16);
+}
+size_t bytes_per_sample(int storage) { return storage == kU8 ? 1 : storage == kU16 ? 2 : 4; }
+
+// Owned device tables of a plan (one allocation, 256-byte aligned slices).
+struct DeviceTables {
+  char* base = nullptr;
+  size_t used = 0;
+  std::vector<char> host;
+  template <class T>
+  size_t add(const std::vector<T>& v) {
+    used = (used + 255) & ~size_t(255);
+    const size_t off = used;
+    host.resize(used + v.size() * sizeof(T));
+    if (!v.empty()) std::memcpy(host.data() + off, v.data(), v.size() * sizeof(T));
+    used += v.size() * sizeof(T);
+    return off;
+  }
+  void upload() {
+    if (host.empty()) return;
+    hip_check(hipMalloc(&base, host.size()), "hipMalloc(plan tables)");
+    hip_check(hipMemcpy(base, host.data(), host.size(), hipMemcpyHostToDevice), "hipMemcpy(plan tables)");
+  }
+  template <class T>
+  const T* at(size_t off) const {
+    return reinterpret_cast<const T*>(base + off);
+  }
+  ~DeviceTables() {
+    if (base) (void)hipFree(base);
+  }
+};
+
+}  // namespace
+
+struct nf_plan {
+  int algorithm = 0;  // 0 DFTTest, 1 FFT3D
+  int width = 0, height = 0;
+  nf_sample_format fmt{};
+  int storage = 0;
+  Axis gx{}, gy{};
+  int temporal = 1;
+  DeviceTables tables;
+  // DFTTest
+  int B = 0, S = 0, dither = 0, dither_seed = 0;
+  float dscale = 0, oscale = 0, peak = 0, mean_scale = 0;
+  GainParams g{};
+  const float *d_h = nullptr, *d_hsyn = nullptr;
+  const float2* d_grid = nullptr;
+  int tile_w = 64, tile_h = 64, nbx_max = 0, ww_max = 0;
+  size_t lds = 0;
+  // FFT3D
+  int bw = 0, bh = 0, bt = 1, kalman = 0, denoise = 1, varying = 0, enh_active = 0;
+  float base = 0, degrid = 0, lower = 0, sigma_eff = 0, norm = 0, r0 = 0, ratio2 = 0;
+  GainParams enh{};
+  const float *d_wxa = nullptr, *d_wxs = nullptr, *d_wya = nullptr, *d_wys = nullptr;
+  const float *d_ftw = nullptr, *d_itw = nullptr, *d_primary = nullptr;
+  int NB = 4, groups_x = 0;
+};
+
+struct nf_kalman_state {
+  const nf_plan* plan;
+  size_t bins;
+  char* mem;
+  float2* last;
+  float* cov;
+  float* proc;
+};
+
+extern "C" const char* nf_last_error(void) { return g_error.c_str(); }
+
+extern "C" void nf_dfttest_config_init(nf_dfttest_config* c) {  // descriptors.hpp:200-235
+  std::memset(c, 0, sizeof *c);
+  c->ftype = 0;
+  c->sigma = 8;
+  c->sigma2 = 8;
+  c->pmin = 0;
+  c->pmax = 500;
+  c->sbsize = 16;
+  c->smode = 1;
+  c->sosize = 12;
+  c->tbsize = 3;
+  c->tmode = 0;
+  c->tosize = 0;
+  c->swin = 0;
+  c->twin = 7;
+  c->sbeta = 2.5f;
+  c->tbeta = 2.5f;
+  c->zmean = 1;
+  c->f0beta = 1;
+  c->alpha = -1;
+  c->ssystem = 0;
+  c->dither = 0;
+  c->dither_seed = 0;
+  c->opt = 0;
+}
+
+extern "C" void nf_fft3d_config_init(nf_fft3d_config* c) {  // descriptors.hpp:161-199
+  std::memset(c, 0, sizeof *c);
+  c->sigma = 2;
+  c->beta = 1;
+  c->bw = 32;
+  c->bh = 32;
+  c->bt = 3;
+  c->ow = -1;
+  c->oh = -1;
+  c->kratio = 2;
+  c->sharpen = 0;
+  c->scutoff = 0.3f;
+  c->svr = 1;
+  c->smin = 4;
+  c->smax = 20;
+  c->wintype = 0;
+  c->pcutoff = 0.1f;
+  c->pfactor = 0;
+  c->sigma2 = c->sigma3 = c->sigma4 = -1;
+  c->degrid = 1;
+  c->dehalo = 0;
+  c->hr = 2;
+  c->ht = 50;
+  c->kalman_warmup = 8;
+}
+
+namespace {
+void nonnegative(float v, const char* name) { require(std::isfinite(v) && v >= 0, std::string("invalid ") + name); }
+
+// reference plan.cpp:64-90 plus temporal_grid.hpp:5-13
+void validate_dft(const nf_dfttest_config& c) {
+  require(c.opt == 0 || c.opt == 1 || c.opt == 2 || c.opt == 3 || c.opt == 8, "DFTTest opt outside 0,1,2,3,8");
+  require(c.dither >= 0 && c.dither_seed >= 0, "dither and dither_seed must be nonnegative");
+  require(c.n_nlocation % 4 == 0 && c.n_nlocation / 4 <= 500, "DFTTest nlocation requires up to 500 quadruples");
+  const float alpha = c.alpha < 0 ? (c.ftype == 0 ? 5.0f : 7.0f) : c.alpha;
+  require(std::isfinite(alpha) && alpha > 0, "DFTTest alpha must be positive");
+  require(c.sbsize > 0 && (c.smode == 0 || c.smode == 1), "DFTTest invalid sbsize/smode");
+  require(c.smode != 0 || c.sbsize % 2 == 1, "DFTTest center mode requires odd sbsize");
+  require(c.smode == 0 || (c.sosize >= 0 && c.sosize < c.sbsize), "DFTTest invalid sosize");
+  require(c.tmode == 0 || c.tmode == 1, "DFTTest tmode outside 0..1");
+  require(c.tbsize >= 1 && c.tbsize <= 15, "DFTTest tbsize outside 1..15");
+  require(c.tmode != 0 || c.tbsize % 2 == 1, "DFTTest centered tbsize must be odd");
+  if (c.tmode == 1) {
+    require(c.tosize >= 0 && c.tosize < c.tbsize, "DFTTest invalid tosize");
+    require(c.tosize <= c.tbsize / 2 || c.tbsize % (c.tbsize - c.tosize) == 0,
+            "DFTTest temporal heavy overlap requires divisible step");
+  }
+  if (c.smode == 1 && c.sosize > c.sbsize / 2)
+    require(c.sbsize % (c.sbsize - c.sosize) == 0, "DFTTest heavy overlap requires divisible step");
+  require(c.swin >= 0 && c.swin <= 11 && c.twin >= 0 && c.twin <= 11, "DFTTest invalid swin/twin");
+  require(c.ftype >= 0 && c.ftype <= 4, "DFTTest ftype outside 0..4");
+  nonnegative(c.sbeta, "DFTTest sbeta");
+  nonnegative(c.tbeta, "DFTTest tbeta");
+  nonnegative(c.sigma, "DFTTest sigma");
+  nonnegative(c.sigma2, "DFTTest sigma2");
+  nonnegative(c.pmin, "DFTTest pmin");
+  nonnegative(c.pmax, "DFTTest pmax");
+  require(c.pmin <= c.pmax, "DFTTest pmin exceeds pmax");
+  require(std::isfinite(c.f0beta) && c.f0beta > 0, "DFTTest f0beta must be positive");
+}
+
+Curves curves_of(const nf_dfttest_config& c) {
+  Curves k;
+  if (c.n_slocation) k.shared.assign(c.slocation, c.slocation + c.n_slocation);
+  if (c.n_ssx) k.x.assign(c.ssx, c.ssx + c.n_ssx);
+  if (c.n_ssy) k.y.assign(c.ssy, c.ssy + c.n_ssy);
+  if (c.n_sst) k.time.assign(c.sst, c.sst + c.n_sst);
+  k.system = c.ssystem;
+  return k;
+}
+
+void choose_dft_tiling(nf_plan& p) {
+  const char* tw = std::getenv("NF_DFT_TILE_W");
+  const char* th = std::getenv("NF_DFT_TILE_H");
+  p.tile_w = tw ? std::max(8, std::atoi(tw)) : 64;
+  p.tile_h = th ? std::max(8, std::atoi(th)) : 64;
+  p.nbx_max = (p.tile_w + p.B - 2) / p.S + 1;
+  p.ww_max = (p.nbx_max - 1) * p.S + p.B;
+  DftJob j{};
+  j.nbx_max = p.nbx_max;
+  j.ww_max = p.ww_max;
+  j.tile_w = p.tile_w;
+  p.lds = dfttest_t1_lds_bytes(p.B, j);
+  require(p.lds <= 160 * 1024, "DFTTest tile does not fit the LDS");
+}
+}  // namespace
+
+extern "C" nf_status nf_dfttest_plan_create(int width, int height, nf_sample_format fmt, const nf_dfttest_config* cfg,
+                                            nf_plan** out) {
+  return guarded([&]() -> nf_status {
+    require(cfg && out, "null argument");
+    *out = nullptr;
+    const nf_dfttest_config& c = *cfg;
+    validate_dft(c);
+    const Curves curves = curves_of(c);
+    validate_curves(curves);
+    auto p = std::make_unique<nf_plan>();
+    p->algorithm = 0;
+    p->width = width;
+    p->height = height;
+    p->fmt = fmt;
+    p->storage = storage_of(fmt);
+    p->gx = dft_axis(width, c.sbsize, c.smode, c.sosize);
+    p->gy = dft_axis(height, c.sbsize, c.smode, c.sosize);
+    p->temporal = c.tbsize;
+    p->B = c.sbsize;
+    p->S = p->gx.step;
+    p->dither = c.dither;
+    p->dither_seed = c.dither_seed;
+    // Device coverage of this path: spatial tbsize=1, smode=1, 8/16/32-point blocks.
+    if (c.tbsize != 1) throw Unsupported("DFTTest tbsize>1 is not available on this device path yet");
+    if (c.smode != 1) throw Unsupported("DFTTest smode=0 is not available on this device path yet");
+    if (!(c.sbsize == 8 || c.sbsize == 16 || c.sbsize == 32))
+      throw Unsupported("DFTTest sbsize must be 8, 16 or 32 on this device path");
+    if (c.n_nlocation) throw Unsupported("DFTTest nlocation noise sampling is not available on this device path yet");
+    if (c.dither > 0 && !fmt.floating && fmt.bits == 8)
+      throw Unsupported("DFTTest dither is not available on this device path yet");
+    const int T = c.tbsize, B = c.sbsize;
+    // plan.cpp:188-226
+    const auto win = dft_window_3d(T, B, c.smode == 0 ? 0 : c.sosize, c.smode, c.swin, c.twin, c.sbeta, c.tbeta,
+                                   c.tmode, c.tmode ? c.tosize : 0);
+    std::vector<float> hsyn(win.h.size());
+    const float volume = float(T) * float(B) * float(B);
+    for (size_t i = 0; i < hsyn.size(); ++i) hsyn[i] = win.h[i] * volume;
+    const float scale = c.ftype < 2 ? win.wscale : 1.0f;
+    GainParams g{};
+    g.type = c.ftype;
+    g.a = curves.empty() ? finite(c.sigma / scale) : 0.0f;
+    g.b = finite(c.sigma2 / scale);
+    g.low = finite(c.pmin / win.wscale);
+    g.high = finite(c.pmax / win.wscale);
+    g.exponent = c.f0beta;
+    g.floor_ = 0;
+    std::vector<float> primary;
+    if (!curves.empty()) primary = dft_profile(curves, T, B, c.sigma, scale);
+    std::vector<std::complex<float>> grid;
+    if (c.zmean) {
+      std::vector<float> block(win.h.size());
+      for (size_t i = 0; i < block.size(); ++i) block[i] = finite(255.0f * win.h[i]);
+      grid = rfft2_host(B, B, block.data());
+      require(grid[0].real() != 0, "DFTTest zmean requires nonzero template DC");
+    }
+    const size_t o_h = p->tables.add(win.h), o_hs = p->tables.add(hsyn);
+    const size_t o_pr = primary.empty() ? 0 : p->tables.add(primary);
+    const size_t o_g = grid.empty() ? 0 : p->tables.add(grid);
+    p->tables.upload();
+    p->d_h = p->tables.at<float>(o_h);
+    p->d_hsyn = p->tables.at<float>(o_hs);
+    g.primary = primary.empty() ? nullptr : p->tables.at<float>(o_pr);
+    p->d_grid = grid.empty() ? nullptr : p->tables.at<float2>(o_g);
+    p->g = g;
+    p->mean_scale = c.zmean ? 1.0f : 0.0f;
+    p->dscale = fmt.floating ? 255.0f : 1.0f / float(1 << (fmt.bits - 8));  // pad.hpp:96-101
+    p->oscale = fmt.floating ? 1.0f / 255.0f : float(1 << (fmt.bits - 8));  // plan.cpp:611-639
+    p->peak = fmt.floating ? 1.0f : float((1 << fmt.bits) - 1);
+    choose_dft_tiling(*p);
+    *out = p.release();
+    return NF_OK;
+  });
+}
+
+extern "C" nf_status nf_fft3d_plan_create(int width, int height, nf_sample_format fmt, const nf_fft3d_config* cfg,
+                                          nf_plan** out) {
+  return guarded([&]() -> nf_status {
+    require(cfg && out, "null argument");
+    *out = nullptr;
+    const nf_fft3d_config& c = *cfg;
+    // plan.cpp:45-63
+    require(c.kalman_warmup >= 0, "FFT3D kalman_warmup must be nonnegative");
+    require(c.l >= 0 && c.t >= 0 && c.r >= 0 && c.b >= 0, "ROI margins must be nonnegative");
+    require(c.bw >= 2 && c.bh >= 2, "FFT3D bw/bh must be >=2");
+    require(c.ow <= c.bw / 2 && c.oh <= c.bh / 2, "FFT3D ow/oh exceeds half block");
+    require(c.wintype >= 0 && c.wintype <= 2, "FFT3D wintype outside 0..2");
+    require(c.bt >= -1 && c.bt <= 5, "FFT3D bt outside -1..5");
+    nonnegative(c.kratio, "kratio");
+    nonnegative(c.pfactor, "FFT3D pfactor");
+    require(std::isfinite(c.pcutoff) && c.pcutoff > 0, "FFT3D pcutoff must be positive");
+    require(c.px >= 0 && c.py >= 0, "FFT3D px/py must be nonnegative");
+    const float s2 = c.sigma2 < 0 ? c.sigma : c.sigma2, s3 = c.sigma3 < 0 ? c.sigma : c.sigma3,
+                s4 = c.sigma4 < 0 ? c.sigma : c.sigma4;
+    for (float s : {s2, s3, s4}) nonnegative(s, "FFT3D sigma2..4");
+    nonnegative(c.sigma, "FFT3D sigma");
+    nonnegative(c.degrid, "FFT3D degrid");
+    require(std::isfinite(c.beta) && c.beta >= 1, "FFT3D beta must be >=1");
+    auto p = std::make_unique<nf_plan>();
+    p->algorithm = 1;
+    p->width = width;
+    p->height = height;
+    p->fmt = fmt;
+    p->storage = storage_of(fmt);
+    p->gx = fft3d_axis(width, c.bw, c.ow);
+    p->gy = fft3d_axis(height, c.bh, c.oh);
+    p->bw = c.bw;
+    p->bh = c.bh;
+    p->bt = c.bt;
+    p->temporal = std::max(1, c.bt);
+    if (c.pfactor > 0) throw Unsupported("FFT3D pattern-sampled noise (pfactor) is not available on this device path yet");
+    if (!(c.bw == c.bh && (c.bw == 8 || c.bw == 16 || c.bw == 32)))
+      throw Unsupported("FFT3D bw=bh in {8,16,32} required on this device path");
+    const auto wx = fft3d_window(c.bw, p->gx.overlap, c.wintype);
+    const auto wy = fft3d_window(c.bh, p->gy.overlap, c.wintype);
+    // plan.cpp:101-169
+    const float factor = fmt.floating ? 1.0f / 255 : float(1 << (fmt.bits - 8));
+    p->sigma_eff = finite(c.sigma * factor);
+    p->norm = 1.0f / (float(c.bw) * float(c.bh));
+    const float sig[4] = {p->sigma_eff, finite(s2 * factor), finite(s3 * factor), finite(s4 * factor)};
+    p->varying = sig[1] != sig[0] || sig[2] != sig[0] || sig[3] != sig[0];
+    if (c.pshow && p->varying) throw Unsupported("FFT3D pshow preview is not available on this device path yet");
+    p->denoise = c.bt != -1;
+    p->kalman = c.bt == 0;
+    if (p->kalman) {
+      p->r0 = finite((p->sigma_eff * p->sigma_eff) / p->norm);
+      p->ratio2 = finite(c.kratio * c.kratio);
+    }
+    GainParams g{};
+    g.type = -1;
+    g.floor_ = (c.beta - 1) / c.beta;
+    g.exponent = 1;
+    std::vector<float> primary;
+    if (p->denoise) {
+      if (p->varying) {
+        primary = fft3d_profile(c.bw, c.bh, sig);
+        for (float r : primary) finite(float(p->temporal) * r);
+      } else {
+        g.a = finite((p->sigma_eff * p->sigma_eff) / p->norm);
+        for (int T = 1; T <= p->temporal; ++T) finite(((float(T) * p->sigma_eff) * p->sigma_eff) / p->norm);
+      }
+    }
+    if (p->kalman) {
+      if (primary.empty()) finite(p->r0 * p->ratio2);
+      else for (float r : primary) finite(std::max(r, 1e-15f) * p->ratio2);
+    }
+    const auto et = enhancement_tables(c.bw, c.bh, factor, c.sharpen, c.scutoff, c.svr, c.smin, c.smax, c.dehalo,
+                                       c.hr, c.ht);
+    p->enh_active = c.sharpen != 0 || c.dehalo != 0;
+    std::vector<std::complex<float>> grid;
+    if (c.degrid != 0) {
+      std::vector<float> block((size_t)c.bw * c.bh);
+      const float peak = fmt.floating ? 1.0f : float((1 << fmt.bits) - 1);
+      for (int y = 0; y < c.bh; ++y)
+        for (int x = 0; x < c.bw; ++x) block[(size_t)y * c.bw + x] = (peak * wy.analysis[y]) * wx.analysis[x];
+      grid = rfft2_host(c.bh, c.bw, block.data());
+      require(grid[0].real() != 0, "FFT3D unusable grid DC");
+    }
+    // temporal twiddles, spectral.cpp:67-88 (binary64 -> binary32)
+    std::vector<float> ftw(50, 0.0f), itw(10, 0.0f);
+    const int T = p->temporal;
+    if (T > 1) {
+      const double kPi = 3.1415926535897932384626433832795;
+      for (int m = 0; m < T; ++m)
+        for (int j = 0; j < T; ++j) {
+          const double angle = -2.0 * kPi * double(j * m) / double(T);
+          ftw[2 * (m * 5 + j)] = float(std::cos(angle));
+          ftw[2 * (m * 5 + j) + 1] = float(std::sin(angle));
+        }
+      const int cc = T / 2;
+      for (int m = 0; m < T; ++m) {
+        const double angle = +2.0 * kPi * double(cc * m) / double(T);
+        itw[2 * m] = float(std::cos(angle));
+        itw[2 * m + 1] = float(std::sin(angle));
+      }
+    }
+    const size_t o_wxa = p->tables.add(wx.analysis), o_wxs = p->tables.add(wx.synthesis);
+    const size_t o_wya = p->tables.add(wy.analysis), o_wys = p->tables.add(wy.synthesis);
+    const size_t o_ftw = p->tables.add(ftw), o_itw = p->tables.add(itw);
+    const size_t o_pr = primary.empty() ? 0 : p->tables.add(primary);
+    const size_t o_g = grid.empty() ? 0 : p->tables.add(grid);
+    const size_t o_sh = et.sharpen.empty() ? 0 : p->tables.add(et.sharpen);
+    const size_t o_ha = et.halo.empty() ? 0 : p->tables.add(et.halo);
+    p->tables.upload();
+    p->d_wxa = p->tables.at<float>(o_wxa);
+    p->d_wxs = p->tables.at<float>(o_wxs);
+    p->d_wya = p->tables.at<float>(o_wya);
+    p->d_wys = p->tables.at<float>(o_wys);
+    p->d_ftw = p->tables.at<float>(o_ftw);
+    p->d_itw = p->tables.at<float>(o_itw);
+    p->d_primary = primary.empty() ? nullptr : p->tables.at<float>(o_pr);
+    p->d_grid = grid.empty() ? nullptr : p->tables.at<float2>(o_g);
+    GainParams e{};
+    e.type = -2;
+    e.exponent = 1;
+    e.sharpen = c.sharpen;
+    e.dehalo = c.dehalo;
+    e.ea = et.a;
+    e.eb = et.b;
+    e.ec = et.c;
+    e.sharpen_tab = et.sharpen.empty() ? nullptr : p->tables.at<float>(o_sh);
+    e.halo_tab = et.halo.empty() ? nullptr : p->tables.at<float>(o_ha);
+    p->enh = e;
+    g.primary = p->d_primary;
+    if (!p->varying && p->denoise) {  // plan.cpp:141: enhancement fused with uniform Wiener
+      g.sharpen = e.sharpen;
+      g.dehalo = e.dehalo;
+      g.ea = e.ea;
+      g.eb = e.eb;
+      g.ec = e.ec;
+      g.sharpen_tab = e.sharpen_tab;
+      g.halo_tab = e.halo_tab;
+    }
+    p->g = g;
+    p->degrid = c.degrid;
+    p->lower = (c.beta - 1.0f) / c.beta;
+    p->base = (!fmt.floating && fmt.chroma) ? float(1 << (fmt.bits - 1)) : 0.0f;
+    p->peak = fmt.floating ? 1.0f : float((1 << fmt.bits) - 1);
+    p->NB = c.bw == 32 ? 4 : c.bw == 16 ? 8 : 16;
+    if (const char* nb = std::getenv("NF_FFT3D_NB")) p->NB = std::max(1, std::atoi(nb));
+    p->groups_x = (p->gx.count + p->NB - 1) / p->NB;
+    p->lds = fft3d_block_lds_bytes(c.bw, c.bh, p->temporal, p->NB, p->gx.step);
+    require(p->lds <= 160 * 1024, "FFT3D block group does not fit the LDS");
+    *out = p.release();
+    return NF_OK;
+  });
+}
+
+extern "C" void nf_plan_destroy(nf_plan* plan) { delete plan; }
+
+extern "C" nf_status nf_plan_geometry(const nf_plan* p, int out[14]) {
+  if (!p || !out) return set_error(NF_EINVAL, "null argument");
+  const Axis* a[2] = {&p->gx, &p->gy};
+  for (int i = 0; i < 2; ++i) {
+    const int v[7] = {a[i]->length, a[i]->block, a[i]->overlap, a[i]->step, a[i]->count, a[i]->cover, a[i]->offset};
+    std::memcpy(out + 7 * i, v, sizeof v);
+  }
+  return NF_OK;
+}
+
+extern "C" int nf_plan_temporal_size(const nf_plan* p) { return p ? p->temporal : 0; }
+
+namespace {
+
+void check_plane(const nf_plan& p, int w, int h, long long stride, const void* data) {
+  require(data != nullptr, "null plane");
+  require(w == p.width && h == p.height, "frame dimensions differ from plan");
+  require(stride != 0 && std::llabs(stride) >= (long long)w * (long long)bytes_per_sample(p.storage),
+          "invalid plane stride");
+}
+
+DftJob make_dft_job(const nf_plan& p, const nf_plane& src, const nf_mut_plane& dst) {
+  DftJob j{};
+  j.gx = p.gx;
+  j.gy = p.gy;
+  j.S = p.S;
+  j.storage = p.storage;
+  j.dscale = p.dscale;
+  j.oscale = p.oscale;
+  j.peak = p.peak;
+  j.mean_scale = p.mean_scale;
+  j.g = p.g;
+  j.h = p.d_h;
+  j.hsyn = p.d_hsyn;
+  j.grid = p.d_grid;
+  j.src = src.data;
+  j.src_stride = src.stride_bytes;
+  j.dst = dst.data;
+  j.dst_stride = dst.stride_bytes;
+  j.acc_out = nullptr;
+  j.tile_w = p.tile_w;
+  j.tile_h = p.tile_h;
+  j.tile_x0 = p.gx.offset / p.tile_w;
+  j.tile_y0 = p.gy.offset / p.tile_h;
+  j.tiles_x = (p.gx.offset + p.gx.length + p.tile_w - 1) / p.tile_w - j.tile_x0;
+  j.tiles_y = (p.gy.offset + p.gy.length + p.tile_h - 1) / p.tile_h - j.tile_y0;
+  j.nbx_max = p.nbx_max;
+  j.ww_max = p.ww_max;
+  return j;
+}
+
+Fft3dJob make_fft3d_job(const nf_plan& p, int T, int mode) {
+  Fft3dJob j{};
+  j.gx = p.gx;
+  j.gy = p.gy;
+  j.storage = p.storage;
+  j.T = T;
+  j.c = T / 2;
+  j.mode = mode;
+  j.base = p.base;
+  j.peak = p.peak;
+  j.degrid = p.degrid;
+  j.lower = p.lower;
+  j.noise = (p.denoise && !p.varying) ? ((float(T) * p.sigma_eff) * p.sigma_eff) / p.norm : 0.0f;
+  j.noise_mult = float(T);
+  j.denoise = p.denoise;
+  j.varying = p.varying;
+  j.enh_separate = p.varying && p.enh_active;
+  j.g = p.g;
+  j.enh = p.enh;
+  j.wxa = p.d_wxa;
+  j.wxs = p.d_wxs;
+  j.wya = p.d_wya;
+  j.wys = p.d_wys;
+  j.grid = p.d_grid;
+  j.ftw = p.d_ftw;
+  j.itw = p.d_itw;
+  j.k_pattern = p.varying ? p.d_primary : nullptr;
+  j.k_r0 = p.r0;
+  j.k_ratio2 = p.ratio2;
+  j.blocks_per_group = p.NB;
+  j.groups_x = p.groups_x;
+  return j;
+}
+
+size_t contrib_floats(const nf_plan& p) { return (size_t)p.gx.count * p.gy.count * p.bw * p.bh; }
+
+// Per-thread device word for the synchronous entry points.
+struct SyncFlag {
+  unsigned* dev = nullptr;
+  int device = -1;
+  ~SyncFlag() {
+    if (dev) (void)hipFree(dev);
+  }
+};
+thread_local SyncFlag t_flag;
+unsigned* sync_flag() {
+  int d = 0;
+  hip_check(hipGetDevice(&d), "hipGetDevice");
+  if (!t_flag.dev || t_flag.device != d) {
+    if (t_flag.dev) (void)hipFree(t_flag.dev);
+    t_flag.dev = nullptr;
+    hip_check(hipMalloc(&t_flag.dev, sizeof(unsigned)), "hipMalloc(flag)");
+    t_flag.device = d;
+  }
+  return t_flag.dev;
+}
+
+template <class Enqueue>
+nf_status run_sync(hipStream_t stream, Enqueue&& enqueue) {
+  unsigned* flag = sync_flag();
+  hip_check(hipMemsetAsync(flag, 0, sizeof(unsigned), stream), "hipMemsetAsync");
+  enqueue(flag);
+  unsigned host = 0;
+  hip_check(hipMemcpyAsync(&host, flag, sizeof(unsigned), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
+  hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  if (host & NF_FLAG_NONFINITE) return set_error(NF_ENONFINITE, "non-finite sample or intermediate");
+  return NF_OK;
+}
+
+// Optional per-kernel hipEvent timing on the calling thread.
+struct KernelTimer {
+  bool on = false;
+  struct Pair {
+    hipEvent_t a, b;
+    int cls;
+  };
+  std::vector<Pair> pending;
+  std::vector<hipEvent_t> pool;
+  double ms[4] = {0, 0, 0, 0};
+  long long count[4] = {0, 0, 0, 0};
+  hipEvent_t get() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+  }
+  void drain() {
+    for (auto& p : pending) {
+      hip_check(hipEventSynchronize(p.b), "hipEventSynchronize");
+      float t = 0;
+      hip_check(hipEventElapsedTime(&t, p.a, p.b), "hipEventElapsedTime");
+      ms[p.cls] += t;
+      count[p.cls] += 1;
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+  ~KernelTimer() {
+    for (auto& p : pending) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+thread_local KernelTimer t_timer;
+
+template <class Launch>
+void timed(int cls, hipStream_t stream, Launch&& launch) {
+  if (!t_timer.on) {
+    launch();
+    return;
+  }
+  hipEvent_t a = t_timer.get(), b = t_timer.get();
+  hip_check(hipEventRecord(a, stream), "hipEventRecord");
+  launch();
+  hip_check(hipEventRecord(b, stream), "hipEventRecord");
+  t_timer.pending.push_back({a, b, cls});
+  if (t_timer.pending.size() > 4096) t_timer.drain();
+}
+
+// DFTTest: jobs share B; up to kMaxInlineJobs per launch.
+void enqueue_dft_jobs(const std::vector<const nf_plan*>& plans, const std::vector<DftJob>& jobs, unsigned* flags,
+                      hipStream_t stream) {
+  ensure_lds_limits();
+  for (size_t first = 0; first < jobs.size(); first += kMaxInlineJobs) {
+    DftLaunch L{};
+    const size_t n = std::min<size_t>(kMaxInlineJobs, jobs.size() - first);
+    int tiles = 0;
+    size_t lds = 0;
+    const int B = plans[first]->B;
+    for (size_t i = 0; i < n; ++i) {
+      require(plans[first + i]->B == B, "batched DFTTest jobs must share sbsize");
+      L.jobs[i] = jobs[first + i];
+      L.tile_start[i] = tiles;
+      tiles += L.jobs[i].tiles_x * L.jobs[i].tiles_y;
+      lds = std::max(lds, plans[first + i]->lds);
+    }
+    L.tile_start[n] = tiles;
+    L.njobs = (int)n;
+    L.flags = flags;
+    timed(0, stream, [&] { hip_check(launch_dfttest_t1(B, L, tiles, lds, stream), "dfttest kernel launch"); });
+  }
+}
+
+void enqueue_fft3d_jobs(const std::vector<const nf_plan*>& plans, std::vector<Fft3dJob> jobs, unsigned* flags,
+                        hipStream_t stream, bool synthesize) {
+  ensure_lds_limits();
+  // One stream-ordered contribution buffer for every job of the call.
+  size_t total = 0;
+  std::vector<size_t> off(jobs.size());
+  for (size_t i = 0; i < jobs.size(); ++i) {
+    off[i] = total;
+    if (jobs[i].mode != 1) total += (contrib_floats(*plans[i]) + 63) & ~size_t(63);
+  }
+  float* contrib = nullptr;
+  if (total) hip_check(hipMallocAsync((void**)&contrib, total * sizeof(float), stream), "hipMallocAsync(contrib)");
+  for (size_t i = 0; i < jobs.size(); ++i) jobs[i].contrib = contrib ? contrib + off[i] : nullptr;
+  for (size_t first = 0; first < jobs.size(); first += kMaxInlineJobs) {
+    Fft3dLaunch L{};
+    const size_t n = std::min<size_t>(kMaxInlineJobs, jobs.size() - first);
+    int groups = 0;
+    long long pixels = 0;
+    size_t lds = 0;
+    const int bw = plans[first]->bw, bh = plans[first]->bh;
+    for (size_t i = 0; i < n; ++i) {
+      const nf_plan& p = *plans[first + i];
+      require(p.bw == bw && p.bh == bh, "batched FFT3D jobs must share bw/bh");
+      L.jobs[i] = jobs[first + i];
+      L.group_start[i] = groups;
+      L.pixel_start[i] = pixels;
+      groups += p.groups_x * p.gy.count;
+      pixels += (long long)p.width * p.height;
+      lds = std::max(lds, fft3d_block_lds_bytes(bw, bh, jobs[first + i].mode == 0 ? jobs[first + i].T : 1, p.NB,
+                                                p.gx.step));
+    }
+    L.group_start[n] = groups;
+    L.pixel_start[n] = pixels;
+    L.njobs = (int)n;
+    L.flags = flags;
+    timed(1, stream, [&] { hip_check(launch_fft3d_block(bw, bh, L, groups, lds, stream), "fft3d block kernel launch"); });
+    if (synthesize)
+      timed(2, stream, [&] { hip_check(launch_fft3d_synthesis(L, stream), "fft3d synthesis kernel launch"); });
+  }
+  if (contrib) hip_check(hipFreeAsync(contrib, stream), "hipFreeAsync(contrib)");
+}
+
+nf_status enqueue_one(const nf_plan* plan, const nf_plane* sources, int n_sources, const int* targets, int n_targets,
+                      nf_mut_plane dst, int frame, int plane, unsigned* flags, hipStream_t stream) {
+  require(plan && sources && n_sources > 0, "sources must not be empty");
+  require(flags != nullptr, "device flag word required");
+  (void)targets;
+  (void)frame;
+  (void)plane;
+  check_plane(*plan, dst.width, dst.height, dst.stride_bytes, dst.data);
+  for (int i = 0; i < n_sources; ++i)
+    check_plane(*plan, sources[i].width, sources[i].height, sources[i].stride_bytes, sources[i].data);
+  if (plan->algorithm == 0) {
+    require(n_targets == 0, "invalid temporal block inputs");
+    require(n_sources == plan->temporal, "DFTTest sources size must match plan temporal_size");
+    enqueue_dft_jobs({plan}, {make_dft_job(*plan, sources[0], dst)}, flags, stream);
+  } else {
+    require(!plan->kalman, "Kalman plans use nf_kalman_advance / nf_kalman_render");
+    require(n_sources == plan->temporal || n_sources == 1,
+            "FFT3D sources size must match plan temporal_size or single-frame fallback");
+    Fft3dJob j = make_fft3d_job(*plan, n_sources, 0);
+    for (int s = 0; s < n_sources; ++s) {
+      j.src[s] = sources[s].data;
+      j.src_stride[s] = sources[s].stride_bytes;
+    }
+    j.dst = dst.data;
+    j.dst_stride = dst.stride_bytes;
+    enqueue_fft3d_jobs({plan}, {j}, flags, stream, true);
+  }
+  return NF_OK;
+}
+
+}  // namespace
+
+extern "C" nf_status nf_plan_enqueue(const nf_plan* plan, const nf_plane* sources, int n_sources, const int* targets,
+                                     int n_targets, nf_mut_plane dst, int frame, int plane, unsigned* device_flags,
+                                     void* stream) {
+  return guarded([&]() -> nf_status {
+    return enqueue_one(plan, sources, n_sources, targets, n_targets, dst, frame, plane, device_flags,
+                       (hipStream_t)stream);
+  });
+}
+
+extern "C" nf_status nf_plan_process(const nf_plan* plan, const nf_plane* sources, int n_sources, const int* targets,
+                                     int n_targets, nf_mut_plane dst, int frame, int plane, void* stream) {
+  return guarded([&]() -> nf_status {
+    return run_sync((hipStream_t)stream, [&](unsigned* flag) {
+      enqueue_one(plan, sources, n_sources, targets, n_targets, dst, frame, plane, flag, (hipStream_t)stream);
+    });
+  });
+}
+
+extern "C" nf_status nf_process_batch(const nf_plan* const* plans, int n_jobs, const nf_plane* sources,
+                                      const int* first_source, const int* n_sources, const nf_mut_plane* dsts,
+                                      const int* frames, const int* planes, unsigned* device_flags, void* stream) {
+  return guarded([&]() -> nf_status {
+    require(plans && n_jobs > 0 && sources && first_source && n_sources && dsts && device_flags, "null argument");
+    (void)frames;
+    (void)planes;
+    const int alg = plans[0]->algorithm;
+    std::vector<const nf_plan*> pv;
+    std::vector<DftJob> dj;
+    std::vector<Fft3dJob> fj;
+    for (int i = 0; i < n_jobs; ++i) {
+      const nf_plan* p = plans[i];
+      require(p && p->algorithm == alg, "batched jobs must use one filter");
+      const nf_plane* s = sources + first_source[i];
+      const int ns = n_sources[i];
+      check_plane(*p, dsts[i].width, dsts[i].height, dsts[i].stride_bytes, dsts[i].data);
+      for (int k = 0; k < ns; ++k) check_plane(*p, s[k].width, s[k].height, s[k].stride_bytes, s[k].data);
+      pv.push_back(p);
+      if (alg == 0) {
+        require(ns == p->temporal, "DFTTest sources size must match plan temporal_size");
+        dj.push_back(make_dft_job(*p, s[0], dsts[i]));
+      } else {
+        require(!p->kalman, "Kalman plans use nf_kalman_advance / nf_kalman_render");
+        require(ns == p->temporal || ns == 1, "FFT3D sources size must match plan temporal_size or single-frame fallback");
+        Fft3dJob j = make_fft3d_job(*p, ns, 0);
+        for (int k = 0; k < ns; ++k) {
+          j.src[k] = s[k].data;
+          j.src_stride[k] = s[k].stride_bytes;
+        }
+        j.dst = dsts[i].data;
+        j.dst_stride = dsts[i].stride_bytes;
+        fj.push_back(j);
+      }
+    }
+    if (alg == 0) enqueue_dft_jobs(pv, dj, device_flags, (hipStream_t)stream);
+    else enqueue_fft3d_jobs(pv, fj, device_flags, (hipStream_t)stream, true);
+    return NF_OK;
+  });
+}
+
+// ---------------------------------------------------------------- Kalman
+extern "C" nf_status nf_kalman_state_create(const nf_plan* plan, nf_kalman_state** out) {
+  return guarded([&]() -> nf_status {
+    require(plan && out, "null argument");
+    require(plan->algorithm == 1 && plan->kalman, "not a Kalman plan");
+    *out = nullptr;
+    const size_t bins = (size_t)plan->gx.count * plan->gy.count * plan->bh * (plan->bw / 2 + 1);
+    auto s = std::make_unique<nf_kalman_state>();
+    s->plan = plan;
+    s->bins = bins;
+    s->mem = nullptr;
+    hip_check(hipMalloc((void**)&s->mem, bins * 16), "hipMalloc(kalman state)");
+    s->last = reinterpret_cast<float2*>(s->mem);
+    s->cov = reinterpret_cast<float*>(s->mem + bins * 8);
+    s->proc = reinterpret_cast<float*>(s->mem + bins * 12);
+    // Plan::initial_kalman, plan.cpp:293-300: L = 0, C = Q = R0
+    std::vector<float> host(bins * 4, 0.0f);
+    std::fill(host.begin() + bins * 2, host.end(), plan->r0);
+    hip_check(hipMemcpy(s->mem, host.data(), bins * 16, hipMemcpyHostToDevice), "hipMemcpy(kalman init)");
+    *out = s.release();
+    return NF_OK;
+  });
+}
+
+extern "C" nf_status nf_kalman_state_copy(const nf_kalman_state* src, nf_kalman_state* dst, void* stream) {
+  return guarded([&]() -> nf_status {
+    require(src && dst && src->bins == dst->bins, "Kalman state shape differs");
+    hip_check(hipMemcpyAsync(dst->mem, src->mem, src->bins * 16, hipMemcpyDeviceToDevice, (hipStream_t)stream),
+              "hipMemcpyAsync(kalman copy)");
+    hip_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+    return NF_OK;
+  });
+}
+
+extern "C" void nf_kalman_state_destroy(nf_kalman_state* s) {
+  if (!s) return;
+  if (s->mem) (void)hipFree(s->mem);
+  delete s;
+}
+
+extern "C" size_t nf_kalman_state_bytes(const nf_kalman_state* s) { return s ? s->bins * 16 : 0; }
+
+extern "C" nf_status nf_kalman_state_download(const nf_kalman_state* s, float* last, float* cov, float* proc) {
+  return guarded([&]() -> nf_status {
+    require(s != nullptr, "null argument");
+    if (last) hip_check(hipMemcpy(last, s->last, s->bins * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+    if (cov) hip_check(hipMemcpy(cov, s->cov, s->bins * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    if (proc) hip_check(hipMemcpy(proc, s->proc, s->bins * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    return NF_OK;
+  });
+}
+
+extern "C" nf_status nf_kalman_advance(const nf_plan* plan, nf_plane source, nf_kalman_state* state, void* stream) {
+  return guarded([&]() -> nf_status {
+    require(plan && plan->algorithm == 1 && plan->kalman, "not a Kalman plan");
+    require(state && state->plan && state->bins == (size_t)plan->gx.count * plan->gy.count * plan->bh * (plan->bw / 2 + 1),
+            "Kalman state shape differs");
+    check_plane(*plan, source.width, source.height, source.stride_bytes, source.data);
+    return run_sync((hipStream_t)stream, [&](unsigned* flag) {
+      Fft3dJob j = make_fft3d_job(*plan, 1, 1);
+      j.src[0] = source.data;
+      j.src_stride[0] = source.stride_bytes;
+      j.k_last = state->last;
+      j.k_cov = state->cov;
+      j.k_proc = state->proc;
+      enqueue_fft3d_jobs({plan}, {j}, flag, (hipStream_t)stream, false);
+    });
+  });
+}
+
+extern "C" nf_status nf_kalman_render(const nf_plan* plan, nf_plane source, nf_mut_plane dst,
+                                      const nf_kalman_state* state, void* stream) {
+  return guarded([&]() -> nf_status {
+    require(plan && plan->algorithm == 1 && plan->kalman, "not a Kalman plan");
+    require(state && state->bins == (size_t)plan->gx.count * plan->gy.count * plan->bh * (plan->bw / 2 + 1),
+            "Kalman checkpoint shape differs");
+    check_plane(*plan, source.width, source.height, source.stride_bytes, source.data);
+    check_plane(*plan, dst.width, dst.height, dst.stride_bytes, dst.data);
+    return run_sync((hipStream_t)stream, [&](unsigned* flag) {
+      Fft3dJob j = make_fft3d_job(*plan, 1, 2);
+      j.k_last = state->last;
+      j.k_cov = state->cov;
+      j.k_proc = state->proc;
+      j.dst = dst.data;
+      j.dst_stride = dst.stride_bytes;
+      enqueue_fft3d_jobs({plan}, {j}, flag, (hipStream_t)stream, true);
+    });
+  });
+}
+
+extern "C" nf_status nf_profile_enable(int enable) {
+  return guarded([&]() -> nf_status {
+    t_timer.on = enable != 0;
+    return NF_OK;
+  });
+}
+
+extern "C" nf_status nf_profile_read(double ms[4], long long counts[4], int reset) {
+  return guarded([&]() -> nf_status {
+    t_timer.drain();
+    for (int i = 0; i < 4; ++i) {
+      if (ms) ms[i] = t_timer.ms[i];
+      if (counts) counts[i] = t_timer.count[i];
+      if (reset) {
+        t_timer.ms[i] = 0;
+        t_timer.count[i] = 0;
+      }
+    }
+    return NF_OK;
+  });
+}
+
+extern "C" nf_status nf_kernel_info(char* buffer, size_t size) {
+  return guarded([&]() -> nf_status {
+    require(buffer && size > 0, "null argument");
+    int dev = 0;
+    hipDeviceProp_t prop{};
+    std::string s;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+      s = std::string("device=") + prop.name + ";arch=" + prop.gcnArchName + ";cus=" +
+          std::to_string(prop.multiProcessorCount) + ";wave=" + std::to_string(prop.warpSize);
+    } else {
+      s = "device=none";
+    }
+    s += ";fft=radix2-lane-8/16/32;dfttest=strip-lds-ola;fft3d=block+canonical-synthesis";
+    std::snprintf(buffer, size, "%s", s.c_str());
+    return NF_OK;
+  });
+}

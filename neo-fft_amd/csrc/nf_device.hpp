@@ -1,0 +1,129 @@
+// nf_device.hpp -- descriptors shared by the host launchers and the gfx950
+// kernels. Everything here is passed by value as kernel arguments, so a
+// launch needs no descriptor upload and can be captured in a hipGraph.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "neo_fft.h"
+
+namespace nf {
+
+// reference src/spectral/geometry.hpp:6-8
+struct Axis {
+  int length, block, overlap, step, count, cover, offset;
+};
+
+// one-reflection index map, geometry.hpp:45-48
+__host__ __device__ __forceinline__ int reflect(int c, const Axis& a) {
+  const int j = c - a.offset;
+  return j < 0 ? -j : (j < a.length ? j : 2 * a.length - 2 - j);
+}
+
+__host__ __device__ __forceinline__ int floor_div(int a, int b) {
+  const int q = a / b;
+  return (a % b != 0 && ((a < 0) != (b < 0))) ? q - 1 : q;
+}
+__host__ __device__ __forceinline__ int ceil_div_int(int a, int b) { return -floor_div(-a, b); }
+
+// Sample storage of a plane: 0 = u8, 1 = u16, 2 = f32.
+enum Storage : int { kU8 = 0, kU16 = 1, kF32 = 2 };
+
+__device__ __forceinline__ float load_sample(const void* base, long long stride, int storage, int y, int x) {
+  const char* row = static_cast<const char*>(base) + (long long)y * stride;
+  if (storage == kU8) return (float)reinterpret_cast<const uint8_t*>(row)[x];
+  if (storage == kU16) return (float)reinterpret_cast<const uint16_t*>(row)[x];
+  return reinterpret_cast<const float*>(row)[x];
+}
+
+// Per-bin gain parameters of the reference SpectralParams
+// (src/kernels/spectral.hpp:19-27).
+struct GainParams {
+  int type;  // -2 enhancement only, -1 FFT3D Wiener, 0..4 DFTTest ftype
+  float a, b, low, high, exponent, floor_;
+  const float* primary;  // per-bin a (Table mode) or nullptr
+  // FFT3D sharpen/dehalo (fft3d_profile.cpp:87-99); active when sharpen||dehalo
+  float sharpen, dehalo, ea, eb, ec;
+  const float* sharpen_tab;
+  const float* halo_tab;
+};
+
+// One DFTTest plane of one output frame (spatial tbsize=1 path).
+struct DftJob {
+  Axis gx, gy;
+  int S;             // block step
+  int storage;       // source/destination storage
+  float dscale;      // decode scale (pad.hpp:96-101)
+  float oscale;      // output scale: 2^(bits-8) (int) or 1/255 (float)
+  float peak;        // 2^bits-1
+  float mean_scale;  // zmean ? 1 : 0
+  GainParams g;
+  const float* h;     // analysis window [B][B]
+  const float* hsyn;  // synthesis window h*V [B][B]
+  const float2* grid; // zmean template spectrum or nullptr
+  const void* src;
+  long long src_stride;
+  void* dst;
+  long long dst_stride;
+  float* acc_out;  // when non-null: write the float accumulator crop here (dither path)
+  long long acc_stride;
+  int tile_x0, tile_y0, tiles_x, tiles_y, tile_w, tile_h;
+  int nbx_max, ww_max;
+};
+
+constexpr int kMaxInlineJobs = 4;
+
+struct DftLaunch {
+  DftJob jobs[kMaxInlineJobs];
+  int tile_start[kMaxInlineJobs + 1];
+  int njobs;
+  unsigned* flags;
+};
+
+// One FFT3D plane of one output frame: block stage writes windowed
+// contributions, the synthesis stage sums them in the canonical order.
+struct Fft3dJob {
+  Axis gx, gy;
+  int storage;
+  int T;            // temporal slots used (1 = spatial / end-of-clip fallback)
+  int c;            // centre slot
+  int mode;         // 0 stateless, 1 Kalman advance, 2 Kalman render
+  float base;       // integer chroma midpoint
+  float peak;
+  float degrid;     // mean_scale
+  float lower;      // (beta-1)/beta
+  float noise;      // uniform temporal noise ((T*s)*s)/norm
+  float noise_mult; // table multiplier (T)
+  int denoise, varying, enh_separate;
+  GainParams g;         // spatial Wiener (+ fused enhancement)
+  GainParams enh;       // enhancement only (type -2)
+  const float* wxa;     // analysis / synthesis shoulders
+  const float* wxs;
+  const float* wya;
+  const float* wys;
+  const float2* grid;   // degrid template or nullptr
+  const float* ftw;     // temporal DFT twiddles fwd[m][j] as (re,im) pairs, 5x5
+  const float* itw;     // inverse twiddles for slot c, 5 pairs
+  const void* src[5];
+  long long src_stride[5];
+  void* dst;
+  long long dst_stride;
+  float* contrib;       // [count_y][count_x][bh][bw] windowed inverse * wx_s
+  // Kalman state (16 B per bin): last (complex), cov, proc
+  float2* k_last;
+  float* k_cov;
+  float* k_proc;
+  const float* k_pattern;  // per-bin R table or nullptr (uniform)
+  float k_r0, k_ratio2;
+  int blocks_per_group;    // blocks one workgroup handles along a block row
+  int groups_x;            // workgroups per block row
+};
+
+struct Fft3dLaunch {
+  Fft3dJob jobs[kMaxInlineJobs];
+  int group_start[kMaxInlineJobs + 1];
+  long long pixel_start[kMaxInlineJobs + 1];
+  int njobs;
+  unsigned* flags;
+};
+
+}  // namespace nf

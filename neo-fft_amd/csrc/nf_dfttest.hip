@@ -1,0 +1,251 @@
+// nf_dfttest.hip -- DFTTest spatial (tbsize=1) path on gfx950.
+//
+// One workgroup owns an output tile [X0,X1)x[Y0,Y1) (cover coordinates) and
+// walks the block rows that touch it from top to bottom. For each block row
+// it stages the reflected, decoded source rows in LDS, runs every covering
+// block through window -> 2D real FFT -> gain -> inverse -> synthesis window
+// with one lane per block row/column transform, and adds the block outputs
+// into an LDS ring of B accumulator rows in the reference's canonical order
+// (block row, then block column; reference plan.cpp:591-602 and
+// docs/specs/phase-5/dfttest/kernel-temporal-ola.md). Rows that no later
+// block row touches are converted and stored. No global atomics, no global
+// intermediate.
+#include "nf_device.hpp"
+#include "nf_fft.hpp"
+#include "nf_gain.hpp"
+#include "nf_launch.hpp"
+
+namespace nf {
+
+template <int B>
+__global__ __launch_bounds__(256) void dfttest_t1_kernel(const DftLaunch L) {
+  extern __shared__ float smem[];
+  constexpr int K = B / 2 + 1;
+  int jid = 0;
+  while (jid + 1 < L.njobs && (int)blockIdx.x >= L.tile_start[jid + 1]) ++jid;
+  const DftJob& J = L.jobs[jid];
+  const int tile = (int)blockIdx.x - L.tile_start[jid];
+  const int tx = tile % J.tiles_x, ty = tile / J.tiles_x;
+  const int S = J.S;
+  const int tid = threadIdx.x, nthreads = blockDim.x;
+
+  const int vx0 = J.gx.offset, vx1 = J.gx.offset + J.gx.length;
+  const int vy0 = J.gy.offset, vy1 = J.gy.offset + J.gy.length;
+  const int X0 = max(vx0, (J.tile_x0 + tx) * J.tile_w), X1 = min(vx1, (J.tile_x0 + tx + 1) * J.tile_w);
+  const int Y0 = max(vy0, (J.tile_y0 + ty) * J.tile_h), Y1 = min(vy1, (J.tile_y0 + ty + 1) * J.tile_h);
+  const int bx_lo = ceil_div_int(X0 - B + 1, S), bx_hi = (X1 - 1) / S;
+  const int by_lo = ceil_div_int(Y0 - B + 1, S), by_hi = (Y1 - 1) / S;
+  const int nbx = bx_hi - bx_lo + 1;
+  const int WW = (nbx - 1) * S + B;
+  const int cx0 = bx_lo * S;
+  const int TW = X1 - X0;
+
+  // LDS carve-up (floats)
+  float* spec = smem;                                  // [nbx_max][B][K][2]
+  float* out = spec + J.nbx_max * B * K * 2;           // [nbx_max][B][B]
+  float* win = out + J.nbx_max * B * B;                // [B][ww_max]
+  float* ring = win + B * J.ww_max;                    // [B][tile_w]
+  float* dc = ring + B * J.tile_w;                     // [nbx_max][2]
+  float* hw = dc + 2 * J.nbx_max;                      // [B*B]
+  float* hs = hw + B * B;                              // [B*B]
+
+  bool bad = false;
+  for (int i = tid; i < B * B; i += nthreads) {
+    hw[i] = J.h[i];
+    hs[i] = J.hsyn[i];
+  }
+  for (int i = tid; i < B * J.tile_w; i += nthreads) ring[i] = 0.0f;
+  const float fct = 1.0f / ((float)B * (float)B);
+  __syncthreads();
+
+  for (int by = by_lo; by <= by_hi; ++by) {
+    const int oy = by * S;
+    // A: decoded, reflected source window [B][WW]
+    for (int idx = tid; idx < B * WW; idx += nthreads) {
+      const int r = idx / WW, c = idx - r * WW;
+      const int sy = reflect(oy + r, J.gy), sx = reflect(cx0 + c, J.gx);
+      const float v = load_sample(J.src, J.src_stride, J.storage, sy, sx) * J.dscale;
+      bad |= !isfinite(v);
+      win[r * J.ww_max + c] = v;
+    }
+    __syncthreads();
+    // B: windowed rows -> forward row transforms, bins 0..B/2
+    for (int t = tid; t < nbx * B; t += nthreads) {
+      const int bi = t / B, y = t - bi * B;
+      float re[B], im[B];
+      const float* src = win + y * J.ww_max + bi * S;
+#pragma unroll
+      for (int x = 0; x < B; ++x) {
+        re[x] = src[x] * hw[y * B + x];
+        im[x] = 0.0f;
+      }
+      cfft<B, true>(re, im);
+      float* dst = spec + ((bi * B + y) * K) * 2;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        dst[2 * k] = re[k];
+        dst[2 * k + 1] = (k == B / 2) ? 0.0f : im[k];
+      }
+    }
+    __syncthreads();
+    // C: forward column transforms
+    for (int t = tid; t < nbx * K; t += nthreads) {
+      const int bi = t / K, k = t - bi * K;
+      float re[B], im[B];
+      float* col = spec + (bi * B * K + k) * 2;
+#pragma unroll
+      for (int y = 0; y < B; ++y) {
+        re[y] = col[y * K * 2];
+        im[y] = col[y * K * 2 + 1];
+      }
+      cfft<B, true>(re, im);
+#pragma unroll
+      for (int y = 0; y < B; ++y) {
+        col[y * K * 2] = re[y];
+        col[y * K * 2 + 1] = im[y];
+      }
+      if (k == 0) {
+        dc[2 * bi] = re[0];
+        dc[2 * bi + 1] = im[0];
+      }
+    }
+    __syncthreads();
+    // D: gain (zmean residual) and inverse column transforms
+    for (int t = tid; t < nbx * K; t += nthreads) {
+      const int bi = t / K, k = t - bi * K;
+      const float gs = J.grid ? (J.mean_scale * dc[2 * bi]) / J.grid[0].x : 0.0f;
+      float re[B], im[B];
+      float* col = spec + (bi * B * K + k) * 2;
+#pragma unroll
+      for (int y = 0; y < B; ++y) {
+        re[y] = col[y * K * 2];
+        im[y] = col[y * K * 2 + 1];
+        const int bin = y * K + k;
+        float mr = 0.0f, mi = 0.0f;
+        if (J.grid) {
+          const float2 gv = J.grid[bin];
+          mr = gs * gv.x;
+          mi = gs * gv.y;
+        }
+        apply_gain(J.g, bin, re[y], im[y], mr, mi, bad);
+      }
+      cfft<B, false>(re, im);
+#pragma unroll
+      for (int y = 0; y < B; ++y) {
+        col[y * K * 2] = re[y];
+        col[y * K * 2 + 1] = im[y];
+      }
+    }
+    __syncthreads();
+    // E: inverse row transforms with Hermitian extension, synthesis window
+    for (int t = tid; t < nbx * B; t += nthreads) {
+      const int bi = t / B, y = t - bi * B;
+      float re[B], im[B];
+      const float* row = spec + ((bi * B + y) * K) * 2;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        re[k] = row[2 * k];
+        im[k] = row[2 * k + 1];
+      }
+#pragma unroll
+      for (int k = K; k < B; ++k) {
+        re[k] = re[B - k];
+        im[k] = 0.0f - im[B - k];
+      }
+      cfft<B, false>(re, im);
+      float* o = out + (bi * B + y) * B;
+#pragma unroll
+      for (int x = 0; x < B; ++x) {
+        const float v = re[x] * fct;
+        bad |= !isfinite(v);
+        o[x] = v * hs[y * B + x];
+      }
+    }
+    __syncthreads();
+    // F: overlap-add into the ring, block columns in ascending order
+    {
+      const int r0 = max(oy, Y0), r1 = min(oy + B, Y1);
+      const int nr = r1 - r0;
+      for (int idx = tid; idx < nr * TW; idx += nthreads) {
+        const int rr = idx / TW;
+        const int x = X0 + (idx - rr * TW);
+        const int y = r0 + rr;
+        float* cell = ring + (y % B) * J.tile_w + (x - X0);
+        float sum = *cell;
+        const int b_first = max(bx_lo, ceil_div_int(x - B + 1, S));
+        const int b_last = min(bx_hi, x / S);
+        for (int bx = b_first; bx <= b_last; ++bx) {
+          const int bi = bx - bx_lo;
+          sum = sum + out[(bi * B + (y - oy)) * B + (x - bx * S)];
+        }
+        *cell = sum;
+      }
+    }
+    __syncthreads();
+    // G: rows [oy, oy+S) are final: convert, store, clear their ring rows
+    {
+      const int r0 = max(oy, Y0), r1 = min(oy + S, Y1);
+      const int nr = r1 - r0;
+      for (int idx = tid; idx < nr * TW; idx += nthreads) {
+        const int rr = idx / TW;
+        const int x = X0 + (idx - rr * TW);
+        const int y = r0 + rr;
+        float* cell = ring + (y % B) * J.tile_w + (x - X0);
+        const float a = *cell;
+        *cell = 0.0f;
+        bad |= !isfinite(a);
+        const int oyy = y - vy0, oxx = x - vx0;
+        if (J.acc_out) {
+          J.acc_out[(long long)oyy * J.acc_stride + oxx] = a;
+        } else {
+          char* drow = static_cast<char*>(J.dst) + (long long)oyy * J.dst_stride;
+          if (J.storage == kF32) {
+            reinterpret_cast<float*>(drow)[oxx] = a * J.oscale;
+          } else {
+            const float v = (a * J.oscale) + 0.5f;
+            const float cl = v < 0.0f ? 0.0f : (J.peak < v ? J.peak : v);
+            if (J.storage == kU8) reinterpret_cast<uint8_t*>(drow)[oxx] = (uint8_t)cl;
+            else reinterpret_cast<uint16_t*>(drow)[oxx] = (uint16_t)cl;
+          }
+        }
+      }
+    }
+    // The next block row's stages A..E touch neither the ring nor the rows
+    // cleared above; stage F runs after four more barriers.
+  }
+  if (bad) atomicOr(L.flags, NF_FLAG_NONFINITE);
+}
+
+size_t dfttest_t1_lds_bytes(int B, const DftJob& j) {
+  const int K = B / 2 + 1;
+  return sizeof(float) * ((size_t)j.nbx_max * B * K * 2 + (size_t)j.nbx_max * B * B + (size_t)B * j.ww_max +
+                          (size_t)B * j.tile_w + 2 * (size_t)j.nbx_max + 2 * (size_t)B * B);
+}
+
+hipError_t launch_dfttest_t1(int B, const DftLaunch& L, int total_tiles, size_t lds, hipStream_t stream) {
+  dim3 grid(total_tiles), block(256);
+  switch (B) {
+    case 8:
+      hipLaunchKernelGGL(dfttest_t1_kernel<8>, grid, block, lds, stream, L);
+      break;
+    case 16:
+      hipLaunchKernelGGL(dfttest_t1_kernel<16>, grid, block, lds, stream, L);
+      break;
+    case 32:
+      hipLaunchKernelGGL(dfttest_t1_kernel<32>, grid, block, lds, stream, L);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t dfttest_t1_set_lds_limit() {
+  hipError_t e = hipFuncSetAttribute((const void*)dfttest_t1_kernel<8>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)dfttest_t1_kernel<16>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void*)dfttest_t1_kernel<32>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+
+}  // namespace nf

@@ -1,0 +1,221 @@
+// nf_fft.hpp -- per-lane radix-2 complex transforms of length 8, 16 and 32,
+// shared by the gfx950 kernels and the host plan builder.
+//
+// Operation order follows the reference's AVX2 codelets so that results are
+// bit-identical to them up to compiler contraction: bit-reversed radix-2 DIT
+// (src/spectral/codelets/fft16x16_avx2.cpp:27-201), the 8-point variant with
+// its sqrt(1/2) rotations (fft8x8_avx2.cpp:49-140), and the 32-point
+// even/odd split over two 16-point transforms (fft32x32_avx2.cpp:251-275).
+// Twiddles are binary32 cos/sin of the binary32 angle k*PI/8 (k*PI/16), with
+// PI the binary32 constant, as the codelets initialise them
+// (fft16x16_avx2.cpp:11-25); the complex twiddle product is
+// (fma(ir,wr,-(ii*wi)), fma(ir,wi,ii*wr)) (fft16x16_avx2.cpp:64-69).
+// Everything else is compiled with -ffp-contract=off.
+#pragma once
+#include <cmath>
+
+#if defined(__HIPCC__) || defined(__HIP__)
+#include <hip/hip_runtime.h>
+#define NF_HD __host__ __device__ __forceinline__
+#else
+#define NF_HD inline
+#endif
+
+namespace nf {
+
+struct Twiddle {
+  static constexpr float w16r[8] = {0x1p+0f, 0x1.d906bcp-1f, 0x1.6a09e6p-1f, 0x1.87de2ap-2f,
+                                    -0x1.777a5cp-25f, -0x1.87de3p-2f, -0x1.6a09e6p-1f, -0x1.d906cp-1f};
+  static constexpr float w16i[8] = {0x0p+0f, 0x1.87de2cp-2f, 0x1.6a09e6p-1f, 0x1.d906bcp-1f,
+                                    0x1p+0f, 0x1.d906bcp-1f, 0x1.6a09e6p-1f, 0x1.87de2p-2f};
+  static constexpr float w32r[16] = {0x1p+0f, 0x1.f6297cp-1f, 0x1.d906bcp-1f, 0x1.a9b662p-1f,
+                                     0x1.6a09e6p-1f, 0x1.1c73b2p-1f, 0x1.87de2ap-2f, 0x1.8f8b78p-3f,
+                                     -0x1.777a5cp-25f, -0x1.8f8b84p-3f, -0x1.87de3p-2f, -0x1.1c73b8p-1f,
+                                     -0x1.6a09e6p-1f, -0x1.a9b664p-1f, -0x1.d906cp-1f, -0x1.f6297ep-1f};
+  static constexpr float w32i[16] = {0x0p+0f, 0x1.8f8b84p-3f, 0x1.87de2cp-2f, 0x1.1c73b4p-1f,
+                                     0x1.6a09e6p-1f, 0x1.a9b664p-1f, 0x1.d906bcp-1f, 0x1.f6297ep-1f,
+                                     0x1p+0f, 0x1.f6297cp-1f, 0x1.d906bcp-1f, 0x1.a9b66p-1f,
+                                     0x1.6a09e6p-1f, 0x1.1c73b2p-1f, 0x1.87de2p-2f, 0x1.8f8b82p-3f};
+};
+
+NF_HD void bf2(float& xr, float& xi, float& yr, float& yi) {
+  const float ur = xr, ui = xi, vr = yr, vi = yi;
+  xr = ur + vr;
+  xi = ui + vi;
+  yr = ur - vr;
+  yi = ui - vi;
+}
+
+template <bool FWD>
+NF_HD void cmul(float& orr, float& oi, float ir, float ii, float wr, float wi_val) {
+  const float wi = FWD ? -wi_val : wi_val;
+  const float p = ii * wi, q = ii * wr;
+  orr = fmaf(ir, wr, -p);
+  oi = fmaf(ir, wi, q);
+}
+
+template <bool FWD>
+NF_HD void mulj(float& orr, float& oi, float ir, float ii) {
+  if (FWD) {
+    orr = ii;
+    oi = 0.0f - ir;
+  } else {
+    orr = 0.0f - ii;
+    oi = ir;
+  }
+}
+
+// Combine butterfly u +/- v with v already multiplied.
+NF_HD void bfv(float& ur, float& ui, float& lr, float& li, float vr, float vi) {
+  const float a = ur, b = ui;
+  ur = a + vr;
+  ui = b + vi;
+  lr = a - vr;
+  li = b - vi;
+}
+
+template <bool FWD>
+NF_HD void fft8(float* r, float* i) {
+  const float s = 0.70710678118654752440f;
+  float ar[8] = {r[0], r[4], r[2], r[6], r[1], r[5], r[3], r[7]};
+  float ai[8] = {i[0], i[4], i[2], i[6], i[1], i[5], i[3], i[7]};
+  bf2(ar[0], ai[0], ar[1], ai[1]);
+  bf2(ar[2], ai[2], ar[3], ai[3]);
+  bf2(ar[4], ai[4], ar[5], ai[5]);
+  bf2(ar[6], ai[6], ar[7], ai[7]);
+  bf2(ar[0], ai[0], ar[2], ai[2]);
+  bf2(ar[4], ai[4], ar[6], ai[6]);
+  {
+    float tr, ti;
+    mulj<FWD>(tr, ti, ar[3], ai[3]);
+    ar[3] = tr;
+    ai[3] = ti;
+    mulj<FWD>(tr, ti, ar[7], ai[7]);
+    ar[7] = tr;
+    ai[7] = ti;
+  }
+  bf2(ar[1], ai[1], ar[3], ai[3]);
+  bf2(ar[5], ai[5], ar[7], ai[7]);
+  bf2(ar[0], ai[0], ar[4], ai[4]);
+  {
+    const float ur = ar[5], ui = ai[5];
+    if (FWD) {
+      ar[5] = s * (ur + ui);
+      ai[5] = s * (ui - ur);
+    } else {
+      ar[5] = s * (ur - ui);
+      ai[5] = s * (ui + ur);
+    }
+  }
+  bf2(ar[1], ai[1], ar[5], ai[5]);
+  {
+    float tr, ti;
+    mulj<FWD>(tr, ti, ar[6], ai[6]);
+    ar[6] = tr;
+    ai[6] = ti;
+  }
+  bf2(ar[2], ai[2], ar[6], ai[6]);
+  {
+    const float ur = ar[7], ui = ai[7];
+    if (FWD) {
+      ar[7] = s * (ui - ur);
+      ai[7] = s * ((0.0f - ur) - ui);
+    } else {
+      ar[7] = s * ((0.0f - ur) - ui);
+      ai[7] = s * (ur - ui);
+    }
+  }
+  bf2(ar[3], ai[3], ar[7], ai[7]);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    r[k] = ar[k];
+    i[k] = ai[k];
+  }
+}
+
+template <bool FWD>
+NF_HD void fft16(float* r, float* i) {
+  constexpr int br[16] = {0, 8, 4, 12, 2, 10, 6, 14, 1, 9, 5, 13, 3, 11, 7, 15};
+  float ar[16], ai[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    ar[k] = r[br[k]];
+    ai[k] = i[br[k]];
+  }
+#pragma unroll
+  for (int k = 0; k < 16; k += 2) bf2(ar[k], ai[k], ar[k + 1], ai[k + 1]);
+#pragma unroll
+  for (int g = 0; g < 16; g += 4) {
+    bf2(ar[g], ai[g], ar[g + 2], ai[g + 2]);
+    float vr, vi;
+    mulj<FWD>(vr, vi, ar[g + 3], ai[g + 3]);
+    bfv(ar[g + 1], ai[g + 1], ar[g + 3], ai[g + 3], vr, vi);
+  }
+#pragma unroll
+  for (int g = 0; g < 16; g += 8) {
+    bf2(ar[g], ai[g], ar[g + 4], ai[g + 4]);
+    float vr, vi;
+    cmul<FWD>(vr, vi, ar[g + 5], ai[g + 5], Twiddle::w16r[2], Twiddle::w16i[2]);
+    bfv(ar[g + 1], ai[g + 1], ar[g + 5], ai[g + 5], vr, vi);
+    mulj<FWD>(vr, vi, ar[g + 6], ai[g + 6]);
+    bfv(ar[g + 2], ai[g + 2], ar[g + 6], ai[g + 6], vr, vi);
+    cmul<FWD>(vr, vi, ar[g + 7], ai[g + 7], Twiddle::w16r[6], Twiddle::w16i[6]);
+    bfv(ar[g + 3], ai[g + 3], ar[g + 7], ai[g + 7], vr, vi);
+  }
+  bf2(ar[0], ai[0], ar[8], ai[8]);
+#pragma unroll
+  for (int k = 1; k < 8; ++k) {
+    float vr, vi;
+    if (k == 4)
+      mulj<FWD>(vr, vi, ar[12], ai[12]);
+    else
+      cmul<FWD>(vr, vi, ar[8 + k], ai[8 + k], Twiddle::w16r[k], Twiddle::w16i[k]);
+    bfv(ar[k], ai[k], ar[8 + k], ai[8 + k], vr, vi);
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    r[k] = ar[k];
+    i[k] = ai[k];
+  }
+}
+
+template <bool FWD>
+NF_HD void fft32(float* r, float* i) {
+  float er[16], ei[16], orr[16], oi[16];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    er[m] = r[2 * m];
+    ei[m] = i[2 * m];
+    orr[m] = r[2 * m + 1];
+    oi[m] = i[2 * m + 1];
+  }
+  fft16<FWD>(er, ei);
+  fft16<FWD>(orr, oi);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    float vr, vi;
+    cmul<FWD>(vr, vi, orr[k], oi[k], Twiddle::w32r[k], Twiddle::w32i[k]);
+    r[k] = er[k] + vr;
+    i[k] = ei[k] + vi;
+    r[k + 16] = er[k] - vr;
+    i[k + 16] = ei[k] - vi;
+  }
+}
+
+template <int N, bool FWD>
+NF_HD void cfft(float* r, float* i) {
+  static_assert(N == 8 || N == 16 || N == 32, "transform length");
+  if constexpr (N == 8) fft8<FWD>(r, i);
+  else if constexpr (N == 16) fft16<FWD>(r, i);
+  else fft32<FWD>(r, i);
+}
+
+// Host-side dispatch by runtime length (plan creation only).
+inline bool cfft_supported(int n) { return n == 8 || n == 16 || n == 32; }
+inline void cfft_host(int n, float* r, float* i, bool fwd) {
+  if (n == 8) fwd ? fft8<true>(r, i) : fft8<false>(r, i);
+  else if (n == 16) fwd ? fft16<true>(r, i) : fft16<false>(r, i);
+  else fwd ? fft32<true>(r, i) : fft32<false>(r, i);
+}
+
+}  // namespace nf

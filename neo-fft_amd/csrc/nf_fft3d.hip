@@ -1,0 +1,391 @@
+// nf_fft3d.hip -- FFT3D paths on gfx950: stateless bt=-1..5 (spatial,
+// temporal Wiener, enhancement) and the Kalman recurrence (bt=0).
+//
+// Stage 1 (fft3d_block_kernel): a workgroup takes a run of consecutive
+// blocks of one block row. Per temporal slot it stages the decoded,
+// reflected source rows in LDS, applies the separable analysis window
+// ((src*wy)*wx, reference scalar/spatial.cpp:12-16), runs the 2D real FFT
+// with one lane per row and per column, then evaluates the per-bin filter
+// (spatial Wiener + sharpen/dehalo, temporal T-point DFT Wiener, Kalman
+// update or Kalman render), the inverse FFT and the horizontal synthesis
+// window, and writes each block's windowed output to a contribution buffer.
+// Stage 2 (fft3d_synthesis_kernel): one lane per visible pixel sums the
+// contributions in the reference's two-level canonical order -- block
+// columns left to right into a row value, times the vertical synthesis
+// weight, block rows top to bottom (plan.cpp:486-490, 520-535) -- and
+// converts to the output format (scalar/spatial.cpp:63-88).
+#include "nf_device.hpp"
+#include "nf_fft.hpp"
+#include "nf_gain.hpp"
+#include "nf_launch.hpp"
+
+namespace nf {
+
+namespace {
+
+// spectral.cpp:162-232 for one bin; x[j] are the T raw spectra.
+__device__ __forceinline__ void temporal_bin(const Fft3dJob& J, int k, const float* xr, const float* xi,
+                                             float g_ratio, float& out_r, float& out_i, bool& bad) {
+  const int T = J.T;
+  float Mr = 0.0f, Mi = 0.0f;
+  if (J.grid) {
+    const float2 gv = J.grid[k];
+    Mr = g_ratio * gv.x;
+    Mi = g_ratio * gv.y;
+  }
+  const float gTr = Mr * (float)T, gTi = Mi * (float)T;
+  float Fr[5], Fi[5];
+  for (int m = 0; m < T; ++m) {
+    float sr = 0.0f, si = 0.0f;
+    for (int j = 0; j < T; ++j) {
+      const float wr = J.ftw[2 * (m * 5 + j)], wi = J.ftw[2 * (m * 5 + j) + 1];
+      const float pr = xr[j] * wr - xi[j] * wi;
+      const float pi = xr[j] * wi + xi[j] * wr;
+      sr = sr + pr;
+      si = si + pi;
+    }
+    Fr[m] = sr;
+    Fi[m] = si;
+  }
+  Fr[0] = Fr[0] - gTr;
+  Fi[0] = Fi[0] - gTi;
+  const float noise = J.g.primary ? J.g.primary[k] * J.noise_mult : J.noise;
+  bad |= !isfinite(noise);
+  for (int m = 0; m < T; ++m) {
+    const float power = Fr[m] * Fr[m] + Fi[m] * Fi[m];
+    bad |= !isfinite(power);
+    const float q = power + 1e-15f;
+    const float gain = max_ref((q - noise) / q, J.lower);
+    Fr[m] = gain * Fr[m];
+    Fi[m] = gain * Fi[m];
+  }
+  Fr[0] = Fr[0] + gTr;
+  Fi[0] = Fi[0] + gTi;
+  float yr = 0.0f, yi = 0.0f;
+  for (int m = 0; m < T; ++m) {
+    const float wr = J.itw[2 * m], wi = J.itw[2 * m + 1];
+    const float pr = Fr[m] * wr - Fi[m] * wi;
+    const float pi = Fr[m] * wi + Fi[m] * wr;
+    yr = yr + pr;
+    yi = yi + pi;
+  }
+  const float invT = 1.0f / (float)T;
+  out_r = yr * invT;
+  out_i = yi * invT;
+}
+
+// kalman.cpp:3-24 for one bin.
+__device__ __forceinline__ void kalman_bin(const Fft3dJob& J, long long s, int k, float xr, float xi, bool& bad) {
+  const float r = J.k_pattern ? max_ref(J.k_pattern[k], 1e-15f) : J.k_r0;
+  const float threshold = r * J.k_ratio2;
+  const float2 l = J.k_last[s];
+  const float c = J.k_cov[s], q = J.k_proc[s];
+  bad |= !isfinite(r) || !isfinite(threshold) || !isfinite(xr) || !isfinite(xi) || !isfinite(l.x) ||
+         !isfinite(l.y) || !isfinite(c) || !isfinite(q);
+  const float dr = xr - l.x, di = xi - l.y;
+  if ((!J.k_pattern && r == 0.0f) || dr * dr > threshold || di * di > threshold) {
+    J.k_last[s] = make_float2(xr, xi);
+    J.k_cov[s] = r;
+    J.k_proc[s] = r;
+    return;
+  }
+  const float sum = c + q;
+  const float gain = sum / (sum + r);
+  const float complement = 1.0f - gain;
+  const float nq = (gain * gain) * r, nc = complement * sum;
+  const float nr = (gain * xr) + (complement * l.x);
+  const float ni = (gain * xi) + (complement * l.y);
+  bad |= !isfinite(sum) || !isfinite(gain) || !isfinite(nq) || !isfinite(nc) || !isfinite(nr) || !isfinite(ni);
+  J.k_last[s] = make_float2(nr, ni);
+  J.k_cov[s] = nc;
+  J.k_proc[s] = nq;
+}
+
+}  // namespace
+
+template <int BW, int BH>
+__global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
+  extern __shared__ float smem[];
+  constexpr int K = BW / 2 + 1;
+  constexpr int BINS = BH * K;
+  int jid = 0;
+  while (jid + 1 < L.njobs && (int)blockIdx.x >= L.group_start[jid + 1]) ++jid;
+  const Fft3dJob& J = L.jobs[jid];
+  const int g = (int)blockIdx.x - L.group_start[jid];
+  const int by = g / J.groups_x, gx = g - by * J.groups_x;
+  const int NB = J.blocks_per_group;
+  const int bx0 = gx * NB;
+  const int nb = min(NB, J.gx.count - bx0);
+  const int Sx = J.gx.step, Sy = J.gy.step;
+  const int oy = by * Sy, ox0 = bx0 * Sx;
+  const int WW = (nb - 1) * Sx + BW;
+  const int ww_max = (NB - 1) * Sx + BW;
+  const int tid = threadIdx.x, nthreads = blockDim.x;
+  const int Tst = J.mode == 0 ? J.T : 1;  // spectra held in LDS
+
+  float* spec = smem;                          // [Tst][NB][BINS][2]
+  float* win = spec + Tst * NB * BINS * 2;     // [BH][ww_max]
+  float* dc = win + BH * ww_max;               // [NB][2]
+  bool bad = false;
+
+  if (J.mode != 2) {
+    for (int j = 0; j < Tst; ++j) {
+      // decoded, reflected source rows: (x - base) * 1 (pad.hpp:33-38)
+      for (int idx = tid; idx < BH * WW; idx += nthreads) {
+        const int r = idx / WW, c = idx - r * WW;
+        const int sy = reflect(oy + r, J.gy), sx = reflect(ox0 + c, J.gx);
+        const float v = load_sample(J.src[j], J.src_stride[j], J.storage, sy, sx) - J.base;
+        bad |= !isfinite(v);
+        win[r * ww_max + c] = v;
+      }
+      __syncthreads();
+      float* sj = spec + j * NB * BINS * 2;
+      for (int t = tid; t < nb * BH; t += nthreads) {
+        const int bi = t / BH, y = t - bi * BH;
+        float re[BW], im[BW];
+        const float* src = win + y * ww_max + bi * Sx;
+        const float wy = J.wya[y];
+#pragma unroll
+        for (int x = 0; x < BW; ++x) {
+          re[x] = (src[x] * wy) * J.wxa[x];
+          im[x] = 0.0f;
+        }
+        cfft<BW, true>(re, im);
+        float* dst = sj + (bi * BINS + y * K) * 2;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          dst[2 * k] = re[k];
+          dst[2 * k + 1] = (k == BW / 2) ? 0.0f : im[k];
+        }
+      }
+      __syncthreads();
+      for (int t = tid; t < nb * K; t += nthreads) {
+        const int bi = t / K, k = t - bi * K;
+        float re[BH], im[BH];
+        float* col = sj + (bi * BINS + k) * 2;
+#pragma unroll
+        for (int y = 0; y < BH; ++y) {
+          re[y] = col[y * K * 2];
+          im[y] = col[y * K * 2 + 1];
+        }
+        cfft<BH, true>(re, im);
+#pragma unroll
+        for (int y = 0; y < BH; ++y) {
+          col[y * K * 2] = re[y];
+          col[y * K * 2 + 1] = im[y];
+        }
+        if (k == 0 && j == 0) {
+          dc[2 * bi] = re[0];
+          dc[2 * bi + 1] = im[0];
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  const long long block_base = (long long)by * J.gx.count + bx0;
+  if (J.mode == 1) {
+    // Kalman advance: update the recurrence with this frame's spectra.
+    for (int t = tid; t < nb * BINS; t += nthreads) {
+      const int bi = t / BINS, k = t - bi * BINS;
+      const long long s = (block_base + bi) * BINS + k;
+      kalman_bin(J, s, k, spec[(bi * BINS + k) * 2], spec[(bi * BINS + k) * 2 + 1], bad);
+    }
+    if (bad) atomicOr(L.flags, NF_FLAG_NONFINITE);
+    return;
+  }
+
+  if (J.mode == 2) {
+    // Kalman render: private copy of L (plan.cpp:471-474)
+    for (int t = tid; t < nb * BINS; t += nthreads) {
+      const int bi = t / BINS, k = t - bi * BINS;
+      const float2 l = J.k_last[(block_base + bi) * BINS + k];
+      spec[(bi * BINS + k) * 2] = l.x;
+      spec[(bi * BINS + k) * 2 + 1] = l.y;
+      if (k == 0) {
+        dc[2 * bi] = l.x;
+        dc[2 * bi + 1] = l.y;
+      }
+    }
+    __syncthreads();
+  } else if (J.T > 1) {
+    // temporal Wiener per bin into slot 0 (spectral.cpp:162-232)
+    const int c = J.c;
+    for (int t = tid; t < nb * BINS; t += nthreads) {
+      const int bi = t / BINS, k = t - bi * BINS;
+      float xr[5], xi[5];
+      for (int j = 0; j < J.T; ++j) {
+        xr[j] = spec[((j * NB + bi) * BINS + k) * 2];
+        xi[j] = spec[((j * NB + bi) * BINS + k) * 2 + 1];
+      }
+      const float dcc = spec[((c * NB + bi) * BINS) * 2];
+      const float g_ratio = (J.grid && J.degrid != 0.0f && J.grid[0].x != 0.0f) ? (J.degrid * dcc) / J.grid[0].x : 0.0f;
+      bad |= !isfinite(g_ratio);
+      float orr, oi;
+      temporal_bin(J, k, xr, xi, g_ratio, orr, oi, bad);
+      spec[(bi * BINS + k) * 2] = orr;
+      spec[(bi * BINS + k) * 2 + 1] = oi;
+      if (k == 0) {
+        dc[2 * bi] = orr;
+        dc[2 * bi + 1] = oi;
+      }
+    }
+    __syncthreads();
+  } else if (J.denoise) {
+    // spatial Wiener (type -1) with fused enhancement when uniform
+    for (int t = tid; t < nb * K; t += nthreads) {
+      const int bi = t / K, k = t - bi * K;
+      const float scale = J.grid ? (J.degrid * dc[2 * bi]) / J.grid[0].x : 0.0f;
+      float* col = spec + (bi * BINS + k) * 2;
+      for (int y = 0; y < BH; ++y) {
+        const int bin = y * K + k;
+        float mr = 0.0f, mi = 0.0f;
+        if (J.grid) {
+          mr = scale * J.grid[bin].x;
+          mi = scale * J.grid[bin].y;
+        }
+        float re = col[y * K * 2], im = col[y * K * 2 + 1];
+        apply_gain(J.g, bin, re, im, mr, mi, bad);
+        col[y * K * 2] = re;
+        col[y * K * 2 + 1] = im;
+      }
+    }
+    __syncthreads();
+    if (J.enh_separate) {
+      for (int bi = tid; bi < nb; bi += nthreads) {
+        dc[2 * bi] = spec[(bi * BINS) * 2];
+        dc[2 * bi + 1] = spec[(bi * BINS) * 2 + 1];
+      }
+      __syncthreads();
+    }
+  }
+
+  // enhancement-only pass (plan.cpp:325-329) fused into the inverse columns
+  const bool enhance = J.mode == 2 || J.T > 1 || !J.denoise ? (J.enh.sharpen != 0.0f || J.enh.dehalo != 0.0f)
+                                                            : J.enh_separate != 0;
+  for (int t = tid; t < nb * K; t += nthreads) {
+    const int bi = t / K, k = t - bi * K;
+    float re[BH], im[BH];
+    float* col = spec + (bi * BINS + k) * 2;
+#pragma unroll
+    for (int y = 0; y < BH; ++y) {
+      re[y] = col[y * K * 2];
+      im[y] = col[y * K * 2 + 1];
+    }
+    if (enhance) {
+      const float scale = J.grid ? (J.degrid * dc[2 * bi]) / J.grid[0].x : 0.0f;
+      bad |= !isfinite(scale);
+#pragma unroll
+      for (int y = 0; y < BH; ++y) {
+        const int bin = y * K + k;
+        float mr = 0.0f, mi = 0.0f;
+        if (J.grid) {
+          mr = scale * J.grid[bin].x;
+          mi = scale * J.grid[bin].y;
+        }
+        apply_gain(J.enh, bin, re[y], im[y], mr, mi, bad);
+      }
+    }
+    cfft<BH, false>(re, im);
+#pragma unroll
+    for (int y = 0; y < BH; ++y) {
+      col[y * K * 2] = re[y];
+      col[y * K * 2 + 1] = im[y];
+    }
+  }
+  __syncthreads();
+  const float fct = 1.0f / ((float)BW * (float)BH);
+  for (int t = tid; t < nb * BH; t += nthreads) {
+    const int bi = t / BH, y = t - bi * BH;
+    float re[BW], im[BW];
+    const float* row = spec + (bi * BINS + y * K) * 2;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      re[k] = row[2 * k];
+      im[k] = row[2 * k + 1];
+    }
+#pragma unroll
+    for (int k = K; k < BW; ++k) {
+      re[k] = re[BW - k];
+      im[k] = 0.0f - im[BW - k];
+    }
+    cfft<BW, false>(re, im);
+    float* dst = J.contrib + ((block_base + bi) * BH + y) * BW;
+#pragma unroll
+    for (int x = 0; x < BW; ++x) {
+      const float v = re[x] * fct;
+      bad |= !isfinite(v);
+      dst[x] = v * J.wxs[x];
+    }
+  }
+  if (bad) atomicOr(L.flags, NF_FLAG_NONFINITE);
+}
+
+// One lane per visible pixel of every job (jobs laid out back to back).
+__global__ __launch_bounds__(256) void fft3d_synthesis_kernel(const Fft3dLaunch L) {
+  const long long* pixel_start = L.pixel_start;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int jid = 0;
+  while (jid + 1 < L.njobs && gid >= pixel_start[jid + 1]) ++jid;
+  if (gid >= pixel_start[L.njobs]) return;
+  const Fft3dJob& J = L.jobs[jid];
+  const long long p = gid - pixel_start[jid];
+  const int w = J.gx.length;
+  const int oy = (int)(p / w), ox = (int)(p - (long long)oy * w);
+  const int y = oy + J.gy.offset, x = ox + J.gx.offset;
+  const int BW = J.gx.block, BH = J.gy.block, Sx = J.gx.step, Sy = J.gy.step;
+  const int by_first = max(0, ceil_div_int(y - BH + 1, Sy)), by_last = min(J.gy.count - 1, y / Sy);
+  const int bx_first = max(0, ceil_div_int(x - BW + 1, Sx)), bx_last = min(J.gx.count - 1, x / Sx);
+  float acc = 0.0f;
+  for (int by = by_first; by <= by_last; ++by) {
+    const int yy = y - by * Sy;
+    float r = 0.0f;
+    for (int bx = bx_first; bx <= bx_last; ++bx) {
+      const long long blk = (long long)by * J.gx.count + bx;
+      r = r + J.contrib[(blk * BH + yy) * BW + (x - bx * Sx)];
+    }
+    acc = acc + r * J.wys[yy];
+  }
+  if (!isfinite(acc)) atomicOr(L.flags, NF_FLAG_NONFINITE);
+  char* drow = static_cast<char*>(J.dst) + (long long)oy * J.dst_stride;
+  if (J.storage == kF32) {
+    reinterpret_cast<float*>(drow)[ox] = acc < 0.0f ? 0.0f : (1.0f < acc ? 1.0f : acc);
+  } else {
+    const float v = (acc + 0.5f) + J.base;
+    const float cl = v < 0.0f ? 0.0f : (J.peak < v ? J.peak : v);
+    if (J.storage == kU8) reinterpret_cast<uint8_t*>(drow)[ox] = (uint8_t)cl;
+    else reinterpret_cast<uint16_t*>(drow)[ox] = (uint16_t)cl;
+  }
+}
+
+size_t fft3d_block_lds_bytes(int BW, int BH, int T, int NB, int Sx) {
+  const int K = BW / 2 + 1;
+  const int ww_max = (NB - 1) * Sx + BW;
+  return sizeof(float) * ((size_t)T * NB * BH * K * 2 + (size_t)BH * ww_max + 2 * (size_t)NB);
+}
+
+hipError_t launch_fft3d_block(int BW, int BH, const Fft3dLaunch& L, int total_groups, size_t lds, hipStream_t stream) {
+  dim3 grid(total_groups), block(128);
+  if (BW == 8 && BH == 8) hipLaunchKernelGGL((fft3d_block_kernel<8, 8>), grid, block, lds, stream, L);
+  else if (BW == 16 && BH == 16) hipLaunchKernelGGL((fft3d_block_kernel<16, 16>), grid, block, lds, stream, L);
+  else if (BW == 32 && BH == 32) hipLaunchKernelGGL((fft3d_block_kernel<32, 32>), grid, block, lds, stream, L);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_fft3d_synthesis(const Fft3dLaunch& L, hipStream_t stream) {
+  const long long total_pixels = L.pixel_start[L.njobs];
+  const long long blocks = (total_pixels + 255) / 256;
+  hipLaunchKernelGGL(fft3d_synthesis_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, L);
+  return hipGetLastError();
+}
+
+hipError_t fft3d_set_lds_limit() {
+  hipError_t e = hipFuncSetAttribute((const void*)fft3d_block_kernel<8, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)fft3d_block_kernel<16, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void*)fft3d_block_kernel<32, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+
+}  // namespace nf

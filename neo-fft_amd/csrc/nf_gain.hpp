@@ -1,0 +1,72 @@
+// nf_gain.hpp -- per-bin spectral gains of the reference's scalar operators,
+// evaluated in the same binary32 operation order (-ffp-contract=off).
+#pragma once
+#include "nf_device.hpp"
+
+namespace nf {
+
+__device__ __forceinline__ float max_ref(float a, float b) { return a < b ? b : a; }  // std::max
+
+// fft3d_profile.cpp:87-99
+__device__ __forceinline__ float enhancement_gain(float q, int k, const GainParams& e) {
+  float s = 1.0f, d = 1.0f;
+  if (e.sharpen != 0.0f && e.eb != 0.0f) {
+    const float num = q * e.eb, den = (q + e.ea) * (q + e.eb);
+    const float strength = e.sharpen * e.sharpen_tab[k];
+    s = 1.0f + strength * sqrtf(num / den);
+  }
+  if (e.dehalo != 0.0f) {
+    const float qc = q + e.ec;
+    d = qc / (qc + (e.dehalo * e.halo_tab[k]) * q);
+  }
+  return s * d;
+}
+
+// spectral.cpp:5-53 (spectral_prevalidated). x is modified in place; (mr, mi)
+// is the bin of scale*grid. Sets bad on any non-finite intermediate that can
+// reach the output.
+__device__ __forceinline__ void apply_gain(const GainParams& p, int k, float& xr, float& xi, float mr, float mi,
+                                           bool& bad) {
+  const float a = p.primary ? p.primary[k] : p.a;
+  const float re = xr - mr, im = xi - mi;
+  const float power = re * re + im * im;
+  float gain = 0.0f;
+  switch (p.type) {
+    case -2:
+      gain = 1.0f;
+      break;
+    case -1: {
+      const float q = power + 1e-15f;
+      gain = max_ref((q - a) / q, p.floor_);
+      break;
+    }
+    case 0: {
+      const float wiener = max_ref((power - a) / (power + 1e-15f), 0.0f);
+      gain = fabsf(p.exponent - 1.0f) < 0.00005f   ? wiener
+             : fabsf(p.exponent - 0.5f) < 0.00005f ? sqrtf(wiener)
+                                                   : powf(wiener, p.exponent);
+      break;
+    }
+    case 1:
+      gain = power < a ? 0.0f : 1.0f;
+      break;
+    case 2:
+      gain = a;
+      break;
+    case 3:
+      gain = (power >= p.low && power <= p.high) ? a : p.b;
+      break;
+    default: {  // 4
+      const float q = power + 1e-15f;
+      const float num = q * p.high, den = (q + p.low) * (q + p.high);
+      gain = a * sqrtf(num / den);
+      break;
+    }
+  }
+  if (p.sharpen != 0.0f || p.dehalo != 0.0f) gain = gain * enhancement_gain(power + 1e-15f, k, p);
+  xr = gain * re + mr;
+  xi = gain * im + mi;
+  bad |= !isfinite(power) || !isfinite(gain) || !isfinite(xr) || !isfinite(xi) || !isfinite(mr) || !isfinite(mi);
+}
+
+}  // namespace nf

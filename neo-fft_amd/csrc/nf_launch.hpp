@@ -1,0 +1,14 @@
+// nf_launch.hpp -- host-side entry points of the gfx950 kernels.
+#pragma once
+#include "nf_device.hpp"
+
+namespace nf {
+size_t dfttest_t1_lds_bytes(int B, const DftJob& j);
+hipError_t launch_dfttest_t1(int B, const DftLaunch& L, int total_tiles, size_t lds, hipStream_t stream);
+hipError_t dfttest_t1_set_lds_limit();
+
+size_t fft3d_block_lds_bytes(int BW, int BH, int T, int NB, int Sx);
+hipError_t launch_fft3d_block(int BW, int BH, const Fft3dLaunch& L, int total_groups, size_t lds, hipStream_t stream);
+hipError_t launch_fft3d_synthesis(const Fft3dLaunch& L, hipStream_t stream);
+hipError_t fft3d_set_lds_limit();
+}  // namespace nf

@@ -1,0 +1,60 @@
+// nf_tables.hpp -- creation-time geometry and tables of a plan, built on the
+// host in the reference's binary64 -> binary32 order and uploaded once.
+#pragma once
+#include <complex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "nf_device.hpp"
+
+namespace nf {
+
+// reference src/base/checked.hpp:13-20: creation-time std::invalid_argument
+struct InvalidArgument : std::invalid_argument {
+  using std::invalid_argument::invalid_argument;
+};
+struct Unsupported : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+inline void require(bool ok, const std::string& message) {
+  if (!ok) throw InvalidArgument(message);
+}
+// reference checked.hpp:37-41 (creation-time use)
+float finite(float v);
+
+Axis fft3d_axis(int length, int block, int overlap);
+Axis dft_axis(int length, int block, int mode, int overlap);
+
+struct AxisWindow {
+  std::vector<float> analysis, synthesis;
+};
+AxisWindow fft3d_window(int block, int overlap, int type);
+double dft_raw_window(int id, int j, int length, float beta);
+struct DftWindow {
+  std::vector<float> h;
+  float wscale;
+};
+DftWindow dft_window_3d(int tbsize, int block, int overlap, int mode, int spatial, int temporal, float sbeta,
+                        float tbeta, int tmode, int tosize);
+
+std::vector<float> fft3d_profile(int w, int h, const float sigmas[4]);
+struct EnhancementTables {
+  std::vector<float> sharpen, halo;
+  float a = 0, b = 0, c = 0;
+};
+EnhancementTables enhancement_tables(int w, int h, float factor, float sharpen, float scutoff, float svr,
+                                     float smin, float smax, float dehalo, float hr, float ht);
+struct Curves {
+  std::vector<float> shared, x, y, time;
+  int system = 0;
+  bool empty() const { return shared.empty() && x.empty() && y.empty() && time.empty(); }
+};
+void validate_curves(const Curves& c);
+std::vector<float> dft_profile(const Curves& c, int T, int S, float sigma, float divisor);
+
+// Real transforms of one block (same butterflies as the kernels).
+std::vector<std::complex<float>> rfft2_host(int h, int w, const float* in);
+std::vector<std::complex<float>> rfft3_host(int t, int h, int w, const float* in);
+
+}  // namespace nf

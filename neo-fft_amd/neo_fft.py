@@ -1,0 +1,363 @@
+"""Python binding of the gfx950 spectral core (libneo_fft_amd.so, C-ABI in
+include/neo_fft.h).
+
+Planes are device buffers: torch CUDA tensors (2D, row-major; uint8, int16
+holding uint16 bits, or float32) or anything exposing data_ptr()/shape/
+stride()/element_size(). Parameter names, meanings and defaults follow the
+reference's public DFTTest / FFT3D functions (src/plugin/descriptors.hpp).
+The library must be built (neo-fft_amd/build.py); there is no fallback.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libneo_fft_amd.so")
+
+NF_OK, NF_EINVAL, NF_EUNSUPPORTED, NF_ENONFINITE, NF_ENOMEM, NF_EDEVICE = range(6)
+NF_FLAG_NONFINITE = 1
+
+c_float_p = ctypes.POINTER(ctypes.c_float)
+c_int_p = ctypes.POINTER(ctypes.c_int)
+
+
+class NeoFFTError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(message)
+        self.status = status
+
+
+class InvalidArgument(NeoFFTError, ValueError):
+    pass
+
+
+class Unsupported(NeoFFTError, NotImplementedError):
+    pass
+
+
+class NonFinite(NeoFFTError):
+    pass
+
+
+class SampleFormat(ctypes.Structure):
+    _fields_ = [("bits", ctypes.c_int), ("floating", ctypes.c_int), ("chroma", ctypes.c_int)]
+
+
+class DFTTestConfig(ctypes.Structure):
+    _fields_ = [("ftype", ctypes.c_int), ("sigma", ctypes.c_float), ("sigma2", ctypes.c_float),
+                ("pmin", ctypes.c_float), ("pmax", ctypes.c_float)] + \
+               [(n, ctypes.c_int) for n in ("sbsize", "smode", "sosize", "tbsize", "tmode", "tosize", "swin",
+                                            "twin")] + \
+               [("sbeta", ctypes.c_float), ("tbeta", ctypes.c_float), ("zmean", ctypes.c_int),
+                ("f0beta", ctypes.c_float), ("nlocation", c_int_p), ("n_nlocation", ctypes.c_int),
+                ("alpha", ctypes.c_float)] + \
+               [(n, c_float_p) for n in ("slocation", "ssx", "ssy", "sst")] + \
+               [(n, ctypes.c_int) for n in ("n_slocation", "n_ssx", "n_ssy", "n_sst", "ssystem", "dither",
+                                            "dither_seed", "opt")]
+
+
+class FFT3DConfig(ctypes.Structure):
+    _fields_ = [("sigma", ctypes.c_float), ("beta", ctypes.c_float)] + \
+               [(n, ctypes.c_int) for n in ("bw", "bh", "bt", "ow", "oh")] + \
+               [(n, ctypes.c_float) for n in ("kratio", "sharpen", "scutoff", "svr", "smin", "smax")] + \
+               [(n, ctypes.c_int) for n in ("interlaced", "wintype", "pframe", "px", "py", "pshow")] + \
+               [(n, ctypes.c_float) for n in ("pcutoff", "pfactor", "sigma2", "sigma3", "sigma4", "degrid",
+                                              "dehalo", "hr", "ht")] + \
+               [(n, ctypes.c_int) for n in ("l", "t", "r", "b", "opt", "kalman_warmup")]
+
+
+class Plane(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("stride_bytes", ctypes.c_longlong)]
+
+
+class MutPlane(ctypes.Structure):
+    _fields_ = Plane._fields_
+
+
+# Every exported symbol of include/neo_fft.h with its ctypes signature.
+_SIGNATURES = {
+    "nf_last_error": (ctypes.c_char_p, []),
+    "nf_dfttest_config_init": (None, [ctypes.POINTER(DFTTestConfig)]),
+    "nf_fft3d_config_init": (None, [ctypes.POINTER(FFT3DConfig)]),
+    "nf_dfttest_plan_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, SampleFormat,
+                                              ctypes.POINTER(DFTTestConfig), ctypes.POINTER(ctypes.c_void_p)]),
+    "nf_fft3d_plan_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, SampleFormat, ctypes.POINTER(FFT3DConfig),
+                                            ctypes.POINTER(ctypes.c_void_p)]),
+    "nf_plan_destroy": (None, [ctypes.c_void_p]),
+    "nf_plan_geometry": (ctypes.c_int, [ctypes.c_void_p, c_int_p]),
+    "nf_plan_temporal_size": (ctypes.c_int, [ctypes.c_void_p]),
+    "nf_plan_process": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Plane), ctypes.c_int, c_int_p, ctypes.c_int,
+                                       MutPlane, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "nf_plan_enqueue": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Plane), ctypes.c_int, c_int_p, ctypes.c_int,
+                                       MutPlane, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "nf_process_batch": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(Plane), c_int_p,
+                                        c_int_p, ctypes.POINTER(MutPlane), c_int_p, c_int_p, ctypes.c_void_p,
+                                        ctypes.c_void_p]),
+    "nf_kalman_state_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "nf_kalman_state_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "nf_kalman_state_destroy": (None, [ctypes.c_void_p]),
+    "nf_kalman_state_bytes": (ctypes.c_size_t, [ctypes.c_void_p]),
+    "nf_kalman_advance": (ctypes.c_int, [ctypes.c_void_p, Plane, ctypes.c_void_p, ctypes.c_void_p]),
+    "nf_kalman_render": (ctypes.c_int, [ctypes.c_void_p, Plane, MutPlane, ctypes.c_void_p, ctypes.c_void_p]),
+    "nf_kalman_state_download": (ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p, c_float_p]),
+    "nf_kernel_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
+    "nf_profile_enable": (ctypes.c_int, [ctypes.c_int]),
+    "nf_profile_read": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong),
+                                       ctypes.c_int]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the shared library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with neo-fft_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status == NF_OK:
+        return
+    msg = lib().nf_last_error().decode()
+    if status == NF_EINVAL:
+        raise InvalidArgument(status, msg)
+    if status == NF_EUNSUPPORTED:
+        raise Unsupported(status, msg)
+    if status == NF_ENONFINITE:
+        raise NonFinite(status, msg)
+    raise NeoFFTError(status, msg)
+
+
+def _float_array(values):
+    if values is None or len(values) == 0:
+        return None, 0
+    arr = (ctypes.c_float * len(values))(*[float(v) for v in values])
+    return arr, len(values)
+
+
+DFTTEST_DEFAULTS = dict(ftype=0, sigma=8.0, sigma2=8.0, pmin=0.0, pmax=500.0, sbsize=16, smode=1, sosize=12,
+                        tbsize=3, tmode=0, tosize=0, swin=0, twin=7, sbeta=2.5, tbeta=2.5, zmean=True, f0beta=1.0,
+                        nlocation=None, alpha=None, slocation=None, ssx=None, ssy=None, sst=None, ssystem=0,
+                        dither=0, dither_seed=0, opt=0)
+FFT3D_DEFAULTS = dict(sigma=2.0, beta=1.0, bw=32, bh=32, bt=3, ow=-1, oh=-1, kratio=2.0, sharpen=0.0, scutoff=0.3,
+                      svr=1.0, smin=4.0, smax=20.0, interlaced=False, wintype=0, pframe=0, px=0, py=0, pshow=False,
+                      pcutoff=0.1, pfactor=0.0, sigma2=None, sigma3=None, sigma4=None, degrid=1.0, dehalo=0.0,
+                      hr=2.0, ht=50.0, l=0, t=0, r=0, b=0, opt=0, kalman_warmup=8)
+
+
+def dfttest_config(**params):
+    unknown = set(params) - set(DFTTEST_DEFAULTS)
+    if unknown:
+        raise TypeError(f"DFTTest: unknown parameters {sorted(unknown)}")
+    p = dict(DFTTEST_DEFAULTS, **params)
+    c = DFTTestConfig()
+    lib().nf_dfttest_config_init(ctypes.byref(c))
+    for n in ("ftype", "sbsize", "smode", "sosize", "tbsize", "tmode", "tosize", "swin", "twin", "ssystem", "dither",
+              "dither_seed", "opt"):
+        setattr(c, n, int(p[n]))
+    for n in ("sigma", "sigma2", "pmin", "pmax", "sbeta", "tbeta", "f0beta"):
+        setattr(c, n, float(p[n]))
+    c.zmean = int(bool(p["zmean"]))
+    c.alpha = -1.0 if p["alpha"] is None else float(p["alpha"])
+    keep = []
+    for n in ("slocation", "ssx", "ssy", "sst"):
+        arr, cnt = _float_array(p[n])
+        if arr is not None:
+            keep.append(arr)
+            setattr(c, n, ctypes.cast(arr, c_float_p))
+        setattr(c, "n_" + n, cnt)
+    if p["nlocation"]:
+        loc = (ctypes.c_int * len(p["nlocation"]))(*[int(v) for v in p["nlocation"]])
+        keep.append(loc)
+        c.nlocation = ctypes.cast(loc, c_int_p)
+        c.n_nlocation = len(p["nlocation"])
+    c._keep = keep
+    return c
+
+
+def fft3d_config(**params):
+    unknown = set(params) - set(FFT3D_DEFAULTS)
+    if unknown:
+        raise TypeError(f"FFT3D: unknown parameters {sorted(unknown)}")
+    p = dict(FFT3D_DEFAULTS, **params)
+    c = FFT3DConfig()
+    lib().nf_fft3d_config_init(ctypes.byref(c))
+    for n in ("bw", "bh", "bt", "ow", "oh", "wintype", "pframe", "px", "py", "l", "t", "r", "b", "opt",
+              "kalman_warmup"):
+        setattr(c, n, int(p[n]))
+    for n in ("sigma", "beta", "kratio", "sharpen", "scutoff", "svr", "smin", "smax", "pcutoff", "pfactor", "degrid",
+              "dehalo", "hr", "ht"):
+        setattr(c, n, float(p[n]))
+    c.interlaced = int(bool(p["interlaced"]))
+    c.pshow = int(bool(p["pshow"]))
+    for n in ("sigma2", "sigma3", "sigma4"):
+        setattr(c, n, -1.0 if p[n] is None else float(p[n]))
+    return c
+
+
+def _plane(t, cls=Plane):
+    shape = tuple(t.shape)
+    if len(shape) != 2:
+        raise ValueError("planes are 2D")
+    stride = t.stride()[0] * t.element_size()
+    return cls(ctypes.c_void_p(t.data_ptr()), shape[1], shape[0], stride)
+
+
+def _stream_handle(stream):
+    if stream is None:
+        return None
+    return ctypes.c_void_p(getattr(stream, "cuda_stream", stream))
+
+
+class Plan:
+    """One immutable per-plane plan (reference Plan, src/algorithms/plan.hpp:49)."""
+
+    def __init__(self, handle, algorithm):
+        self._h = handle
+        self.algorithm = algorithm
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().nf_plan_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def geometry(self):
+        out = (ctypes.c_int * 14)()
+        _check(lib().nf_plan_geometry(self._h, out))
+        keys = ("length", "block", "overlap", "step", "count", "cover", "offset")
+        return dict(zip(keys, out[:7])), dict(zip(keys, out[7:]))
+
+    @property
+    def temporal_size(self):
+        return lib().nf_plan_temporal_size(self._h)
+
+    def process(self, sources, dst, targets=None, frame=0, plane=0, stream=None):
+        srcs = (Plane * len(sources))(*[_plane(s) for s in sources])
+        tg, nt = None, 0
+        if targets:
+            tg = (ctypes.c_int * len(targets))(*targets)
+            nt = len(targets)
+        _check(lib().nf_plan_process(self._h, srcs, len(sources), tg, nt, _plane(dst, MutPlane), frame, plane,
+                                     _stream_handle(stream)))
+
+    def enqueue(self, sources, dst, flags, targets=None, frame=0, plane=0, stream=None):
+        srcs = (Plane * len(sources))(*[_plane(s) for s in sources])
+        tg, nt = None, 0
+        if targets:
+            tg = (ctypes.c_int * len(targets))(*targets)
+            nt = len(targets)
+        _check(lib().nf_plan_enqueue(self._h, srcs, len(sources), tg, nt, _plane(dst, MutPlane), frame, plane,
+                                     ctypes.c_void_p(flags.data_ptr()), _stream_handle(stream)))
+
+
+def _fmt(bits, floating, chroma):
+    return SampleFormat(int(bits), int(bool(floating)), int(bool(chroma)))
+
+
+def dfttest_plan(width, height, bits=8, floating=False, chroma=False, **params):
+    cfg = dfttest_config(**params)
+    h = ctypes.c_void_p()
+    _check(lib().nf_dfttest_plan_create(width, height, _fmt(bits, floating, chroma), ctypes.byref(cfg),
+                                        ctypes.byref(h)))
+    return Plan(h, "DFTTest")
+
+
+def fft3d_plan(width, height, bits=8, floating=False, chroma=False, **params):
+    cfg = fft3d_config(**params)
+    h = ctypes.c_void_p()
+    _check(lib().nf_fft3d_plan_create(width, height, _fmt(bits, floating, chroma), ctypes.byref(cfg),
+                                      ctypes.byref(h)))
+    return Plan(h, "FFT3D")
+
+
+def process_batch(jobs, flags, stream=None):
+    """jobs: list of (plan, [sources], dst). One launch per group of jobs."""
+    n = len(jobs)
+    plans = (ctypes.c_void_p * n)(*[j[0].handle for j in jobs])
+    all_src = []
+    first = []
+    counts = []
+    for _, srcs, _ in jobs:
+        first.append(len(all_src))
+        counts.append(len(srcs))
+        all_src.extend(srcs)
+    src_arr = (Plane * len(all_src))(*[_plane(s) for s in all_src])
+    dst_arr = (MutPlane * n)(*[_plane(j[2], MutPlane) for j in jobs])
+    zeros = (ctypes.c_int * n)()
+    _check(lib().nf_process_batch(plans, n, src_arr, (ctypes.c_int * n)(*first), (ctypes.c_int * n)(*counts),
+                                  dst_arr, zeros, zeros, ctypes.c_void_p(flags.data_ptr()), _stream_handle(stream)))
+
+
+class KalmanState:
+    """Device Kalman state of one plane (reference KalmanState)."""
+
+    def __init__(self, plan):
+        self.plan = plan
+        h = ctypes.c_void_p()
+        _check(lib().nf_kalman_state_create(plan.handle, ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().nf_kalman_state_destroy(h)
+            self._h = None
+
+    @property
+    def nbytes(self):
+        return lib().nf_kalman_state_bytes(self._h)
+
+    def copy_to(self, other, stream=None):
+        _check(lib().nf_kalman_state_copy(self._h, other._h, _stream_handle(stream)))
+
+    def advance(self, source, stream=None):
+        _check(lib().nf_kalman_advance(self.plan.handle, _plane(source), self._h, _stream_handle(stream)))
+
+    def render(self, source, dst, stream=None):
+        _check(lib().nf_kalman_render(self.plan.handle, _plane(source), _plane(dst, MutPlane), self._h,
+                                      _stream_handle(stream)))
+
+    def download(self):
+        import numpy as np
+        bins = self.nbytes // 16
+        last = np.zeros(bins, np.complex64)
+        cov = np.zeros(bins, np.float32)
+        proc = np.zeros(bins, np.float32)
+        _check(lib().nf_kalman_state_download(self._h, last.ctypes.data_as(c_float_p), cov.ctypes.data_as(c_float_p),
+                                              proc.ctypes.data_as(c_float_p)))
+        return last, cov, proc
+
+
+KERNEL_CLASSES = ("dfttest_strip", "fft3d_block", "fft3d_synthesis", "other")
+
+
+def profile_enable(on=True):
+    _check(lib().nf_profile_enable(int(bool(on))))
+
+
+def profile_read(reset=True):
+    """{kernel class: (total_ms, launches)} from the hipEvent pairs."""
+    ms = (ctypes.c_double * 4)()
+    cnt = (ctypes.c_longlong * 4)()
+    _check(lib().nf_profile_read(ms, cnt, int(bool(reset))))
+    return {KERNEL_CLASSES[i]: (ms[i], cnt[i]) for i in range(4)}
+
+
+def kernel_info():
+    buf = ctypes.create_string_buffer(512)
+    _check(lib().nf_kernel_info(buf, 512))
+    return dict(kv.split("=", 1) for kv in buf.value.decode().split(";") if "=" in kv)

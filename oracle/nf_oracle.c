@@ -1,0 +1,1452 @@
+/*
+ * nf_oracle.c -- CPU restatement of neo-fft's DFTTest / FFT3D spectral core.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker of the GPU path, never shipped or
+ * called by it. Compiled with -O2 -fno-fast-math -ffp-contract=off, like the
+ * reference's core (CMakeLists.txt:119). Every function cites the reference
+ * file:line whose semantics it restates. Sources under src/ and docs/ of
+ * HomeOfAviSynthPlusEvolution/neo-fft v0.9.0.
+ *
+ * Parity status of each part (what pins it is in tests/test_oracle_*.py):
+ *   geometry, windows, tables, gain branches, temporal kernel, Kalman step,
+ *   dither, temporal lattice ............ pinned by the reference's KATs,
+ *                                         golden states and closed forms
+ *   8/16/32 transforms ................... pinned to a binary64 DFT at the
+ *                                         reference's codelet tolerances
+ *   other transform lengths .............. pinned to a binary64 DFT; the
+ *                                         reference's PocketFFT bits are
+ *                                         "parity unpinned"
+ *   plane pipelines ...................... pinned by identity, constant-gain,
+ *                                         worked-example and numpy-composition
+ *                                         checks
+ *   sampled noise models (nlocation,
+ *   pfactor, pshow) ...................... not restated here
+ */
+#include "nf_oracle.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static __thread char g_error[256];
+
+const char* nfo_last_error(void) { return g_error; }
+
+static int fail(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_error, sizeof g_error, fmt, ap);
+  va_end(ap);
+  return -1;
+}
+
+#define REQUIRE(cond, ...)            \
+  do {                                \
+    if (!(cond)) return fail(__VA_ARGS__); \
+  } while (0)
+
+/* finite() of src/base/checked.hpp:37-41 raises a frame error; here a flag. */
+static __thread int g_nonfinite;
+static inline float fin(float v) {
+  if (!isfinite(v)) g_nonfinite = 1;
+  return v;
+}
+
+static inline float fmaxf_ref(float a, float b) { return a < b ? b : a; } /* std::max */
+static inline float clampf_ref(float v, float lo, float hi) {             /* std::clamp */
+  return v < lo ? lo : (hi < v ? hi : v);
+}
+
+/* ------------------------------------------------------------------ geometry
+ * src/spectral/geometry.hpp:9-48 */
+static int admit(nfo_axis* a) {
+  REQUIRE(a->offset >= 0 && (long long)a->cover >= (long long)a->offset + a->length &&
+              a->offset < a->length && (long long)a->cover - a->offset - a->length < a->length,
+          "selected-plane padding exceeds one-reflection domain");
+  return 0;
+}
+
+int nfo_fft3d_axis(int length, int block, int overlap, nfo_axis* out) {
+  REQUIRE(length > 0, "invalid dimension");
+  REQUIRE(block >= 2 && length >= block, "FFT3D block must be >=2 and fit selected plane");
+  if (overlap < 0) overlap = block / 3;
+  REQUIRE(overlap <= block / 2, "FFT3D overlap exceeds half block");
+  const int step = block - overlap;
+  const long long num = (long long)length - overlap;
+  const int count = (int)((num + step - 1) / step + 2);
+  const int cover = count * step + overlap;
+  nfo_axis a = {length, block, overlap, step, count, cover, step};
+  if (admit(&a)) return -1;
+  *out = a;
+  return 0;
+}
+
+int nfo_dft_axis(int length, int block, int mode, int overlap, nfo_axis* out) {
+  REQUIRE(length > 0 && block > 0, "invalid dimension");
+  REQUIRE(mode == 0 || mode == 1, "DFTTest smode must be 0 or 1");
+  if (mode == 0) {
+    REQUIRE(block % 2 == 1, "DFTTest center mode requires odd sbsize");
+    const int c = block / 2;
+    nfo_axis a = {length, block, 0, 1, length, length + 2 * c, c};
+    if (admit(&a)) return -1;
+    *out = a;
+    return 0;
+  }
+  REQUIRE(overlap >= 0 && overlap < block, "DFTTest sosize outside block");
+  const int step = block - overlap;
+  REQUIRE(overlap <= block / 2 || block % step == 0, "DFTTest heavy overlap requires divisible step");
+  const int big = step > overlap ? step : overlap;
+  const int cover = block * ((length + block - 1) / block) + 2 * big;
+  nfo_axis a = {length, block, overlap, step, (cover - block) / step + 1, cover, (cover - length) / 2};
+  if (admit(&a)) return -1;
+  REQUIRE(a.offset >= block - step && (long long)a.offset + length <= (long long)a.count * step,
+          "DFTTest visible crop lacks complete phase coverage");
+  *out = a;
+  return 0;
+}
+
+int nfo_reflect(int c, const nfo_axis* a) {
+  const long long j = (long long)c - a->offset;
+  return (int)(j < 0 ? -j : j < a->length ? j : 2LL * a->length - 2 - j);
+}
+
+/* ------------------------------------------------------------------ windows
+ * src/algorithms/windows.cpp:6-25 */
+int nfo_fft3d_window(int block, int overlap, int type, float* an, float* syn) {
+  REQUIRE(block >= 2 && overlap >= 0 && overlap <= block / 2, "FFT3D window geometry invalid");
+  REQUIRE(type >= 0 && type <= 2, "FFT3D wintype outside 0..2");
+  for (int i = 0; i < block; ++i) an[i] = syn[i] = 1.0f;
+  const float pi = 3.14159265358979323846f;
+  for (int j = 0; j < overlap; ++j) {
+    const float left = cosf(pi * ((float)(j - overlap) + 0.5f) / (2.0f * (float)overlap));
+    const float right = cosf(pi * ((float)j + 0.5f) / (2.0f * (float)overlap));
+    for (int side = 0; side < 2; ++side) {
+      const float c = side ? right : left;
+      const int pos = side ? block - overlap + j : j;
+      const float a = type == 0 ? c : type == 1 ? sqrtf(c) : 1.0f;
+      an[pos] = a;
+      syn[pos] = type == 0 ? c : type == 1 ? (a * a) * a : c * c;
+    }
+  }
+  return 0;
+}
+
+/* windows.cpp:27-39 */
+static double i0a(double arg) {
+  double num = 1, den = 1, total = 1;
+  for (int k = 1; k <= 14; ++k) {
+    num *= arg / 2;
+    den *= k;
+    const double v = num / den;
+    total += v * v;
+    if (!isfinite(total)) return NAN;
+    if (v <= 1e-8) break;
+  }
+  return total;
+}
+
+/* windows.cpp:41-96 */
+double nfo_dft_raw_window(int id, int j, int length, float beta) {
+  if (!(id >= 0 && id <= 11 && length > 0 && j >= 0 && j < length && isfinite(beta) && beta >= 0)) {
+    fail("DFTTest window parameter invalid");
+    return NAN;
+  }
+  const double pi = 3.1415926535897932384626433832795;
+  const double n = j + 0.5, r = n / length, theta = 2 * pi * r;
+#define C(m) cos((m) * theta)
+  double value = 0;
+  switch (id) {
+    case 0: value = 0.5 - 0.5 * C(1); break;
+    case 1: value = 0.53836 - 0.46164 * C(1); break;
+    case 2: value = 0.42 - 0.5 * C(1) + 0.08 * C(2); break;
+    case 3: value = 0.35875 - 0.48829 * C(1) + 0.14128 * C(2) - 0.01168 * C(3); break;
+    case 4: value = i0a(pi * beta * sqrt(1 - (2 * r - 1) * (2 * r - 1))) / i0a(pi * beta); break;
+    case 5: {
+      static const double a[7] = {0.27105140069342415,   -0.433297939234486060, 0.218122999543110620,
+                                  -0.065925446388030898, 0.010811742098372268,  -7.7658482522509342e-4,
+                                  1.3887217350903198e-5};
+      value = a[0];
+      for (int m = 1; m < 7; ++m) value += a[m] * C(m);
+      break;
+    }
+    case 6: value = 0.2810639 - 0.5208972 * C(1) + 0.1980399 * C(2); break;
+    case 7: value = 1; break;
+    case 8: value = (2.0 / length) * (length / 2.0 - fabs(n - length / 2.0)); break;
+    case 9: value = 0.62 - 0.48 * (r - 0.5) - 0.38 * C(1); break;
+    case 10: value = 0.355768 - 0.487396 * C(1) + 0.144232 * C(2) - 0.012604 * C(3); break;
+    case 11: value = 0.3635819 - 0.4891775 * C(1) + 0.1365995 * C(2) - 0.0106411 * C(3); break;
+  }
+#undef C
+  if (!isfinite(value)) fail("DFTTest non-finite window");
+  return value;
+}
+
+/* windows.cpp:97-152 (with table.cpp:153-160 for the ordered binary32 energy) */
+int nfo_dft_window_3d(int T, int block, int overlap, int mode, int swin, int twin, float sbeta,
+                      float tbeta, int tmode, int tosize, float* h, float* wscale) {
+  REQUIRE(T > 0 && T <= 15, "DFTTest tbsize outside 1..15");
+  REQUIRE(tmode == 0 || tmode == 1, "DFTTest temporal window mode invalid");
+  REQUIRE(tmode == 0 || (tosize >= 0 && tosize < T), "DFTTest temporal overlap invalid");
+  REQUIRE(mode == 0 || mode == 1, "DFTTest window mode invalid");
+  REQUIRE(overlap >= 0 && overlap < block, "DFTTest window overlap invalid");
+  double* raw = (double*)malloc(sizeof(double) * block);
+  double* sw = (double*)malloc(sizeof(double) * block);
+  double tw[15], raw_time[15];
+  int rc = 0;
+  for (int j = 0; j < block; ++j) {
+    raw[j] = nfo_dft_raw_window(swin, j, block, sbeta);
+    if (!isfinite(raw[j])) { rc = -1; goto out; }
+    sw[j] = raw[j];
+  }
+  if (mode == 1) {
+    const int step = block - overlap;
+    for (int j = 0; j < block; ++j) {
+      double d = 0;
+      for (int k = j; k >= 0; k -= step) d += raw[k] * raw[k];
+      for (long long k = (long long)j + step; k < block; k += step) d += raw[k] * raw[k];
+      if (!(isfinite(d) && d > 0)) { rc = fail("DFTTest zero or invalid window denominator"); goto out; }
+      sw[j] = raw[j] / sqrt(d);
+    }
+  }
+  for (int z = 0; z < T; ++z) {
+    tw[z] = nfo_dft_raw_window(twin, z, T, tbeta);
+    if (!isfinite(tw[z])) { rc = -1; goto out; }
+  }
+  if (tmode == 1) {
+    for (int z = 0; z < T; ++z) raw_time[z] = tw[z];
+    const int step = T - tosize;
+    for (int z = 0; z < T; ++z) {
+      double energy = 0;
+      for (int k = z; k >= 0; k -= step) energy += raw_time[k] * raw_time[k];
+      for (int k = z + step; k < T; k += step) energy += raw_time[k] * raw_time[k];
+      if (!(isfinite(energy) && energy > 0)) { rc = fail("DFTTest temporal phase energy invalid"); goto out; }
+      tw[z] = raw_time[z] / sqrt(energy);
+    }
+  }
+  {
+    const double volume = (double)T * (double)block * (double)block;
+    const double inv_sqrt_v = 1.0 / sqrt(volume);
+    float energy = 0.0f;
+    for (int z = 0; z < T; ++z)
+      for (int y = 0; y < block; ++y) {
+        const double factor = tw[z] * sw[y];
+        float* row = h + ((size_t)z * block + y) * block;
+        for (int x = 0; x < block; ++x) {
+          const double v = factor * sw[x] * inv_sqrt_v;
+          if (!(isfinite(v) && fabs(v) <= 3.4028234663852886e38)) { rc = fail("DFTTest window overflow"); goto out; }
+          const float hv = (float)v;
+          row[x] = hv;
+          energy += hv * hv;
+        }
+      }
+    if (!(isfinite(energy) && energy > 0)) { rc = fail("DFTTest zero or invalid window energy"); goto out; }
+    *wscale = 1.0f / energy;
+    if (!(isfinite(*wscale) && *wscale > 0)) { rc = fail("DFTTest invalid wscale"); goto out; }
+  }
+out:
+  free(raw);
+  free(sw);
+  return rc;
+}
+
+/* ---------------------------------------------------------------- transforms
+ * Butterfly order of src/spectral/codelets/fft8x8_avx2.cpp:49-140,
+ * fft16x16_avx2.cpp:27-201 and fft32x32_avx2.cpp:251-275. */
+static float W16R[8], W16I[8], W32R[16], W32I[16];
+static int g_tw_ready;
+
+static void init_twiddles(void) {
+  if (g_tw_ready) return;
+  const float PI = 3.14159265358979323846f;
+  /* fft16x16_avx2.cpp:11-25: float cos/sin of float(k*PI/8), correctly rounded */
+  for (int k = 0; k < 8; ++k) {
+    const float arg = (float)k * PI / 8.0f;
+    W16R[k] = (float)cos((double)arg);
+    W16I[k] = (float)sin((double)arg);
+  }
+  for (int k = 0; k < 16; ++k) {
+    const float arg = (float)k * PI / 16.0f;
+    W32R[k] = (float)cos((double)arg);
+    W32I[k] = (float)sin((double)arg);
+  }
+  g_tw_ready = 1;
+}
+
+#define BF2(xr, xi, yr, yi)        \
+  do {                             \
+    const float ur_ = xr, ui_ = xi; \
+    const float vr_ = yr, vi_ = yi; \
+    xr = ur_ + vr_;                \
+    xi = ui_ + vi_;                \
+    yr = ur_ - vr_;                \
+    yi = ui_ - vi_;                \
+  } while (0)
+
+/* CMUL_SCALAR: fmsub / fmadd with one rounded product (fft16x16_avx2.cpp:64-69) */
+static inline void cmul(float* orr, float* oi, float ir, float ii, float wr, float wi_val, int fwd) {
+  const float wi = fwd ? -wi_val : wi_val;
+  const float p = ii * wi, q = ii * wr;
+  *orr = fmaf(ir, wr, -p);
+  *oi = fmaf(ir, wi, q);
+}
+static inline void mulj(float* orr, float* oi, float ir, float ii, int fwd) {
+  if (fwd) { *orr = ii; *oi = 0.0f - ir; }
+  else { *orr = 0.0f - ii; *oi = ir; }
+}
+
+static void fft8(float* r, float* i, int fwd) {
+  const float s = 0.70710678118654752440f;
+  static const int br[8] = {0, 4, 2, 6, 1, 5, 3, 7};
+  float ar[8], ai[8];
+  for (int k = 0; k < 8; ++k) { ar[k] = r[br[k]]; ai[k] = i[br[k]]; }
+  BF2(ar[0], ai[0], ar[1], ai[1]);
+  BF2(ar[2], ai[2], ar[3], ai[3]);
+  BF2(ar[4], ai[4], ar[5], ai[5]);
+  BF2(ar[6], ai[6], ar[7], ai[7]);
+  BF2(ar[0], ai[0], ar[2], ai[2]);
+  BF2(ar[4], ai[4], ar[6], ai[6]);
+  {
+    float tr, ti;
+    mulj(&tr, &ti, ar[3], ai[3], fwd); ar[3] = tr; ai[3] = ti;
+    mulj(&tr, &ti, ar[7], ai[7], fwd); ar[7] = tr; ai[7] = ti;
+  }
+  BF2(ar[1], ai[1], ar[3], ai[3]);
+  BF2(ar[5], ai[5], ar[7], ai[7]);
+  BF2(ar[0], ai[0], ar[4], ai[4]);
+  {
+    const float ur = ar[5], ui = ai[5];
+    if (fwd) { ar[5] = s * (ur + ui); ai[5] = s * (ui - ur); }
+    else { ar[5] = s * (ur - ui); ai[5] = s * (ui + ur); }
+  }
+  BF2(ar[1], ai[1], ar[5], ai[5]);
+  {
+    float tr, ti;
+    mulj(&tr, &ti, ar[6], ai[6], fwd); ar[6] = tr; ai[6] = ti;
+  }
+  BF2(ar[2], ai[2], ar[6], ai[6]);
+  {
+    const float ur = ar[7], ui = ai[7];
+    if (fwd) { ar[7] = s * (ui - ur); ai[7] = s * ((0.0f - ur) - ui); }
+    else { ar[7] = s * ((0.0f - ur) - ui); ai[7] = s * (ur - ui); }
+  }
+  BF2(ar[3], ai[3], ar[7], ai[7]);
+  for (int k = 0; k < 8; ++k) { r[k] = ar[k]; i[k] = ai[k]; }
+}
+
+static void fft16(float* r, float* i, int fwd) {
+  static const int br[16] = {0, 8, 4, 12, 2, 10, 6, 14, 1, 9, 5, 13, 3, 11, 7, 15};
+  float ar[16], ai[16];
+  for (int k = 0; k < 16; ++k) { ar[k] = r[br[k]]; ai[k] = i[br[k]]; }
+  for (int k = 0; k < 16; k += 2) BF2(ar[k], ai[k], ar[k + 1], ai[k + 1]);
+  for (int g = 0; g < 16; g += 4) { /* BUTTERFLY4 */
+    BF2(ar[g], ai[g], ar[g + 2], ai[g + 2]);
+    float vr, vi;
+    mulj(&vr, &vi, ar[g + 3], ai[g + 3], fwd);
+    const float ur = ar[g + 1], ui = ai[g + 1];
+    ar[g + 1] = ur + vr; ai[g + 1] = ui + vi;
+    ar[g + 3] = ur - vr; ai[g + 3] = ui - vi;
+  }
+  for (int g = 0; g < 16; g += 8) { /* BUTTERFLY8 */
+    BF2(ar[g], ai[g], ar[g + 4], ai[g + 4]);
+    float vr, vi, ur, ui;
+    cmul(&vr, &vi, ar[g + 5], ai[g + 5], W16R[2], W16I[2], fwd);
+    ur = ar[g + 1]; ui = ai[g + 1];
+    ar[g + 1] = ur + vr; ai[g + 1] = ui + vi; ar[g + 5] = ur - vr; ai[g + 5] = ui - vi;
+    mulj(&vr, &vi, ar[g + 6], ai[g + 6], fwd);
+    ur = ar[g + 2]; ui = ai[g + 2];
+    ar[g + 2] = ur + vr; ai[g + 2] = ui + vi; ar[g + 6] = ur - vr; ai[g + 6] = ui - vi;
+    cmul(&vr, &vi, ar[g + 7], ai[g + 7], W16R[6], W16I[6], fwd);
+    ur = ar[g + 3]; ui = ai[g + 3];
+    ar[g + 3] = ur + vr; ai[g + 3] = ui + vi; ar[g + 7] = ur - vr; ai[g + 7] = ui - vi;
+  }
+  BF2(ar[0], ai[0], ar[8], ai[8]);
+  for (int k = 1; k < 8; ++k) {
+    float vr, vi;
+    if (k == 4) mulj(&vr, &vi, ar[12], ai[12], fwd);
+    else cmul(&vr, &vi, ar[8 + k], ai[8 + k], W16R[k], W16I[k], fwd);
+    const float ur = ar[k], ui = ai[k];
+    ar[k] = ur + vr; ai[k] = ui + vi;
+    ar[8 + k] = ur - vr; ai[8 + k] = ui - vi;
+  }
+  for (int k = 0; k < 16; ++k) { r[k] = ar[k]; i[k] = ai[k]; }
+}
+
+static void fft32(float* r, float* i, int fwd) {
+  float er[16], ei[16], orr[16], oi[16];
+  for (int m = 0; m < 16; ++m) {
+    er[m] = r[2 * m]; ei[m] = i[2 * m];
+    orr[m] = r[2 * m + 1]; oi[m] = i[2 * m + 1];
+  }
+  fft16(er, ei, fwd);
+  fft16(orr, oi, fwd);
+  for (int k = 0; k < 16; ++k) {
+    float vr, vi;
+    cmul(&vr, &vi, orr[k], oi[k], W32R[k], W32I[k], fwd);
+    r[k] = er[k] + vr; i[k] = ei[k] + vi;
+    r[k + 16] = er[k] - vr; i[k + 16] = ei[k] - vi;
+  }
+}
+
+/* Direct DFT for lengths the codelets do not cover. Twiddles are binary64
+ * cos/sin rounded to float; products and sums in float, index order. */
+static void dft_generic(int n, float* r, float* i, int fwd) {
+  float* tr = (float*)malloc(sizeof(float) * 2 * n);
+  float* ti = tr + n;
+  const double pi = 3.1415926535897932384626433832795;
+  for (int k = 0; k < n; ++k) {
+    float sr = 0.0f, si = 0.0f;
+    for (int j = 0; j < n; ++j) {
+      const int e = (int)(((long long)j * k) % n);
+      const double ang = 2.0 * pi * (double)e / (double)n;
+      const float c = (float)cos(ang);
+      const float s = fwd ? -(float)sin(ang) : (float)sin(ang);
+      const float pr = r[j] * c - i[j] * s;
+      const float pi_ = r[j] * s + i[j] * c;
+      sr = sr + pr;
+      si = si + pi_;
+    }
+    tr[k] = sr;
+    ti[k] = si;
+  }
+  memcpy(r, tr, sizeof(float) * n);
+  memcpy(i, ti, sizeof(float) * n);
+  free(tr);
+}
+
+void nfo_cfft(int n, float* r, float* i, int fwd) {
+  init_twiddles();
+  if (n == 8) fft8(r, i, fwd);
+  else if (n == 16) fft16(r, i, fwd);
+  else if (n == 32) fft32(r, i, fwd);
+  else if (n > 1) dft_generic(n, r, i, fwd);
+}
+
+/* Real 2D forward: rows as complex transforms of real data keeping bins
+ * 0..w/2 (Nyquist imaginary part stored as 0), then columns
+ * (fft16x16_avx2.cpp:233-334). Output [h][w/2+1] interleaved complex. */
+void nfo_rfft2(int h, int w, const float* in, float* out) {
+  const int K = w / 2 + 1;
+  float stack_inter[2 * 64 * 33], stack_b[2 * 64];
+  const int small = h <= 64 && w <= 64;
+  float* inter = small ? stack_inter : (float*)malloc(sizeof(float) * 2 * h * K);
+  float* br = small ? stack_b : (float*)malloc(sizeof(float) * 2 * (w > h ? w : h));
+  float* bi = br + (w > h ? w : h);
+  for (int y = 0; y < h; ++y) {
+    for (int x = 0; x < w; ++x) { br[x] = in[(size_t)y * w + x]; bi[x] = 0.0f; }
+    nfo_cfft(w, br, bi, 1);
+    for (int k = 0; k < K; ++k) {
+      inter[((size_t)y * K + k) * 2] = br[k];
+      inter[((size_t)y * K + k) * 2 + 1] = (w % 2 == 0 && k == w / 2) ? 0.0f : bi[k];
+    }
+  }
+  for (int k = 0; k < K; ++k) {
+    for (int y = 0; y < h; ++y) { br[y] = inter[((size_t)y * K + k) * 2]; bi[y] = inter[((size_t)y * K + k) * 2 + 1]; }
+    nfo_cfft(h, br, bi, 1);
+    for (int y = 0; y < h; ++y) { out[((size_t)y * K + k) * 2] = br[y]; out[((size_t)y * K + k) * 2 + 1] = bi[y]; }
+  }
+  if (!small) { free(inter); free(br); }
+}
+
+/* Real 2D inverse: columns, then rows with Hermitian extension of bins
+ * 1..ceil(w/2)-1, real part times fct (fft16x16_avx2.cpp:336-448). */
+void nfo_irfft2(int h, int w, const float* in, float* out, float fct) {
+  const int K = w / 2 + 1;
+  float stack_inter[2 * 64 * 33], stack_b[2 * 64];
+  const int small = h <= 64 && w <= 64;
+  float* inter = small ? stack_inter : (float*)malloc(sizeof(float) * 2 * h * K);
+  float* br = small ? stack_b : (float*)malloc(sizeof(float) * 2 * (w > h ? w : h));
+  float* bi = br + (w > h ? w : h);
+  for (int k = 0; k < K; ++k) {
+    for (int y = 0; y < h; ++y) { br[y] = in[((size_t)y * K + k) * 2]; bi[y] = in[((size_t)y * K + k) * 2 + 1]; }
+    nfo_cfft(h, br, bi, 0);
+    for (int y = 0; y < h; ++y) { inter[((size_t)y * K + k) * 2] = br[y]; inter[((size_t)y * K + k) * 2 + 1] = bi[y]; }
+  }
+  for (int y = 0; y < h; ++y) {
+    for (int k = 0; k < K; ++k) { br[k] = inter[((size_t)y * K + k) * 2]; bi[k] = inter[((size_t)y * K + k) * 2 + 1]; }
+    for (int k = K; k < w; ++k) { br[k] = br[w - k]; bi[k] = 0.0f - bi[w - k]; }
+    nfo_cfft(w, br, bi, 0);
+    for (int x = 0; x < w; ++x) out[(size_t)y * w + x] = br[x] * fct;
+  }
+  if (!small) { free(inter); free(br); }
+}
+
+/* Temporal butterflies of src/spectral/fft_backend.cpp:397-420. */
+typedef struct { float r, i; } cpx;
+static inline cpx cadd(cpx a, cpx b) { cpx c = {a.r + b.r, a.i + b.i}; return c; }
+static inline cpx csub(cpx a, cpx b) { cpx c = {a.r - b.r, a.i - b.i}; return c; }
+static inline cpx cscale(cpx a, float s) { cpx c = {a.r * s, a.i * s}; return c; }
+static inline cpx crot(cpx z, int fwd) {
+  cpx c;
+  if (fwd) { c.r = z.i; c.i = -z.r; } else { c.r = -z.i; c.i = z.r; }
+  return c;
+}
+static const float K3 = 0.86602540378443864676f;
+static const float K5A = 0.30901699437494742410f, K5B = 0.80901699437494742410f;
+static const float K5C = 0.95105651629515357212f, K5D = 0.58778525229247312917f;
+
+static void temporal_butterfly(int n, cpx* v, int fwd) {
+  if (n == 3) {
+    const cpx sum = cadd(v[1], v[2]), diff = csub(v[1], v[2]);
+    const cpx center = csub(v[0], cscale(sum, 0.5f));
+    const cpx side = crot(cscale(diff, K3), fwd);
+    v[0] = cadd(v[0], sum);
+    v[1] = cadd(center, side);
+    v[2] = csub(center, side);
+  } else if (n == 4) {
+    const cpx es = cadd(v[0], v[2]), ed = csub(v[0], v[2]);
+    const cpx os = cadd(v[1], v[3]), od = crot(csub(v[1], v[3]), fwd);
+    v[0] = cadd(es, os);
+    v[2] = csub(es, os);
+    v[1] = cadd(ed, od);
+    v[3] = csub(ed, od);
+  } else {
+    const cpx a = cadd(v[1], v[4]), b = cadd(v[2], v[3]), c = csub(v[1], v[4]), d = csub(v[2], v[3]);
+    const cpx base1 = csub(cadd(v[0], cscale(a, K5A)), cscale(b, K5B));
+    const cpx base2 = cadd(csub(v[0], cscale(a, K5B)), cscale(b, K5A));
+    const cpx side1 = crot(cadd(cscale(c, K5C), cscale(d, K5D)), fwd);
+    const cpx side2 = crot(csub(cscale(c, K5D), cscale(d, K5C)), fwd);
+    v[0] = cadd(cadd(v[0], a), b);
+    v[1] = cadd(base1, side1);
+    v[4] = csub(base1, side1);
+    v[2] = cadd(base2, side2);
+    v[3] = csub(base2, side2);
+  }
+}
+
+/* Direct temporal DFT for depths without butterflies. */
+static void temporal_generic(int n, cpx* v, int fwd) {
+  cpx t[15];
+  float r[15], i[15];
+  for (int k = 0; k < n; ++k) { r[k] = v[k].r; i[k] = v[k].i; }
+  dft_generic(n, r, i, fwd);
+  for (int k = 0; k < n; ++k) { t[k].r = r[k]; t[k].i = i[k]; v[k] = t[k]; }
+}
+
+/* 3D forward: spatial transforms per slice then the temporal axis
+ * (fft_backend.cpp:618-656). Output [t][h][w/2+1]. */
+void nfo_rfft3(int t, int h, int w, const float* in, float* out) {
+  const int K = w / 2 + 1;
+  const size_t bins2 = (size_t)h * K;
+  for (int z = 0; z < t; ++z) nfo_rfft2(h, w, in + (size_t)z * h * w, out + 2 * (size_t)z * bins2);
+  if (t == 1) return;
+  for (size_t b = 0; b < bins2; ++b) {
+    cpx v[15];
+    for (int z = 0; z < t; ++z) { v[z].r = out[2 * ((size_t)z * bins2 + b)]; v[z].i = out[2 * ((size_t)z * bins2 + b) + 1]; }
+    if (t >= 3 && t <= 5) temporal_butterfly(t, v, 1);
+    else temporal_generic(t, v, 1);
+    for (int z = 0; z < t; ++z) { out[2 * ((size_t)z * bins2 + b)] = v[z].r; out[2 * ((size_t)z * bins2 + b) + 1] = v[z].i; }
+  }
+}
+
+/* One temporal slice of the 3D inverse (fft_backend.cpp:526-586): temporal
+ * inverse for the slice (centre shortcut for T=3,5), then the spatial
+ * inverse with fct = 1/(T*H*W). */
+void nfo_irfft3_slice(int t, int h, int w, int slice, const float* in, float* out, float fct) {
+  const int K = w / 2 + 1;
+  const size_t bins2 = (size_t)h * K;
+  float stack_sel[2 * 64 * 33];
+  float* sel = bins2 <= 64 * 33 ? stack_sel : (float*)malloc(sizeof(float) * 2 * bins2);
+  for (size_t b = 0; b < bins2; ++b) {
+    cpx v[15];
+    for (int z = 0; z < t; ++z) { v[z].r = in[2 * ((size_t)z * bins2 + b)]; v[z].i = in[2 * ((size_t)z * bins2 + b) + 1]; }
+    cpx res;
+    if (t == 1) res = v[0];
+    else if ((t == 3 || t == 5) && slice == t / 2) {
+      if (t == 3) {
+        const cpx sum = cadd(v[1], v[2]), diff = csub(v[1], v[2]);
+        const cpx center = csub(v[0], cscale(sum, 0.5f));
+        const cpx side = cscale(diff, K3);
+        const cpx rot = {-side.i, side.r};
+        res = cadd(center, rot);
+      } else {
+        const cpx a = cadd(v[1], v[4]), b2 = cadd(v[2], v[3]), c = csub(v[1], v[4]), d = csub(v[2], v[3]);
+        const cpx base = cadd(csub(v[0], cscale(a, K5B)), cscale(b2, K5A));
+        const cpx side = csub(cscale(c, K5D), cscale(d, K5C));
+        const cpx rot = {-side.i, side.r};
+        res = cadd(base, rot);
+      }
+    } else if (t >= 3 && t <= 5) {
+      temporal_butterfly(t, v, 0);
+      res = v[slice];
+    } else {
+      temporal_generic(t, v, 0);
+      res = v[slice];
+    }
+    sel[2 * b] = res.r;
+    sel[2 * b + 1] = res.i;
+  }
+  nfo_irfft2(h, w, sel, out, fct);
+  if (sel != stack_sel) free(sel);
+}
+
+/* ------------------------------------------------------------------- tables
+ * src/algorithms/fft3d_profile.cpp:31-46 with kernels/scalar/table.cpp:5-16 */
+int nfo_fft3d_profile(int W, int H, const float* s, float* out) {
+  REQUIRE(W >= 2 && H >= 2, "FFT3D profile shape invalid");
+  for (int k = 0; k < 4; ++k) REQUIRE(isfinite(s[k]) && s[k] >= 0, "FFT3D invalid scaled sigma");
+  const int K = W / 2 + 1;
+  const float norm = 1 / ((float)W * (float)H);
+  const float a = sqrtf(.5f) / 4, b = sqrtf(.5f) / 2;
+  g_nonfinite = 0;
+  for (int y = 0; y < H; ++y) {
+    const int dy = y - H / 2;
+    const float fy = (float)(H - 2 * (dy < 0 ? -dy : dy)) / (float)H;
+    const float other = fy * fy;
+    for (int x = 0; x < K; ++x) {
+      const float fx = (float)x / (float)K;
+      const float f = fin(sqrtf(fin(fin(other + fin(fx * fx)) / 2.0f)));
+      const float v = f < a   ? fin(s[3] + fin(fin((s[2] - s[3]) * f) / a))
+                      : f < b ? fin(s[2] + fin(fin((s[1] - s[2]) * (f - a)) / (b - a)))
+                              : fin(s[0] + fin(fin((s[1] - s[0]) * (1 - f)) / (1 - b)));
+      out[(size_t)y * K + x] = fin(fin(v * v) / norm);
+    }
+  }
+  REQUIRE(!g_nonfinite, "non-finite sample or intermediate");
+  return 0;
+}
+
+/* fft3d_profile.cpp:50-86 with table.cpp:39-55 */
+int nfo_enhancement_tables(int W, int H, float factor, float sharpen, float scutoff, float svr,
+                           float smin, float smax, float dehalo, float hr, float ht, float* sh,
+                           float* halo, float* abc) {
+  REQUIRE(W >= 2 && H >= 2 && isfinite(factor) && factor > 0, "FFT3D invalid enhancement geometry");
+  abc[0] = abc[1] = abc[2] = 0;
+  if (sharpen == 0 && dehalo == 0) return 0;
+  g_nonfinite = 0;
+  const float norm = 1 / ((float)W * (float)H);
+  float cutoff = 1;
+  if (sharpen != 0) {
+    const float lo = fin(smin * factor), hi = fin(smax * factor);
+    abc[0] = fin(fin(lo * lo) / norm);
+    abc[1] = fin(fin(hi * hi) / norm);
+    cutoff = fin(fin(2 * scutoff) * scutoff);
+    REQUIRE(cutoff > 0, "FFT3D sharpen cutoff underflow");
+  }
+  if (dehalo != 0) abc[2] = fin(fin(ht * ht) / norm);
+  const int K = W / 2 + 1;
+  for (int y = 0; y < H; ++y) {
+    const float dy = (float)(y < H / 2 ? y : H - y);
+    const float y2 = fin(fin(fin(dy * dy) * svr) * svr) / ((float)(H / 2) * (float)(H / 2));
+    for (int x = 0; x < K; ++x) {
+      const float x2 = (float)x * (float)x / ((float)(W / 2) * (float)(W / 2));
+      const float d2 = fin(y2 + x2);
+      const size_t k = (size_t)y * K + x;
+      if (sharpen != 0) sh[k] = fin(1 - expf(fin(-d2 / cutoff)));
+      if (dehalo != 0) {
+        const float first = fin(fin(fin(-.7f * d2) * hr) * hr);
+        const float second = fin(fin(-d2 * hr) * hr);
+        halo[k] = fin(expf(first) - expf(second));
+      }
+    }
+  }
+  REQUIRE(!g_nonfinite, "non-finite sample or intermediate");
+  if (dehalo != 0) {
+    float maximum = 0;
+    for (int k = 0; k < H * K; ++k) maximum = fmaxf_ref(maximum, halo[k]);
+    REQUIRE(isfinite(maximum) && maximum > 0, "FFT3D degenerate dehalo window");
+    for (int k = 0; k < H * K; ++k) halo[k] = fin(halo[k] / maximum);
+  }
+  return 0;
+}
+
+/* fft3d_profile.cpp:87-99 */
+static float enhancement_gain(float q, size_t k, float sharpen, float dehalo, float a, float b, float c,
+                              const float* sw, const float* hw) {
+  float s = 1, d = 1;
+  if (sharpen != 0 && b != 0) {
+    const float num = fin(q * b), den = fin(fin(q + a) * fin(q + b));
+    const float strength = fin(sharpen * sw[k]);
+    s = fin(1 + fin(strength * fin(sqrtf(fin(num / den)))));
+  }
+  if (dehalo != 0) {
+    const float qc = fin(q + c);
+    d = fin(qc / fin(qc + fin(fin(dehalo * hw[k]) * q)));
+  }
+  return fin(s * d);
+}
+
+/* src/algorithms/profile.cpp:123-215 */
+typedef struct { float p, v; } knot;
+static int knot_cmp(const void* a, const void* b) {
+  const knot* x = (const knot*)a;
+  const knot* y = (const knot*)b;
+  if (x->p < y->p) return -1;
+  if (y->p < x->p) return 1;
+  if (x->v < y->v) return -1;
+  if (y->v < x->v) return 1;
+  return 0;
+}
+static int prepare_curve(const float* vals, int n, float sigma, float exponent, knot* out, int* count) {
+  REQUIRE(n % 2 == 0, "DFTTest curve requires frequency/value pairs");
+  int m = n / 2;
+  for (int i = 0; i < m; ++i) {
+    REQUIRE(isfinite(vals[2 * i]) && vals[2 * i] >= 0 && vals[2 * i] <= 1, "DFTTest curve position outside 0..1");
+    REQUIRE(isfinite(vals[2 * i + 1]) && vals[2 * i + 1] >= 0, "DFTTest invalid curve value");
+    out[i].p = vals[2 * i];
+    out[i].v = vals[2 * i + 1];
+  }
+  qsort(out, (size_t)m, sizeof(knot), knot_cmp);
+  if (m) {
+    REQUIRE(out[0].p == 0 && out[m - 1].p == 1, "DFTTest curve requires endpoints 0 and 1");
+    for (int i = 1; i < m; ++i) REQUIRE(out[i - 1].p < out[i].p, "DFTTest duplicate curve position after binary32 conversion");
+  } else {
+    out[0].p = 0; out[0].v = sigma;
+    out[1].p = 1; out[1].v = sigma;
+    m = 2;
+  }
+  for (int i = 0; i < m; ++i) out[i].v = fin(powf(out[i].v, exponent));
+  *count = m;
+  return 0;
+}
+/* table.cpp:17-30 (lower_bound on positions) */
+static int curve_eval(const knot* kn, int m, float f, float divisor, float* out) {
+  int k = 0;
+  while (k < m && kn[k].p < f) ++k;
+  REQUIRE(k < m, "DFTTest frequency outside curve");
+  float value;
+  if (kn[k].p == f) value = kn[k].v;
+  else {
+    REQUIRE(k > 0, "DFTTest frequency outside curve");
+    const float t = fin((f - kn[k - 1].p) / (kn[k].p - kn[k - 1].p));
+    value = fin(fin(kn[k - 1].v * (1 - t)) + fin(kn[k].v * t));
+  }
+  *out = fin(value / divisor);
+  return 0;
+}
+static float frequency(int i, int length) {
+  return length == 1 ? 0 : (float)(i < length - i ? i : length - i) / (float)(length / 2);
+}
+
+int nfo_dft_profile(const nfo_dft_cfg* c, int T, int S, float sigma, float divisor, float* out) {
+  REQUIRE(c->ssystem == 0 || c->ssystem == 1, "DFTTest ssystem outside 0..1");
+  REQUIRE(T > 0 && S > 0 && isfinite(sigma) && sigma >= 0 && isfinite(divisor) && divisor > 0,
+          "DFTTest invalid profile shape or scale");
+  const int K = S / 2 + 1;
+  const size_t total = (size_t)T * S * K;
+  g_nonfinite = 0;
+  const int empty = !c->n_slocation && !c->n_ssx && !c->n_ssy && !c->n_sst;
+  if (empty) {
+    const float v = fin(sigma / divisor);
+    for (size_t i = 0; i < total; ++i) out[i] = v;
+    REQUIRE(!g_nonfinite, "non-finite sample or intermediate");
+    return 0;
+  }
+  const int dims = (T > 1 ? 1 : 0) + (S > 1 ? 2 : 0);
+  const int shared = c->n_slocation > 0;
+  const float exponent = dims == 0 || (shared && c->ssystem == 1) ? 1.0f : 1.0f / (float)dims;
+  knot kt[64], ky[64], kx[64];
+  int nt, ny, nx;
+  REQUIRE(c->n_slocation <= 126 && c->n_ssx <= 126 && c->n_ssy <= 126 && c->n_sst <= 126, "DFTTest curve too long");
+  if (prepare_curve(shared ? c->slocation : c->sst, shared ? c->n_slocation : c->n_sst, sigma, exponent, kt, &nt)) return -1;
+  if (dims == 0) {
+    if (curve_eval(kt, nt, 0, divisor, &out[0])) return -1;
+    return g_nonfinite ? fail("non-finite sample or intermediate") : 0;
+  }
+  if (prepare_curve(shared ? c->slocation : c->ssy, shared ? c->n_slocation : c->n_ssy, sigma, exponent, ky, &ny)) return -1;
+  if (prepare_curve(shared ? c->slocation : c->ssx, shared ? c->n_slocation : c->n_ssx, sigma, exponent, kx, &nx)) return -1;
+  if (c->ssystem == 1) {
+    for (int z = 0; z < T; ++z)
+      for (int j = 0; j < S; ++j) {
+        const float ftz = frequency(z, T), fyj = frequency(j, S);
+        const float other = ftz * ftz + fyj * fyj;
+        for (int i = 0; i < K; ++i) {
+          const float fx = frequency(i, S);
+          const float r = fin(sqrtf(fin(fin(other + fin(fx * fx)) / (float)dims)));
+          if (curve_eval(kt, nt, r, divisor, &out[((size_t)z * S + j) * K + i])) return -1;
+        }
+      }
+  } else {
+    float vt[15], vy[64], vx[33];
+    REQUIRE(S <= 64, "DFTTest curve block too large");
+    if (T == 1) vt[0] = 1;
+    else for (int z = 0; z < T; ++z) if (curve_eval(kt, nt, frequency(z, T), 1, &vt[z])) return -1;
+    if (S == 1) { vy[0] = 1; vx[0] = 1; }
+    else {
+      for (int j = 0; j < S; ++j) if (curve_eval(ky, ny, frequency(j, S), 1, &vy[j])) return -1;
+      for (int i = 0; i < K; ++i) if (curve_eval(kx, nx, frequency(i, S), 1, &vx[i])) return -1;
+    }
+    for (int z = 0; z < T; ++z)
+      for (int j = 0; j < S; ++j) {
+        const float factor = fin(vt[z] * vy[j]);
+        for (int i = 0; i < K; ++i) out[((size_t)z * S + j) * K + i] = fin(fin(factor * vx[i]) / divisor);
+      }
+  }
+  REQUIRE(!g_nonfinite, "non-finite sample or intermediate");
+  return 0;
+}
+
+/* ---------------------------------------------------------- gain operators
+ * src/kernels/scalar/spectral.cpp:5-53 */
+int nfo_spectral(float* x, const float* grid, int count, float scale, int type, float a_, float b,
+                 float low, float high, float exponent, float floor_, const float* primary,
+                 float sharpen, float dehalo, float ea, float eb, float ec, const float* sw,
+                 const float* hw) {
+  g_nonfinite = 0;
+  const int enh = sharpen != 0 || dehalo != 0;
+  for (int k = 0; k < count; ++k) {
+    const float a = primary ? primary[k] : a_;
+    const float mr = grid ? fin(scale * grid[2 * k]) : 0;
+    const float mi = grid ? fin(scale * grid[2 * k + 1]) : 0;
+    const float re = fin(x[2 * k] - mr), im = fin(x[2 * k + 1] - mi);
+    const float power = fin(re * re + im * im);
+    float gain = 0;
+    switch (type) {
+      case -2: gain = 1; break;
+      case -1: {
+        const float q = power + 1e-15f;
+        gain = fmaxf_ref((q - a) / q, floor_);
+        break;
+      }
+      case 0: {
+        const float wiener = fmaxf_ref((power - a) / (power + 1e-15f), 0.0f);
+        gain = fabsf(exponent - 1.0f) < 0.00005f   ? wiener
+               : fabsf(exponent - 0.5f) < 0.00005f ? sqrtf(wiener)
+                                                   : powf(wiener, exponent);
+        break;
+      }
+      case 1: gain = power < a ? 0.0f : 1.0f; break;
+      case 2: gain = a; break;
+      case 3: gain = power >= low && power <= high ? a : b; break;
+      case 4: {
+        const float q = power + 1e-15f;
+        const float num = fin(q * high), den = fin(fin(q + low) * fin(q + high));
+        gain = a * sqrtf(fin(num / den));
+        break;
+      }
+      default: return fail("invalid spectral filter type");
+    }
+    if (enh) gain = fin(gain * enhancement_gain(fin(power + 1e-15f), (size_t)k, sharpen, dehalo, ea, eb, ec, sw, hw));
+    fin(gain);
+    x[2 * k] = fin(gain * re + mr);
+    x[2 * k + 1] = fin(gain * im + mi);
+  }
+  REQUIRE(!g_nonfinite, "non-finite sample or intermediate");
+  return 0;
+}
+
+/* spectral.cpp:67-88 (twiddles) and :90-160 (temporal_prevalidated) */
+int nfo_fft3d_temporal(const float* const* spectra, int T, int c, int bins, float degrid,
+                       const float* grid, float noise_uniform, const float* noise_table,
+                       float noise_multiplier, float lower, float* out) {
+  REQUIRE(T >= 1 && T <= 5, "FFT3D invalid T");
+  REQUIRE(c >= 0 && c < T, "FFT3D invalid c");
+  g_nonfinite = 0;
+#define NOISE(k) (noise_table ? fin(noise_table[k] * noise_multiplier) : noise_uniform)
+  if (T == 1) {
+    const float mr_scale = (grid && degrid != 0.0f && grid[0] != 0.0f) ? fin((degrid * spectra[0][0]) / grid[0]) : 0.0f;
+    for (int k = 0; k < bins; ++k) {
+      const float mr = grid ? fin(mr_scale * grid[2 * k]) : 0.0f;
+      const float mi = grid ? fin(mr_scale * grid[2 * k + 1]) : 0.0f;
+      const float re = fin(spectra[0][2 * k] - mr);
+      const float im = fin(spectra[0][2 * k + 1] - mi);
+      const float power = fin(re * re + im * im);
+      const float q = power + 1e-15f;
+      const float gain = fmaxf_ref((q - NOISE(k)) / q, lower);
+      out[2 * k] = fin(gain * re + mr);
+      out[2 * k + 1] = fin(gain * im + mi);
+    }
+    REQUIRE(!g_nonfinite, "non-finite sample or intermediate");
+    return 0;
+  }
+  const double kPi = 3.1415926535897932384626433832795;
+  cpx fwd[5][5], inv[5];
+  for (int m = 0; m < T; ++m)
+    for (int j = 0; j < T; ++j) {
+      const double angle = -2.0 * kPi * (double)(j * m) / (double)T;
+      fwd[m][j].r = (float)cos(angle);
+      fwd[m][j].i = (float)sin(angle);
+    }
+  for (int m = 0; m < T; ++m) {
+    const double angle = +2.0 * kPi * (double)(c * m) / (double)T;
+    inv[m].r = (float)cos(angle);
+    inv[m].i = (float)sin(angle);
+  }
+  const float g_ratio = (grid && degrid != 0.0f && grid[0] != 0.0f) ? fin((degrid * spectra[c][0]) / grid[0]) : 0.0f;
+  const float inv_T = 1.0f / (float)T;
+  for (int k = 0; k < bins; ++k) {
+    cpx M = {0.0f, 0.0f};
+    if (grid) { M.r = g_ratio * grid[2 * k]; M.i = g_ratio * grid[2 * k + 1]; }
+    const cpx gridT = {M.r * (float)T, M.i * (float)T};
+    cpx F[5];
+    for (int m = 0; m < T; ++m) {
+      cpx sum = {0.0f, 0.0f};
+      for (int j = 0; j < T; ++j) {
+        const float xr = spectra[j][2 * k], xi = spectra[j][2 * k + 1];
+        const float pr = xr * fwd[m][j].r - xi * fwd[m][j].i;
+        const float pi_ = xr * fwd[m][j].i + xi * fwd[m][j].r;
+        sum.r = sum.r + pr;
+        sum.i = sum.i + pi_;
+      }
+      F[m] = sum;
+    }
+    F[0].r = F[0].r - gridT.r;
+    F[0].i = F[0].i - gridT.i;
+    const float noise = NOISE(k);
+    for (int m = 0; m < T; ++m) {
+      const float power = fin(F[m].r * F[m].r + F[m].i * F[m].i);
+      const float q = power + 1e-15f;
+      const float gain = fmaxf_ref((q - noise) / q, lower);
+      F[m].r = gain * F[m].r;
+      F[m].i = gain * F[m].i;
+    }
+    F[0].r = F[0].r + gridT.r;
+    F[0].i = F[0].i + gridT.i;
+    cpx y = {0.0f, 0.0f};
+    for (int m = 0; m < T; ++m) {
+      const float pr = F[m].r * inv[m].r - F[m].i * inv[m].i;
+      const float pi_ = F[m].r * inv[m].i + F[m].i * inv[m].r;
+      y.r = y.r + pr;
+      y.i = y.i + pi_;
+    }
+    out[2 * k] = y.r * inv_T;
+    out[2 * k + 1] = y.i * inv_T;
+  }
+#undef NOISE
+  REQUIRE(!g_nonfinite, "non-finite sample or intermediate");
+  return 0;
+}
+
+/* src/kernels/scalar/kalman.cpp:3-24 */
+int nfo_kalman_step(const float* x, float* last, float* cov, float* proc, const float* pattern,
+                    float uniform, float ratio2, int count) {
+  g_nonfinite = 0;
+  for (int k = 0; k < count; ++k) {
+    const float r = pattern ? fmaxf_ref(fin(pattern[k]), 1e-15f) : fin(uniform);
+    const float threshold = fin(r * ratio2);
+    const float xr = fin(x[2 * k]), xi = fin(x[2 * k + 1]);
+    const float lr = fin(last[2 * k]), li = fin(last[2 * k + 1]);
+    const float c = fin(cov[k]), q = fin(proc[k]);
+    const float dr = xr - lr, di = xi - li;
+    if ((!pattern && r == 0) || dr * dr > threshold || di * di > threshold) {
+      last[2 * k] = x[2 * k];
+      last[2 * k + 1] = x[2 * k + 1];
+      cov[k] = proc[k] = r;
+      continue;
+    }
+    const float sum = fin(c + q);
+    const float gain = fin(sum / fin(sum + r));
+    const float complement = 1 - gain;
+    const float nq = fin((gain * gain) * r), nc = fin(complement * sum);
+    const float nr = fin(fin(gain * xr) + fin(complement * lr));
+    const float ni = fin(fin(gain * xi) + fin(complement * li));
+    last[2 * k] = nr;
+    last[2 * k + 1] = ni;
+    cov[k] = nc;
+    proc[k] = nq;
+  }
+  REQUIRE(!g_nonfinite, "non-finite sample or intermediate");
+  return 0;
+}
+
+/* ------------------------------------------------------------------- dither
+ * src/kernels/scalar/dither.cpp:4-41, rows.cpp:6-8 */
+static uint32_t mix32(uint32_t v) {
+  v ^= v >> 16;
+  v *= 0x7feb352du;
+  v ^= v >> 15;
+  v *= 0x846ca68bu;
+  return v ^ (v >> 16);
+}
+uint32_t nfo_dither_hash(uint32_t seed, uint32_t n, uint32_t p, uint32_t y, uint32_t x) {
+  return mix32(mix32(mix32(mix32(mix32(seed ^ 0xa511e9b3u) ^ n) ^ p) ^ y) ^ x);
+}
+
+int nfo_dither(const float* src, int w, int h, long long sstride, uint8_t* dst, long long dstride,
+               int mode, int seed, int frame, int plane) {
+  REQUIRE(mode >= 1 && seed >= 0 && frame >= 0 && plane >= 0, "invalid dither coordinates or mode");
+  float* cur = (float*)calloc((size_t)w, sizeof(float));
+  float* nxt = (float*)calloc((size_t)w, sizeof(float));
+  const float scale = (float)(mode - 1) + .5f, off = scale * .5f;
+  g_nonfinite = 0;
+  for (int y = 0; y < h; ++y) {
+    memset(nxt, 0, sizeof(float) * (size_t)w);
+    for (int x = 0; x < w; ++x) {
+      const float e = fin(src[(size_t)y * sstride + x]);
+      float v;
+      if (mode == 1) v = fin(fin(e + cur[x]) + .5f);
+      else {
+        const float u = (float)(nfo_dither_hash((uint32_t)seed, (uint32_t)frame, (uint32_t)plane, (uint32_t)y, (uint32_t)x) >> 8) * 0x1p-24f;
+        v = fin(fin(fin(fin(e + u * scale) - off) + cur[x]) + .5f);
+      }
+      const uint8_t d = (uint8_t)(v <= 0 ? 0 : v >= 255 ? 255 : v);
+      dst[(size_t)y * dstride + x] = d;
+      const float error = fin(e - (float)d);
+      if (x > 0) nxt[x - 1] = fin(nxt[x - 1] + error * .1875f);
+      nxt[x] = fin(nxt[x] + error * .3125f);
+      if (x + 1 < w) {
+        cur[x + 1] = fin(cur[x + 1] + error * .4375f);
+        nxt[x + 1] = fin(nxt[x + 1] + error * .0625f);
+      }
+    }
+    float* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  free(cur);
+  free(nxt);
+  REQUIRE(!g_nonfinite, "non-finite sample or intermediate");
+  return 0;
+}
+
+/* ---------------------------------------------------- temporal lattice
+ * src/algorithms/temporal_grid.hpp:5-40 */
+static long long floor_div(long long value, int divisor) { return value / divisor - (value % divisor < 0); }
+
+int nfo_temporal_blocks(int frame, int length, int size, int overlap, long long* starts, int* targets, int* slots) {
+  REQUIRE(size >= 1 && size <= 15, "DFTTest tbsize outside 1..15");
+  REQUIRE(overlap >= 0 && overlap < size, "DFTTest invalid tosize");
+  REQUIRE(overlap <= size / 2 || size % (size - overlap) == 0, "DFTTest temporal heavy overlap requires divisible step");
+  REQUIRE(length >= size && frame >= 0 && frame < length, "invalid temporal clip/index");
+  const int step = size - overlap;
+  const long long anchor = -(long long)(step > overlap ? step : overlap), n = frame;
+  long long first = floor_div(n - size - anchor, step) + 1;
+  if (first < 0) first = 0;
+  const long long last = floor_div(n - anchor, step);
+  int count = 0;
+  for (long long j = first; j <= last; ++j) {
+    const long long start = anchor + j * step;
+    starts[count] = start;
+    targets[count] = (int)(n - start);
+    for (int z = 0; z < size; ++z) {
+      long long s = start + z;
+      s = s < 0 ? 0 : s > (long long)length - 1 ? (long long)length - 1 : s;
+      slots[count * size + z] = (int)s;
+    }
+    ++count;
+  }
+  return count;
+}
+
+/* ------------------------------------------------------------ plane helpers */
+static inline float load_sample(const void* base, long long stride, int bits, int floating, int y, int x) {
+  const char* row = (const char*)base + (long long)y * stride;
+  if (floating) return ((const float*)row)[x];
+  if (bits == 8) return (float)((const uint8_t*)row)[x];
+  return (float)((const uint16_t*)row)[x];
+}
+
+/* pad.hpp:16-45 + scalar/model.cpp:6-16: decode (x-base)*scale, one reflection */
+static void pad_plane(const void* src, long long stride, int bits, int floating, const nfo_axis* gx,
+                      const nfo_axis* gy, float base, float scale, float* pad) {
+  for (int cy = 0; cy < gy->cover; ++cy) {
+    const int sy = nfo_reflect(cy, gy);
+    for (int cx = 0; cx < gx->cover; ++cx) {
+      const int sx = nfo_reflect(cx, gx);
+      pad[(size_t)cy * gx->cover + cx] = fin((fin(load_sample(src, stride, bits, floating, sy, sx)) - base) * scale);
+    }
+  }
+}
+
+static int valid_format(int bits, int floating) {
+  REQUIRE(floating ? bits == 32 : (bits == 8 || bits == 10 || bits == 12 || bits == 14 || bits == 16),
+          "unsupported sample format");
+  return 0;
+}
+
+/* scalar/spatial.cpp:48-88 */
+static void store_plane(const float* acc, int cover_w, const nfo_axis* gx, const nfo_axis* gy, int bits,
+                        int floating, int fft3d, float base, void* dst, long long dstride) {
+  const int w = gx->length, h = gy->length;
+  const float peak = floating ? 1.0f : (float)((1 << bits) - 1);
+  const float scale = floating ? (fft3d ? 1.0f : 1.0f / 255.0f) : (float)(1 << (bits - 8));
+  for (int y = 0; y < h; ++y) {
+    const float* a = acc + (size_t)(y + gy->offset) * cover_w + gx->offset;
+    char* row = (char*)dst + (long long)y * dstride;
+    for (int x = 0; x < w; ++x) {
+      if (floating) {
+        ((float*)row)[x] = fft3d ? clampf_ref(a[x], 0.0f, 1.0f) : a[x] * scale;
+      } else {
+        const float v = fft3d ? (a[x] + 0.5f) + base : (a[x] * scale) + 0.5f;
+        const float cl = clampf_ref(v, 0.0f, peak);
+        if (bits == 8) ((uint8_t*)row)[x] = (uint8_t)cl;
+        else ((uint16_t*)row)[x] = (uint16_t)cl;
+      }
+    }
+  }
+}
+
+/* ----------------------------------------------------------------- DFTTest
+ * src/algorithms/plan.cpp:64-90 (validate), :170-226 (plan), :331-640 (run) */
+int nfo_dfttest_plane(const nfo_dft_cfg* c, int w, int h, int bits, int floating,
+                      const void* const* srcs, const long long* strides, int nsrc, const int* targets,
+                      int ntargets, void* dst, long long dst_stride, int frame, int plane) {
+  if (valid_format(bits, floating)) return -1;
+  REQUIRE(c->ftype >= 0 && c->ftype <= 4, "DFTTest ftype outside 0..4");
+  REQUIRE(c->tbsize >= 1 && c->tbsize <= 15, "DFTTest tbsize outside 1..15");
+  REQUIRE(c->tmode == 0 || c->tmode == 1, "DFTTest tmode outside 0..1");
+  REQUIRE(c->tmode != 0 || c->tbsize % 2 == 1, "DFTTest centered tbsize must be odd");
+  REQUIRE(c->dither >= 0 && c->dither_seed >= 0, "dither and dither_seed must be nonnegative");
+  REQUIRE(c->swin >= 0 && c->swin <= 11 && c->twin >= 0 && c->twin <= 11, "DFTTest invalid swin/twin");
+  REQUIRE(isfinite(c->f0beta) && c->f0beta > 0, "DFTTest f0beta must be positive");
+  const int B = c->sbsize, T = c->tbsize, tmode1 = c->tmode == 1;
+  nfo_axis gx, gy;
+  if (nfo_dft_axis(w, B, c->smode, c->sosize, &gx) || nfo_dft_axis(h, B, c->smode, c->sosize, &gy)) return -1;
+  if (tmode1) {
+    REQUIRE(ntargets >= 1 && ntargets <= 15 && nsrc == ntargets * T, "invalid temporal block inputs");
+    for (int i = 0; i < ntargets; ++i) REQUIRE(targets[i] >= 0 && targets[i] < T, "invalid target temporal slice");
+  } else {
+    REQUIRE(nsrc == T, "DFTTest sources size must match plan temporal_size");
+  }
+  const size_t S2 = (size_t)B * B, V = (size_t)T * S2;
+  const int K = B / 2 + 1;
+  const size_t bins2 = (size_t)B * K, bins = (size_t)T * bins2;
+  float* hwin = (float*)malloc(sizeof(float) * V);
+  float* hsyn = (float*)malloc(sizeof(float) * V);
+  float* primary = NULL;
+  float* grid = NULL;
+  float* pads = NULL;
+  float* acc = NULL;
+  float* blk = (float*)malloc(sizeof(float) * V);
+  float* spec = (float*)malloc(sizeof(float) * 2 * bins);
+  float* inv = (float*)malloc(sizeof(float) * S2);
+  int rc = 0;
+  float wscale;
+  if (nfo_dft_window_3d(T, B, c->smode == 0 ? 0 : c->sosize, c->smode, c->swin, c->twin, c->sbeta, c->tbeta,
+                        c->tmode, c->tmode ? c->tosize : 0, hwin, &wscale)) { rc = -1; goto out; }
+  {
+    const float volume = (float)T * (float)B * (float)B;
+    for (size_t i = 0; i < V; ++i) hsyn[i] = hwin[i] * volume;
+  }
+  g_nonfinite = 0;
+  const float scale = c->ftype < 2 ? wscale : 1.0f;
+  const int curves = c->n_slocation || c->n_ssx || c->n_ssy || c->n_sst;
+  const float pa = curves ? 0 : fin(c->sigma / scale);
+  const float pb = fin(c->sigma2 / scale);
+  const float plow = fin(c->pmin / wscale), phigh = fin(c->pmax / wscale);
+  if (g_nonfinite) { rc = fail("non-finite sample or intermediate"); goto out; }
+  if (curves) {
+    primary = (float*)malloc(sizeof(float) * bins);
+    if (nfo_dft_profile(c, T, B, c->sigma, scale, primary)) { rc = -1; goto out; }
+  }
+  if (c->zmean) {
+    for (size_t i = 0; i < V; ++i) blk[i] = fin(255.0f * hwin[i]);
+    grid = (float*)malloc(sizeof(float) * 2 * bins);
+    if (T > 1) nfo_rfft3(T, B, B, blk, grid);
+    else nfo_rfft2(B, B, blk, grid);
+    if (grid[0] == 0) { rc = fail("DFTTest zmean requires nonzero template DC"); goto out; }
+  }
+  pads = (float*)malloc(sizeof(float) * (size_t)T * gx.cover * gy.cover);
+  acc = (float*)calloc((size_t)gx.cover * gy.cover, sizeof(float));
+  {
+    const float dscale = floating ? 255.0f : 1.0f / (float)(1 << (bits - 8));
+    const int nblocks = tmode1 ? ntargets : 1;
+    const size_t plane_sz = (size_t)gx.cover * gy.cover;
+    const float mean_scale = c->zmean ? 1.0f : 0.0f;
+    for (int tb = 0; tb < nblocks; ++tb) {
+      g_nonfinite = 0;
+      for (int z = 0; z < T; ++z) {
+        const int s = tmode1 ? tb * T + z : z;
+        pad_plane(srcs[s], strides[s], bits, floating, &gx, &gy, 0.0f, dscale, pads + z * plane_sz);
+      }
+      if (g_nonfinite) { rc = fail("non-finite sample or intermediate"); goto out; }
+      const int center = tmode1 ? targets[tb] : T / 2;
+      for (int by = 0; by < gy.count; ++by) {
+        const int oy = by * gy.step;
+        for (int bx = 0; bx < gx.count; ++bx) {
+          const int ox = bx * gx.step;
+          /* gather_dfttest: scalar/spatial.cpp:18-28 */
+          for (int z = 0; z < T; ++z)
+            for (int y = 0; y < B; ++y)
+              for (int x = 0; x < B; ++x)
+                blk[((size_t)z * B + y) * B + x] =
+                    pads[z * plane_sz + (size_t)(oy + y) * gx.cover + ox + x] * hwin[((size_t)z * B + y) * B + x];
+          if (T == 1) nfo_rfft2(B, B, blk, spec);
+          else nfo_rfft3(T, B, B, blk, spec);
+          const float gs = (!grid || (T > 1 && (mean_scale == 0 || grid[0] == 0))) ? 0 : (mean_scale * spec[0]) / grid[0];
+          if (nfo_spectral(spec, grid, (int)bins, gs, c->ftype, pa, pb, plow, phigh, c->f0beta, 0.0f, primary,
+                           0, 0, 0, 0, 0, NULL, NULL)) { rc = -1; goto out; }
+          if (T == 1) nfo_irfft2(B, B, spec, inv, 1.0f / ((float)B * (float)B));
+          else nfo_irfft3_slice(T, B, B, center, spec, inv, 1.0f / ((float)T * (float)B * (float)B));
+          g_nonfinite = 0;
+          for (size_t i = 0; i < S2; ++i) fin(inv[i]);
+          if (g_nonfinite) { rc = fail("non-finite sample or intermediate"); goto out; }
+          if (tmode1) {
+            const float factor = (float)T * (float)B * (float)B;
+            for (size_t i = 0; i < S2; ++i) inv[i] = factor * inv[i];
+          }
+          const float* syn = (tmode1 ? hwin : hsyn) + (size_t)center * S2;
+          if (c->smode == 0) {
+            const int cy = B / 2, cx = B / 2;
+            const size_t k = (size_t)cy * B + cx;
+            float* a = &acc[(size_t)(oy + cy) * gx.cover + ox + cx];
+            if (tmode1) *a += inv[k] * syn[k];
+            else *a = inv[k] * syn[k];
+          } else {
+            for (int y = 0; y < B; ++y)
+              for (int x = 0; x < B; ++x) {
+                float* a = &acc[(size_t)(oy + y) * gx.cover + ox + x];
+                *a += inv[(size_t)y * B + x] * syn[(size_t)y * B + x];
+              }
+          }
+        }
+      }
+    }
+  }
+  g_nonfinite = 0;
+  for (size_t i = 0; i < (size_t)gx.cover * gy.cover; ++i) fin(acc[i]);
+  if (g_nonfinite) { rc = fail("non-finite sample or intermediate"); goto out; }
+  if (!floating && bits == 8 && c->dither > 0) {
+    rc = nfo_dither(acc + (size_t)gy.offset * gx.cover + gx.offset, w, h, gx.cover, (uint8_t*)dst, dst_stride,
+                    c->dither, c->dither_seed, frame, plane);
+  } else {
+    store_plane(acc, gx.cover, &gx, &gy, bits, floating, 0, 0.0f, dst, dst_stride);
+  }
+out:
+  free(hwin); free(hsyn); free(primary); free(grid); free(pads); free(acc); free(blk); free(spec); free(inv);
+  return rc;
+}
+
+/* ------------------------------------------------------------------- FFT3D
+ * src/algorithms/plan.cpp:45-63 (validate), :91-169 (plan), :331-537 (run) */
+typedef struct {
+  nfo_axis gx, gy;
+  float *wxa, *wxs, *wya, *wys;
+  float *grid, *primary, *sh, *halo;
+  float abc[3];
+  float a, floor_, sigma_eff, norm, beta, degrid, r0, ratio2;
+  int varying, denoise, kalman, enh, T;
+  float base;
+} fft3d_plan;
+
+static void fft3d_free(fft3d_plan* p) {
+  free(p->wxa); free(p->wxs); free(p->wya); free(p->wys);
+  free(p->grid); free(p->primary); free(p->sh); free(p->halo);
+  memset(p, 0, sizeof *p);
+}
+
+static int fft3d_build(const nfo_fft3d_cfg* c, int w, int h, int bits, int floating, int chroma, fft3d_plan* p) {
+  memset(p, 0, sizeof *p);
+  if (valid_format(bits, floating)) return -1;
+  REQUIRE(c->bw >= 2 && c->bh >= 2, "FFT3D bw/bh must be >=2");
+  REQUIRE(c->ow <= c->bw / 2 && c->oh <= c->bh / 2, "FFT3D ow/oh exceeds half block");
+  REQUIRE(c->wintype >= 0 && c->wintype <= 2, "FFT3D wintype outside 0..2");
+  REQUIRE(c->bt >= -1 && c->bt <= 5, "FFT3D bt outside -1..5");
+  REQUIRE(isfinite(c->beta) && c->beta >= 1, "FFT3D beta must be >=1");
+  REQUIRE(isfinite(c->sigma) && c->sigma >= 0 && isfinite(c->degrid) && c->degrid >= 0, "FFT3D invalid sigma/degrid");
+  if (nfo_fft3d_axis(w, c->bw, c->ow, &p->gx) || nfo_fft3d_axis(h, c->bh, c->oh, &p->gy)) return -1;
+  p->wxa = (float*)malloc(sizeof(float) * c->bw); p->wxs = (float*)malloc(sizeof(float) * c->bw);
+  p->wya = (float*)malloc(sizeof(float) * c->bh); p->wys = (float*)malloc(sizeof(float) * c->bh);
+  if (nfo_fft3d_window(c->bw, p->gx.overlap, c->wintype, p->wxa, p->wxs) ||
+      nfo_fft3d_window(c->bh, p->gy.overlap, c->wintype, p->wya, p->wys)) { fft3d_free(p); return -1; }
+  p->T = c->bt > 1 ? c->bt : 1;
+  p->denoise = c->bt != -1;
+  p->kalman = c->bt == 0;
+  p->base = (!floating && chroma) ? (float)(1 << (bits - 1)) : 0.0f;
+  g_nonfinite = 0;
+  const float factor = floating ? 1.0f / 255 : (float)(1 << (bits - 8));
+  p->sigma_eff = fin(c->sigma * factor);
+  p->norm = 1.0f / ((float)c->bw * (float)c->bh);
+  p->beta = c->beta;
+  p->degrid = c->degrid;
+  const float sig[4] = {p->sigma_eff, fin(c->sigma2 * factor), fin(c->sigma3 * factor), fin(c->sigma4 * factor)};
+  p->varying = sig[1] != sig[0] || sig[2] != sig[0] || sig[3] != sig[0];
+  if (p->kalman) {
+    p->r0 = fin((p->sigma_eff * p->sigma_eff) / p->norm);
+    p->ratio2 = fin(c->kratio * c->kratio);
+  }
+  p->floor_ = (c->beta - 1) / c->beta;
+  const int K = c->bw / 2 + 1;
+  const size_t bins = (size_t)c->bh * K;
+  if (p->denoise) {
+    if (p->varying) {
+      p->primary = (float*)malloc(sizeof(float) * bins);
+      if (nfo_fft3d_profile(c->bw, c->bh, sig, p->primary)) { fft3d_free(p); return -1; }
+      g_nonfinite = 0;
+      for (size_t k = 0; k < bins; ++k) fin((float)p->T * p->primary[k]);
+    } else {
+      p->a = fin((p->sigma_eff * p->sigma_eff) / p->norm);
+      for (int T = 1; T <= p->T; ++T) fin((((float)T * p->sigma_eff) * p->sigma_eff) / p->norm);
+    }
+  }
+  if (g_nonfinite) { fft3d_free(p); return fail("non-finite sample or intermediate"); }
+  p->enh = c->sharpen != 0 || c->dehalo != 0;
+  if (p->enh) {
+    p->sh = (float*)malloc(sizeof(float) * bins);
+    p->halo = (float*)malloc(sizeof(float) * bins);
+    if (nfo_enhancement_tables(c->bw, c->bh, factor, c->sharpen, c->scutoff, c->svr, c->smin, c->smax, c->dehalo,
+                               c->hr, c->ht, p->sh, p->halo, p->abc)) { fft3d_free(p); return -1; }
+  }
+  if (c->degrid != 0) {
+    float* block = (float*)malloc(sizeof(float) * c->bw * c->bh);
+    const float peak = floating ? 1.0f : (float)((1 << bits) - 1);
+    for (int y = 0; y < c->bh; ++y)
+      for (int x = 0; x < c->bw; ++x) block[y * c->bw + x] = (peak * p->wya[y]) * p->wxa[x];
+    p->grid = (float*)malloc(sizeof(float) * 2 * bins);
+    nfo_rfft2(c->bh, c->bw, block, p->grid);
+    free(block);
+    if (p->grid[0] == 0) { fft3d_free(p); return fail("FFT3D unusable grid DC"); }
+  }
+  return 0;
+}
+
+/* Plan::enhance, plan.cpp:325-329 */
+static int fft3d_enhance(const nfo_fft3d_cfg* c, const fft3d_plan* p, float* spec, int bins) {
+  if (!p->enh) return 0;
+  g_nonfinite = 0;
+  const float scale = p->grid ? fin(fin(p->degrid * spec[0]) / p->grid[0]) : 0;
+  if (g_nonfinite) return fail("non-finite sample or intermediate");
+  return nfo_spectral(spec, p->grid, bins, scale, -2, 0, 0, 0, 0, 1, 0, NULL, c->sharpen, c->dehalo, p->abc[0],
+                      p->abc[1], p->abc[2], p->sh, p->halo);
+}
+
+/* Inverse + two-level synthesis shared by FFT3D paths (plan.cpp:482-490, 516-535) */
+static int fft3d_emit(const nfo_fft3d_cfg* c, const fft3d_plan* p, const float* spec, float* inv, float* row,
+                      int ox) {
+  const int bw = c->bw, bh = c->bh;
+  nfo_irfft2(bh, bw, spec, inv, 1.0f / ((float)bw * (float)bh));
+  g_nonfinite = 0;
+  for (int i = 0; i < bw * bh; ++i) fin(inv[i]);
+  if (g_nonfinite) return fail("non-finite sample or intermediate");
+  for (int y = 0; y < bh; ++y)
+    for (int x = 0; x < bw; ++x) {
+      float* r = &row[(size_t)y * p->gx.cover + ox + x];
+      *r += inv[y * bw + x] * p->wxs[x];
+    }
+  return 0;
+}
+
+static void fft3d_gather(const fft3d_plan* p, const float* pad, int oy, int ox, int bw, int bh, float* blk) {
+  for (int y = 0; y < bh; ++y)
+    for (int x = 0; x < bw; ++x)
+      blk[y * bw + x] = (pad[(size_t)(oy + y) * p->gx.cover + ox + x] * p->wya[y]) * p->wxa[x];
+}
+
+static int fft3d_run(const nfo_fft3d_cfg* c, const fft3d_plan* p, int bits, int floating,
+                     const void* const* srcs, const long long* strides, int T, const float* kalman_last,
+                     void* dst, long long dst_stride) {
+  const int bw = c->bw, bh = c->bh, K = bw / 2 + 1;
+  const int bins = bh * K;
+  const size_t plane_sz = (size_t)p->gx.cover * p->gy.cover;
+  float* pads = kalman_last ? NULL : (float*)malloc(sizeof(float) * plane_sz * T);
+  float* acc = (float*)calloc(plane_sz, sizeof(float));
+  float* row = (float*)malloc(sizeof(float) * (size_t)bh * p->gx.cover);
+  float* blk = (float*)malloc(sizeof(float) * bw * bh);
+  float* spectra = (float*)malloc(sizeof(float) * 2 * bins * (T + 1));
+  float* inv = (float*)malloc(sizeof(float) * bw * bh);
+  int rc = 0;
+  const int c_slot = T / 2;
+  if (pads) {
+    g_nonfinite = 0;
+    for (int j = 0; j < T; ++j) pad_plane(srcs[j], strides[j], bits, floating, &p->gx, &p->gy, p->base, 1.0f, pads + j * plane_sz);
+    if (g_nonfinite) { rc = fail("non-finite sample or intermediate"); goto out; }
+  }
+  {
+    const float noise_uniform = p->denoise && !p->varying ? (((float)T * p->sigma_eff) * p->sigma_eff) / p->norm : 0;
+    const float lower = (p->beta - 1.0f) / p->beta;
+    for (int by = 0; by < p->gy.count; ++by) {
+      const int oy = by * p->gy.step;
+      memset(row, 0, sizeof(float) * (size_t)bh * p->gx.cover);
+      for (int bx = 0; bx < p->gx.count; ++bx) {
+        const int ox = bx * p->gx.step;
+        float* out_spec = spectra + 2 * (size_t)bins * T;
+        if (T == 1) {
+          float* s = spectra;
+          if (kalman_last) {
+            memcpy(s, kalman_last + 2 * ((size_t)by * p->gx.count + bx) * bins, sizeof(float) * 2 * bins);
+            if (fft3d_enhance(c, p, s, bins)) { rc = -1; goto out; }
+          } else {
+            fft3d_gather(p, pads, oy, ox, bw, bh, blk);
+            nfo_rfft2(bh, bw, blk, s);
+            const float scale = p->grid ? (p->degrid * s[0]) / p->grid[0] : 0;
+            if (p->denoise) {
+              const int e = !p->varying && p->enh;
+              if (nfo_spectral(s, p->grid, bins, scale, -1, p->a, 0, 0, 0, 1, p->floor_, p->primary,
+                               e ? c->sharpen : 0, e ? c->dehalo : 0, p->abc[0], p->abc[1], p->abc[2], p->sh,
+                               p->halo)) { rc = -1; goto out; }
+              if (p->varying && fft3d_enhance(c, p, s, bins)) { rc = -1; goto out; }
+            } else if (fft3d_enhance(c, p, s, bins)) { rc = -1; goto out; }
+          }
+          if (fft3d_emit(c, p, s, inv, row, ox)) { rc = -1; goto out; }
+        } else {
+          const float* ptrs[5];
+          for (int j = 0; j < T; ++j) {
+            fft3d_gather(p, pads + j * plane_sz, oy, ox, bw, bh, blk);
+            nfo_rfft2(bh, bw, blk, spectra + 2 * (size_t)bins * j);
+            ptrs[j] = spectra + 2 * (size_t)bins * j;
+          }
+          if (nfo_fft3d_temporal(ptrs, T, c_slot, bins, p->degrid, p->grid, noise_uniform,
+                                 p->varying ? p->primary : NULL, (float)T, lower, out_spec)) { rc = -1; goto out; }
+          if (fft3d_enhance(c, p, out_spec, bins)) { rc = -1; goto out; }
+          if (fft3d_emit(c, p, out_spec, inv, row, ox)) { rc = -1; goto out; }
+        }
+      }
+      for (int y = 0; y < bh; ++y) {
+        const float wy = p->wys[y];
+        for (int x = 0; x < p->gx.cover; ++x) acc[(size_t)(oy + y) * p->gx.cover + x] += row[(size_t)y * p->gx.cover + x] * wy;
+      }
+    }
+  }
+  g_nonfinite = 0;
+  for (size_t i = 0; i < plane_sz; ++i) fin(acc[i]);
+  if (g_nonfinite) { rc = fail("non-finite sample or intermediate"); goto out; }
+  store_plane(acc, p->gx.cover, &p->gx, &p->gy, bits, floating, 1, p->base, dst, dst_stride);
+out:
+  free(pads); free(acc); free(row); free(blk); free(spectra); free(inv);
+  return rc;
+}
+
+int nfo_fft3d_plane(const nfo_fft3d_cfg* c, int w, int h, int bits, int floating, int chroma,
+                    const void* const* srcs, const long long* strides, int nsrc, void* dst, long long dst_stride) {
+  fft3d_plan p;
+  if (fft3d_build(c, w, h, bits, floating, chroma, &p)) return -1;
+  int rc;
+  if (p.kalman) rc = fail("Kalman planes use nfo_kalman_*");
+  else if (!(nsrc == p.T || nsrc == 1)) rc = fail("FFT3D sources size must match plan temporal_size or single-frame fallback");
+  else rc = fft3d_run(c, &p, bits, floating, srcs, strides, nsrc, NULL, dst, dst_stride);
+  fft3d_free(&p);
+  return rc;
+}
+
+/* Plan::state_bins / initial_kalman / advance_kalman, plan.cpp:290-321 */
+long long nfo_kalman_bins(const nfo_fft3d_cfg* c, int w, int h) {
+  nfo_axis gx, gy;
+  if (nfo_fft3d_axis(w, c->bw, c->ow, &gx) || nfo_fft3d_axis(h, c->bh, c->oh, &gy)) return -1;
+  return (long long)gx.count * gy.count * c->bh * (c->bw / 2 + 1);
+}
+
+int nfo_kalman_init(const nfo_fft3d_cfg* c, int w, int h, int bits, int floating, float* last, float* cov, float* proc) {
+  fft3d_plan p;
+  if (fft3d_build(c, w, h, bits, floating, 0, &p)) return -1;
+  if (!p.kalman) { fft3d_free(&p); return fail("not a Kalman plan"); }
+  const long long n = (long long)p.gx.count * p.gy.count * c->bh * (c->bw / 2 + 1);
+  for (long long k = 0; k < n; ++k) { last[2 * k] = last[2 * k + 1] = 0; cov[k] = proc[k] = p.r0; }
+  fft3d_free(&p);
+  return 0;
+}
+
+int nfo_kalman_advance(const nfo_fft3d_cfg* c, int w, int h, int bits, int floating, int chroma, const void* src,
+                       long long stride, float* last, float* cov, float* proc) {
+  fft3d_plan p;
+  if (fft3d_build(c, w, h, bits, floating, chroma, &p)) return -1;
+  int rc = 0;
+  if (!p.kalman) { fft3d_free(&p); return fail("not a Kalman plan"); }
+  const int bw = c->bw, bh = c->bh, bins = bh * (bw / 2 + 1);
+  const size_t plane_sz = (size_t)p.gx.cover * p.gy.cover;
+  float* pad = (float*)malloc(sizeof(float) * plane_sz);
+  float* blk = (float*)malloc(sizeof(float) * bw * bh);
+  float* spec = (float*)malloc(sizeof(float) * 2 * bins);
+  g_nonfinite = 0;
+  pad_plane(src, stride, bits, floating, &p.gx, &p.gy, p.base, 1.0f, pad);
+  if (g_nonfinite) rc = fail("non-finite sample or intermediate");
+  for (int by = 0; by < p.gy.count && !rc; ++by)
+    for (int bx = 0; bx < p.gx.count && !rc; ++bx) {
+      fft3d_gather(&p, pad, by * p.gy.step, bx * p.gx.step, bw, bh, blk);
+      nfo_rfft2(bh, bw, blk, spec);
+      const size_t off = ((size_t)by * p.gx.count + bx) * bins;
+      rc = nfo_kalman_step(spec, last + 2 * off, cov + off, proc + off, p.varying ? p.primary : NULL, p.r0, p.ratio2, bins);
+    }
+  free(pad); free(blk); free(spec);
+  fft3d_free(&p);
+  return rc;
+}
+
+int nfo_kalman_render(const nfo_fft3d_cfg* c, int w, int h, int bits, int floating, int chroma, const float* last,
+                      void* dst, long long dst_stride) {
+  fft3d_plan p;
+  if (fft3d_build(c, w, h, bits, floating, chroma, &p)) return -1;
+  int rc = p.kalman ? fft3d_run(c, &p, bits, floating, NULL, NULL, 1, last, dst, dst_stride) : fail("not a Kalman plan");
+  fft3d_free(&p);
+  return rc;
+}

@@ -1,0 +1,351 @@
+"""ctypes binding of the CPU oracle (oracle/libnf_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker of the GPU path. The product
+package never imports this module.
+"""
+import ctypes
+import os
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libnf_oracle.so")
+_lib = None
+
+c_float_p = ctypes.POINTER(ctypes.c_float)
+
+
+class Axis(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("length", "block", "overlap", "step", "count", "cover", "offset")]
+
+    def as_tuple(self):
+        return tuple(getattr(self, n) for n, _ in self._fields_)
+
+
+class DftCfg(ctypes.Structure):
+    _fields_ = ([(n, ctypes.c_int) for n in ("ftype", "sbsize", "smode", "sosize", "tbsize", "tmode", "tosize",
+                                             "swin", "twin")]
+                + [(n, ctypes.c_float) for n in ("sigma", "sigma2", "pmin", "pmax", "sbeta", "tbeta", "f0beta")]
+                + [(n, ctypes.c_int) for n in ("zmean", "dither", "dither_seed")]
+                + [(n, c_float_p) for n in ("slocation", "ssx", "ssy", "sst")]
+                + [(n, ctypes.c_int) for n in ("n_slocation", "n_ssx", "n_ssy", "n_sst", "ssystem")])
+
+
+class Fft3dCfg(ctypes.Structure):
+    _fields_ = ([(n, ctypes.c_int) for n in ("bw", "bh", "ow", "oh", "bt", "wintype")]
+                + [(n, ctypes.c_float) for n in ("sigma", "beta", "degrid", "kratio", "sigma2", "sigma3", "sigma4",
+                                                 "sharpen", "scutoff", "svr", "smin", "smax", "dehalo", "hr", "ht")])
+
+
+DFT_DEFAULTS = dict(ftype=0, sigma=8.0, sigma2=8.0, pmin=0.0, pmax=500.0, sbsize=16, smode=1, sosize=12, tbsize=3,
+                    tmode=0, tosize=0, swin=0, twin=7, sbeta=2.5, tbeta=2.5, zmean=True, f0beta=1.0, dither=0,
+                    dither_seed=0, slocation=None, ssx=None, ssy=None, sst=None, ssystem=0)
+FFT3D_DEFAULTS = dict(sigma=2.0, beta=1.0, bw=32, bh=32, bt=3, ow=-1, oh=-1, kratio=2.0, sharpen=0.0, scutoff=0.3,
+                      svr=1.0, smin=4.0, smax=20.0, wintype=0, degrid=1.0, dehalo=0.0, hr=2.0, ht=50.0,
+                      sigma2=None, sigma3=None, sigma4=None)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError("oracle library missing: run `make -C oracle` or __graft_entry__.build()")
+        L = ctypes.CDLL(_LIB_PATH)
+        L.nfo_last_error.restype = ctypes.c_char_p
+        L.nfo_dft_raw_window.restype = ctypes.c_double
+        L.nfo_dither_hash.restype = ctypes.c_uint32
+        L.nfo_dither_hash.argtypes = [ctypes.c_uint32] * 5
+        L.nfo_kalman_bins.restype = ctypes.c_longlong
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(lib().nfo_last_error().decode())
+
+
+def _fptr(a):
+    return a.ctypes.data_as(c_float_p)
+
+
+def _curve(values):
+    if values is None or len(values) == 0:
+        return None, 0
+    arr = np.ascontiguousarray(values, dtype=np.float32)
+    return arr, len(arr)
+
+
+def dft_cfg(**kw):
+    p = dict(DFT_DEFAULTS)
+    p.update(kw)
+    c = DftCfg()
+    for n in ("ftype", "sbsize", "smode", "sosize", "tbsize", "tmode", "tosize", "swin", "twin", "dither",
+              "dither_seed", "ssystem"):
+        setattr(c, n, int(p[n]))
+    for n in ("sigma", "sigma2", "pmin", "pmax", "sbeta", "tbeta", "f0beta"):
+        setattr(c, n, float(p[n]))
+    c.zmean = int(bool(p["zmean"]))
+    keep = []
+    for n in ("slocation", "ssx", "ssy", "sst"):
+        arr, cnt = _curve(p[n])
+        if arr is not None:
+            keep.append(arr)
+            setattr(c, n, _fptr(arr))
+        setattr(c, "n_" + n, cnt)
+    c._keep = keep
+    return c
+
+
+def fft3d_cfg(**kw):
+    p = dict(FFT3D_DEFAULTS)
+    p.update(kw)
+    for n in ("sigma2", "sigma3", "sigma4"):
+        if p[n] is None:
+            p[n] = p["sigma"]
+    c = Fft3dCfg()
+    for n in ("bw", "bh", "ow", "oh", "bt", "wintype"):
+        setattr(c, n, int(p[n]))
+    for n in ("sigma", "beta", "degrid", "kratio", "sigma2", "sigma3", "sigma4", "sharpen", "scutoff", "svr", "smin",
+              "smax", "dehalo", "hr", "ht"):
+        setattr(c, n, float(p[n]))
+    return c
+
+
+def fft3d_axis(length, block, overlap):
+    a = Axis()
+    _check(lib().nfo_fft3d_axis(length, block, overlap, ctypes.byref(a)))
+    return a
+
+
+def dft_axis(length, block, mode, overlap):
+    a = Axis()
+    _check(lib().nfo_dft_axis(length, block, mode, overlap, ctypes.byref(a)))
+    return a
+
+
+def fft3d_window(block, overlap, wtype):
+    an = np.zeros(block, np.float32)
+    sy = np.zeros(block, np.float32)
+    _check(lib().nfo_fft3d_window(block, overlap, wtype, _fptr(an), _fptr(sy)))
+    return an, sy
+
+
+def dft_raw_window(wid, j, length, beta):
+    return lib().nfo_dft_raw_window(wid, j, length, ctypes.c_float(beta))
+
+
+def dft_window_3d(T, block, overlap, mode, swin, twin, sbeta=2.5, tbeta=2.5, tmode=0, tosize=0):
+    h = np.zeros(T * block * block, np.float32)
+    ws = ctypes.c_float()
+    _check(lib().nfo_dft_window_3d(T, block, overlap, mode, swin, twin, ctypes.c_float(sbeta), ctypes.c_float(tbeta),
+                                   tmode, tosize, _fptr(h), ctypes.byref(ws)))
+    return h, ws.value
+
+
+def rfft2(block):
+    block = np.ascontiguousarray(block, np.float32)
+    h, w = block.shape
+    out = np.zeros((h, w // 2 + 1), np.complex64)
+    lib().nfo_rfft2(h, w, _fptr(block), out.ctypes.data_as(c_float_p))
+    return out
+
+
+def irfft2(spec, w, fct=None):
+    spec = np.ascontiguousarray(spec, np.complex64)
+    h = spec.shape[0]
+    out = np.zeros((h, w), np.float32)
+    if fct is None:
+        fct = np.float32(1.0) / (np.float32(w) * np.float32(h))
+    lib().nfo_irfft2(h, w, spec.ctypes.data_as(c_float_p), _fptr(out), ctypes.c_float(fct))
+    return out
+
+
+def rfft3(vol):
+    vol = np.ascontiguousarray(vol, np.float32)
+    t, h, w = vol.shape
+    out = np.zeros((t, h, w // 2 + 1), np.complex64)
+    lib().nfo_rfft3(t, h, w, _fptr(vol), out.ctypes.data_as(c_float_p))
+    return out
+
+
+def irfft3_slice(spec, w, slice_index, fct=None):
+    spec = np.ascontiguousarray(spec, np.complex64)
+    t, h = spec.shape[:2]
+    out = np.zeros((h, w), np.float32)
+    if fct is None:
+        fct = np.float32(1.0) / (np.float32(t) * np.float32(h) * np.float32(w))
+    lib().nfo_irfft3_slice(t, h, w, slice_index, spec.ctypes.data_as(c_float_p), _fptr(out), ctypes.c_float(fct))
+    return out
+
+
+def spectral(x, grid=None, scale=0.0, type=-1, a=0.0, b=0.0, low=0.0, high=0.0, exponent=1.0, floor=0.0,
+             primary=None, enhancement=None):
+    x = np.ascontiguousarray(x, np.complex64).copy()
+    g = None if grid is None else np.ascontiguousarray(grid, np.complex64)
+    pr = None if primary is None else np.ascontiguousarray(primary, np.float32)
+    e = enhancement or {}
+    sh = e.get("sharpen_tab")
+    ha = e.get("halo_tab")
+    _check(lib().nfo_spectral(
+        x.ctypes.data_as(c_float_p), None if g is None else g.ctypes.data_as(c_float_p), len(x), ctypes.c_float(scale),
+        type, ctypes.c_float(a), ctypes.c_float(b), ctypes.c_float(low), ctypes.c_float(high),
+        ctypes.c_float(exponent), ctypes.c_float(floor), None if pr is None else _fptr(pr),
+        ctypes.c_float(e.get("sharpen", 0.0)), ctypes.c_float(e.get("dehalo", 0.0)), ctypes.c_float(e.get("a", 0.0)),
+        ctypes.c_float(e.get("b", 0.0)), ctypes.c_float(e.get("c", 0.0)),
+        None if sh is None else _fptr(np.ascontiguousarray(sh, np.float32)),
+        None if ha is None else _fptr(np.ascontiguousarray(ha, np.float32))))
+    return x
+
+
+def fft3d_temporal(spectra, c, degrid=0.0, grid=None, noise=0.0, noise_table=None, multiplier=1.0, lower=0.0):
+    arrs = [np.ascontiguousarray(s, np.complex64) for s in spectra]
+    T = len(arrs)
+    bins = len(arrs[0])
+    ptrs = (c_float_p * T)(*[a.ctypes.data_as(c_float_p) for a in arrs])
+    out = np.zeros(bins, np.complex64)
+    g = None if grid is None else np.ascontiguousarray(grid, np.complex64)
+    nt = None if noise_table is None else np.ascontiguousarray(noise_table, np.float32)
+    _check(lib().nfo_fft3d_temporal(ptrs, T, c, bins, ctypes.c_float(degrid),
+                                    None if g is None else g.ctypes.data_as(c_float_p), ctypes.c_float(noise),
+                                    None if nt is None else _fptr(nt), ctypes.c_float(multiplier),
+                                    ctypes.c_float(lower), out.ctypes.data_as(c_float_p)))
+    return out
+
+
+def kalman_step(x, last, cov, proc, pattern=None, uniform=0.0, ratio2=4.0):
+    x = np.ascontiguousarray(x, np.complex64)
+    last = np.ascontiguousarray(last, np.complex64).copy()
+    cov = np.ascontiguousarray(cov, np.float32).copy()
+    proc = np.ascontiguousarray(proc, np.float32).copy()
+    pt = None if pattern is None else np.ascontiguousarray(pattern, np.float32)
+    _check(lib().nfo_kalman_step(x.ctypes.data_as(c_float_p), last.ctypes.data_as(c_float_p), _fptr(cov), _fptr(proc),
+                                 None if pt is None else _fptr(pt), ctypes.c_float(uniform), ctypes.c_float(ratio2),
+                                 len(x)))
+    return last, cov, proc
+
+
+def dither_hash(seed, n, p, y, x):
+    return lib().nfo_dither_hash(seed, n, p, y, x)
+
+
+def dither(src, mode, seed=0, frame=0, plane=0):
+    src = np.ascontiguousarray(src, np.float32)
+    h, w = src.shape
+    dst = np.zeros((h, w), np.uint8)
+    _check(lib().nfo_dither(_fptr(src), w, h, ctypes.c_longlong(w), dst.ctypes.data_as(ctypes.c_void_p),
+                            ctypes.c_longlong(w), mode, seed, frame, plane))
+    return dst
+
+
+def temporal_blocks(frame, length, size, overlap):
+    starts = (ctypes.c_longlong * 16)()
+    targets = (ctypes.c_int * 16)()
+    slots = (ctypes.c_int * (16 * 15))()
+    n = lib().nfo_temporal_blocks(frame, length, size, overlap, starts, targets, slots)
+    if n < 0:
+        _check(n)
+    return [(starts[i], targets[i], [slots[i * size + z] for z in range(size)]) for i in range(n)]
+
+
+def fft3d_profile(w, h, sigmas):
+    s = np.ascontiguousarray(sigmas, np.float32)
+    out = np.zeros(h * (w // 2 + 1), np.float32)
+    _check(lib().nfo_fft3d_profile(w, h, _fptr(s), _fptr(out)))
+    return out
+
+
+def enhancement_tables(w, h, factor, sharpen=0.0, scutoff=0.3, svr=1.0, smin=4.0, smax=20.0, dehalo=0.0, hr=2.0,
+                       ht=50.0):
+    n = h * (w // 2 + 1)
+    sh = np.zeros(n, np.float32)
+    ha = np.zeros(n, np.float32)
+    abc = np.zeros(3, np.float32)
+    f = ctypes.c_float
+    _check(lib().nfo_enhancement_tables(w, h, f(factor), f(sharpen), f(scutoff), f(svr), f(smin), f(smax), f(dehalo),
+                                        f(hr), f(ht), _fptr(sh), _fptr(ha), _fptr(abc)))
+    return sh, ha, abc
+
+
+def dft_profile(T, S, sigma, divisor, **curves):
+    c = dft_cfg(**curves)
+    out = np.zeros(T * S * (S // 2 + 1), np.float32)
+    _check(lib().nfo_dft_profile(ctypes.byref(c), T, S, ctypes.c_float(sigma), ctypes.c_float(divisor), _fptr(out)))
+    return out
+
+
+def _fmt_dtype(bits, floating):
+    if floating:
+        return np.float32
+    return np.uint8 if bits == 8 else np.uint16
+
+
+def dfttest_plane(slots, bits=8, floating=False, targets=None, frame=0, plane=0, **params):
+    """Process one plane. slots: list of 2D arrays (temporal slots; for tmode=1
+    the concatenation of every covering block's slots)."""
+    c = dft_cfg(**params)
+    dt = _fmt_dtype(bits, floating)
+    arrs = [np.ascontiguousarray(s, dt) for s in slots]
+    h, w = arrs[0].shape
+    ptrs = (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    strides = (ctypes.c_longlong * len(arrs))(*[a.strides[0] for a in arrs])
+    dst = np.zeros((h, w), dt)
+    tg = None
+    nt = 0
+    if targets is not None:
+        nt = len(targets)
+        tg = (ctypes.c_int * nt)(*targets)
+    _check(lib().nfo_dfttest_plane(ctypes.byref(c), w, h, bits, int(floating), ptrs, strides, len(arrs), tg, nt,
+                                   dst.ctypes.data_as(ctypes.c_void_p), ctypes.c_longlong(dst.strides[0]), frame,
+                                   plane))
+    return dst
+
+
+def fft3d_plane(slots, bits=8, floating=False, chroma=False, **params):
+    c = fft3d_cfg(**params)
+    dt = _fmt_dtype(bits, floating)
+    arrs = [np.ascontiguousarray(s, dt) for s in slots]
+    h, w = arrs[0].shape
+    ptrs = (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    strides = (ctypes.c_longlong * len(arrs))(*[a.strides[0] for a in arrs])
+    dst = np.zeros((h, w), dt)
+    _check(lib().nfo_fft3d_plane(ctypes.byref(c), w, h, bits, int(floating), int(chroma), ptrs, strides, len(arrs),
+                                 dst.ctypes.data_as(ctypes.c_void_p), ctypes.c_longlong(dst.strides[0])))
+    return dst
+
+
+class KalmanState:
+    """Host copy of one plane's Kalman state (complex L, float C, float Q)."""
+
+    def __init__(self, params, w, h, bits=8, floating=False):
+        self.cfg = fft3d_cfg(**dict(params, bt=0))
+        self.w, self.h, self.bits, self.floating = w, h, bits, floating
+        n = lib().nfo_kalman_bins(ctypes.byref(self.cfg), w, h)
+        if n < 0:
+            _check(-1)
+        self.last = np.zeros(n, np.complex64)
+        self.cov = np.zeros(n, np.float32)
+        self.proc = np.zeros(n, np.float32)
+        _check(lib().nfo_kalman_init(ctypes.byref(self.cfg), w, h, bits, int(floating),
+                                     self.last.ctypes.data_as(c_float_p), _fptr(self.cov), _fptr(self.proc)))
+
+    def copy(self):
+        other = KalmanState.__new__(KalmanState)
+        other.__dict__.update(self.__dict__)
+        other.last, other.cov, other.proc = self.last.copy(), self.cov.copy(), self.proc.copy()
+        return other
+
+    def advance(self, src, chroma=False):
+        dt = _fmt_dtype(self.bits, self.floating)
+        a = np.ascontiguousarray(src, dt)
+        _check(lib().nfo_kalman_advance(ctypes.byref(self.cfg), self.w, self.h, self.bits, int(self.floating),
+                                        int(chroma), ctypes.c_void_p(a.ctypes.data), ctypes.c_longlong(a.strides[0]),
+                                        self.last.ctypes.data_as(c_float_p), _fptr(self.cov), _fptr(self.proc)))
+
+    def render(self, chroma=False):
+        dt = _fmt_dtype(self.bits, self.floating)
+        dst = np.zeros((self.h, self.w), dt)
+        _check(lib().nfo_kalman_render(ctypes.byref(self.cfg), self.w, self.h, self.bits, int(self.floating),
+                                       int(chroma), self.last.ctypes.data_as(c_float_p),
+                                       dst.ctypes.data_as(ctypes.c_void_p), ctypes.c_longlong(dst.strides[0])))
+        return dst

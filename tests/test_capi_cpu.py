@@ -1,0 +1,60 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+declared in include/neo_fft.h, and the config initialisers carry the
+reference's public defaults (src/plugin/descriptors.hpp:161-235)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "neo_fft.h")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(nf_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def nfl():
+    import neo_fft
+    return neo_fft
+
+
+def test_library_exports_every_declared_symbol(nfl):
+    lib = nfl.lib()
+    names = header_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # the binding declares a ctypes signature for each of them
+    assert set(names) <= set(nfl._SIGNATURES), set(names) - set(nfl._SIGNATURES)
+
+
+def test_dfttest_defaults(nfl):
+    c = nfl.DFTTestConfig()
+    nfl.lib().nf_dfttest_config_init(ctypes.byref(c))
+    expect = dict(ftype=0, sigma=8.0, sigma2=8.0, pmin=0.0, pmax=500.0, sbsize=16, smode=1, sosize=12, tbsize=3,
+                  tmode=0, tosize=0, swin=0, twin=7, sbeta=2.5, tbeta=2.5, zmean=1, f0beta=1.0, ssystem=0, dither=0,
+                  dither_seed=0, opt=0)
+    for k, v in expect.items():
+        assert getattr(c, k) == pytest.approx(v), k
+
+
+def test_fft3d_defaults(nfl):
+    c = nfl.FFT3DConfig()
+    nfl.lib().nf_fft3d_config_init(ctypes.byref(c))
+    expect = dict(sigma=2.0, beta=1.0, bw=32, bh=32, bt=3, ow=-1, oh=-1, kratio=2.0, sharpen=0.0, scutoff=0.3,
+                  svr=1.0, smin=4.0, smax=20.0, wintype=0, pcutoff=0.1, pfactor=0.0, degrid=1.0, dehalo=0.0, hr=2.0,
+                  ht=50.0, kalman_warmup=8)
+    for k, v in expect.items():
+        assert getattr(c, k) == pytest.approx(v), k
+
+
+def test_config_keyword_validation(nfl):
+    with pytest.raises(TypeError):
+        nfl.dfttest_config(not_a_parameter=1)
+    with pytest.raises(TypeError):
+        nfl.fft3d_config(sbsize=16)

@@ -1,0 +1,216 @@
+"""GPU parity: the HIP kernels, called through the C-ABI, against the CPU
+oracle on identical seeded inputs.
+
+The kernels evaluate the oracle's operations in the same order (same
+butterflies, same binary32 steps, no contraction), so outputs are expected to
+be bit-identical; every test asserts the tolerance bar (integers: 1 LSB on at
+most 0.1% of samples; floats: 1e-5 relative) and, where the operations are
+identical, exact equality too."""
+import numpy as np
+import pytest
+
+from gpu_util import assert_parity, compare, empty_like_np, to_device, to_host
+from synthetic import gray8_textured, yuv420_sinusoids
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(plan, srcs_np, **kw):
+    srcs = [to_device(s) for s in srcs_np]
+    dst = empty_like_np(srcs_np[0])
+    plan.process(srcs, dst, **kw)
+    return to_host(dst, srcs_np[0].dtype)
+
+
+def _rand_plane(rng, h, w, bits, floating):
+    if floating:
+        return rng.uniform(0.0, 1.0, size=(h, w)).astype(np.float32)
+    peak = (1 << bits) - 1
+    return rng.integers(0, peak + 1, size=(h, w)).astype(np.uint8 if bits == 8 else np.uint16)
+
+
+def test_dfttest_c1_frames_bit_exact(nf, oracle):
+    frames = gray8_textured(640, 360, 3, seed=11)
+    params = dict(tbsize=1, sbsize=16, sosize=12, swin=0, ftype=0, sigma=8.0)
+    plan = nf.dfttest_plan(640, 360, bits=8, **params)
+    for f in frames:
+        ref = oracle.dfttest_plane([f], bits=8, **params)
+        got = _run(plan, [f])
+        assert_parity(got, ref, what="C1 DFTTest")
+        assert np.array_equal(got, ref), compare(got, ref)
+
+
+@pytest.mark.parametrize("B,O", [(8, 6), (8, 4), (16, 8), (16, 12), (32, 24), (32, 16)])
+@pytest.mark.parametrize("ftype", [0, 1, 2, 3, 4])
+def test_dfttest_t1_shapes_and_ftypes(nf, oracle, B, O, ftype):
+    rng = np.random.default_rng(B * 100 + O * 10 + ftype)
+    h, w = 61, 83
+    img = _rand_plane(rng, h, w, 8, False)
+    params = dict(tbsize=1, sbsize=B, sosize=O, ftype=ftype, sigma=6.0, sigma2=1.5, pmin=1.0, pmax=300.0, swin=ftype)
+    plan = nf.dfttest_plan(w, h, bits=8, **params)
+    ref = oracle.dfttest_plane([img], bits=8, **params)
+    got = _run(plan, [img])
+    assert_parity(got, ref, what=f"B{B}/O{O}/ftype{ftype}")
+    assert np.array_equal(got, ref), compare(got, ref)
+
+
+@pytest.mark.parametrize("bits,floating", [(10, False), (16, False), (32, True)])
+def test_dfttest_t1_formats(nf, oracle, bits, floating):
+    rng = np.random.default_rng(bits)
+    h, w = 45, 70
+    img = _rand_plane(rng, h, w, bits, floating)
+    params = dict(tbsize=1, sbsize=16, sosize=12, ftype=0, sigma=4.0, zmean=True)
+    plan = nf.dfttest_plan(w, h, bits=bits, floating=floating, **params)
+    ref = oracle.dfttest_plane([img], bits=bits, floating=floating, **params)
+    got = _run(plan, [img])
+    assert_parity(got, ref, floating=floating, what=f"bits{bits}")
+    assert np.array_equal(got, ref), compare(got, ref)
+
+
+def test_dfttest_t1_no_zmean_and_f0beta(nf, oracle):
+    rng = np.random.default_rng(5)
+    img = _rand_plane(rng, 40, 52, 8, False)
+    for kw in (dict(zmean=False), dict(f0beta=0.5), dict(f0beta=0.75)):
+        params = dict(tbsize=1, sbsize=16, sosize=12, ftype=0, sigma=6.0, **kw)
+        plan = nf.dfttest_plan(52, 40, bits=8, **params)
+        ref = oracle.dfttest_plane([img], bits=8, **params)
+        got = _run(plan, [img])
+        assert_parity(got, ref, what=str(kw))
+
+
+def test_dfttest_curves(nf, oracle):
+    rng = np.random.default_rng(9)
+    img = _rand_plane(rng, 48, 64, 8, False)
+    curve = [0.0, 4.0, 0.25, 9.0, 0.5, 6.0, 1.0, 12.0]
+    for kw in (dict(slocation=curve), dict(ssx=curve, ssy=[0.0, 2.0, 1.0, 8.0], ssystem=0),
+               dict(slocation=curve, ssystem=1)):
+        params = dict(tbsize=1, sbsize=16, sosize=12, ftype=0, **kw)
+        plan = nf.dfttest_plan(64, 48, bits=8, **params)
+        ref = oracle.dfttest_plane([img], bits=8, **params)
+        got = _run(plan, [img])
+        assert np.array_equal(got, ref), (kw, compare(got, ref))
+
+
+@pytest.mark.parametrize("bt", [1, 2, 3, 4, 5, -1])
+@pytest.mark.parametrize("bw", [8, 16, 32])
+def test_fft3d_bt_and_sizes(nf, oracle, bt, bw):
+    rng = np.random.default_rng(bt * 7 + bw)
+    h, w = 3 * bw + 5, 4 * bw + 3
+    T = max(1, bt)
+    frames = [_rand_plane(rng, h, w, 8, False) for _ in range(T)]
+    params = dict(bw=bw, bh=bw, bt=bt, sigma=3.0, ow=bw // 2, oh=bw // 3)
+    if bt == -1:
+        params.update(sharpen=0.5)
+    plan = nf.fft3d_plan(w, h, bits=8, **params)
+    ref = oracle.fft3d_plane(frames, bits=8, **params)
+    got = _run(plan, frames)
+    assert_parity(got, ref, what=f"bt{bt}/bw{bw}")
+    assert np.array_equal(got, ref), compare(got, ref)
+
+
+def test_fft3d_end_of_clip_fallback(nf, oracle):
+    rng = np.random.default_rng(3)
+    img = _rand_plane(rng, 70, 90, 8, False)
+    params = dict(bw=32, bh=32, bt=3, sigma=2.5)
+    plan = nf.fft3d_plan(90, 70, bits=8, **params)
+    ref = oracle.fft3d_plane([img], bits=8, **params)
+    got = _run(plan, [img])
+    assert np.array_equal(got, ref), compare(got, ref)
+
+
+@pytest.mark.parametrize("kw", [dict(sharpen=0.4, dehalo=0.3, hr=2.5, ht=30.0),
+                                dict(sigma2=1.0, sigma3=4.0, sigma4=6.0),
+                                dict(sigma2=1.0, sigma3=4.0, sigma4=6.0, sharpen=0.3, dehalo=0.2),
+                                dict(beta=1.5, degrid=0.0),
+                                dict(wintype=1), dict(wintype=2)])
+@pytest.mark.parametrize("bt", [1, 3])
+def test_fft3d_models(nf, oracle, kw, bt):
+    rng = np.random.default_rng(len(str(kw)) + bt)
+    h, w = 80, 96
+    frames = [_rand_plane(rng, h, w, 8, False) for _ in range(max(1, bt))]
+    params = dict(bw=16, bh=16, bt=bt, sigma=4.0, **kw)
+    plan = nf.fft3d_plan(w, h, bits=8, **params)
+    ref = oracle.fft3d_plane(frames, bits=8, **params)
+    got = _run(plan, frames)
+    assert_parity(got, ref, what=str(kw))
+    assert np.array_equal(got, ref), compare(got, ref)
+
+
+@pytest.mark.parametrize("bits,floating,chroma", [(10, False, True), (16, False, True), (16, False, False),
+                                                  (32, True, True)])
+def test_fft3d_formats(nf, oracle, bits, floating, chroma):
+    rng = np.random.default_rng(bits + chroma)
+    h, w = 64, 96
+    frames = [_rand_plane(rng, h, w, bits, floating) for _ in range(3)]
+    params = dict(bw=32, bh=32, bt=3, ow=16, oh=16, sigma=2.0)
+    plan = nf.fft3d_plan(w, h, bits=bits, floating=floating, chroma=chroma, **params)
+    ref = oracle.fft3d_plane(frames, bits=bits, floating=floating, chroma=chroma, **params)
+    got = _run(plan, frames)
+    assert_parity(got, ref, floating=floating, what=f"bits{bits}")
+    assert np.array_equal(got, ref), compare(got, ref)
+
+
+def test_fft3d_c2_shape_yuv420p16(nf, oracle):
+    clip = yuv420_sinusoids(1920, 1080, frames=3, bits=16, seed=21)
+    params = dict(bt=3, sigma=2.0, bw=32, bh=32, ow=16, oh=16)
+    for p in range(3):
+        planes = [clip[j][p] for j in range(3)]
+        h, w = planes[0].shape
+        plan = nf.fft3d_plan(w, h, bits=16, chroma=p > 0, **params)
+        ref = oracle.fft3d_plane(planes, bits=16, chroma=p > 0, **params)
+        got = _run(plan, planes)
+        assert_parity(got, ref, what=f"C2 plane {p}")
+        assert np.array_equal(got, ref), compare(got, ref)
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(sigma2=1.0, sigma3=3.0, sigma4=5.0), dict(sharpen=0.3)])
+def test_kalman_sequence(nf, oracle, kw):
+    rng = np.random.default_rng(17)
+    h, w = 72, 100
+    params = dict(bw=16, bh=16, bt=0, sigma=3.0, kratio=2.0, **kw)
+    base = _rand_plane(rng, h, w, 8, False).astype(np.int32)
+    frames = [np.clip(base + rng.integers(-6, 7, size=(h, w)), 0, 255).astype(np.uint8) for _ in range(6)]
+    plan = nf.fft3d_plan(w, h, bits=8, **params)
+    st = nf.KalmanState(plan)
+    ref = oracle.KalmanState(params, w, h, bits=8)
+    for f in frames:
+        st.advance(to_device(f))
+        ref.advance(f)
+        last, cov, proc = st.download()
+        assert np.array_equal(last.view(np.float32), ref.last.view(np.float32))
+        assert np.array_equal(cov, ref.cov) and np.array_equal(proc, ref.proc)
+        dst = empty_like_np(f)
+        st.render(to_device(f), dst)
+        got = to_host(dst, np.uint8)
+        assert np.array_equal(got, ref.render()), compare(got, ref.render())
+
+
+def test_nonfinite_input_is_frame_error(nf):
+    img = np.full((40, 48), 0.5, np.float32)
+    img[7, 9] = np.nan
+    plan = nf.dfttest_plan(48, 40, bits=32, floating=True, tbsize=1)
+    with pytest.raises(nf.NonFinite):
+        _run(plan, [img])
+    plan3 = nf.fft3d_plan(48, 40, bits=32, floating=True, bt=1, bw=16, bh=16)
+    with pytest.raises(nf.NonFinite):
+        _run(plan3, [img])
+
+
+def test_batch_matches_single(nf, oracle):
+    import torch
+    clip = yuv420_sinusoids(320, 180, frames=1, bits=16, seed=4)[0]
+    params = dict(tbsize=1, sbsize=16, sosize=12, sigma=8.0)
+    plans, jobs, refs = [], [], []
+    for p, pl in enumerate(clip):
+        h, w = pl.shape
+        plan = nf.dfttest_plan(w, h, bits=16, chroma=p > 0, **params)
+        plans.append(plan)
+        dst = empty_like_np(pl)
+        jobs.append((plan, [to_device(pl)], dst))
+        refs.append(oracle.dfttest_plane([pl], bits=16, **params))
+    flags = torch.zeros(1, dtype=torch.int32, device="cuda")
+    nf.process_batch(jobs, flags)
+    torch.cuda.synchronize()
+    assert int(flags.item()) == 0
+    for (_, _, dst), ref in zip(jobs, refs):
+        assert np.array_equal(to_host(dst, np.uint16), ref)

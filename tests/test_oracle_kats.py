@@ -1,0 +1,585 @@
+"""Pins the CPU oracle (oracle/nf_oracle.c) before it is trusted as the
+checker of the GPU path: the reference's own known-answer tests, golden
+states and worked examples (cited per test), binary64 transforms and closed
+forms. Each check is chosen so that a dropped term, wrong sign or transposed
+index in the oracle fails it."""
+import math
+
+import numpy as np
+import pytest
+
+F = np.float32
+
+
+# ---------------------------------------------------------------- geometry
+def test_axis_kats(oracle):
+    # reference tests/unit/algorithms.cpp:9-17
+    assert oracle.fft3d_axis(64, 8, 2).as_tuple()[4:] == (13, 80, 6)
+    assert oracle.fft3d_axis(48, 6, 2).as_tuple()[4:] == (14, 58, 4)
+    with pytest.raises(RuntimeError):
+        oracle.fft3d_axis(8, 8, 0)
+    d = oracle.dft_axis(17, 8, 1, 6)
+    assert (d.cover, d.offset, d.count) == (36, 9, 15)
+    with pytest.raises(RuntimeError):
+        oracle.dft_axis(17, 8, 1, 5)
+    assert oracle.dft_axis(1, 1, 0, 12).cover == 1
+    with pytest.raises(RuntimeError):
+        oracle.dft_axis(1, 1, 1, 0)
+
+
+def test_benchmark_geometry():
+    """Block grids of the benchmark configurations, recomputed from the
+    documented formulas (docs/knowledge/en/shared/block-geometry.md)."""
+    from oracle import oracle as O
+    x, y = O.dft_axis(640, 16, 1, 12), O.dft_axis(360, 16, 1, 12)
+    assert (x.cover, x.count, x.offset, x.step) == (664, 163, 12, 4)
+    assert (y.cover, y.count, y.offset) == (392, 95, 16)
+    assert x.count * y.count == 15485
+    lx, ly = O.fft3d_axis(1920, 32, 16), O.fft3d_axis(1080, 32, 16)
+    cx, cy = O.fft3d_axis(960, 32, 16), O.fft3d_axis(540, 32, 16)
+    assert (lx.count, ly.count, lx.cover, ly.cover) == (121, 69, 1952, 1120)
+    assert lx.count * ly.count + 2 * cx.count * cy.count == 12619
+
+
+# ------------------------------------------------------------------ windows
+@pytest.mark.parametrize("wtype", [0, 1, 2])
+@pytest.mark.parametrize("b", [4, 8, 9])
+def test_fft3d_window_partition_of_unity(oracle, wtype, b):
+    # algorithms.cpp:18-32
+    for o in (0, 1, b // 2):
+        g = oracle.fft3d_axis(64, b, o)
+        an, sy = oracle.fft3d_window(b, o, wtype)
+        for pos in range(g.offset, g.offset + g.length):
+            w = 0.0
+            for i in range(g.count):
+                local = pos - i * g.step
+                if 0 <= local < b:
+                    w += float(an[local]) * float(sy[local])
+            assert abs(w - 1) <= 5e-7
+
+
+def test_dft_window_kats(oracle):
+    # algorithms.cpp:33-52, :66-78
+    assert abs(oracle.dft_raw_window(9, 0, 2, 2.5) - .74) <= 1e-12
+    assert abs(oracle.dft_raw_window(9, 1, 2, 2.5) - .50) <= 1e-12
+    h, ws = oracle.dft_window_3d(1, 2, 1, 1, 7, 7)
+    assert np.allclose(h, .25, atol=1e-8) and abs(ws - 4) <= 1e-8
+    for wid in range(12):
+        for o in (0, 4, 6):
+            h, _ = oracle.dft_window_3d(1, 8, o, 1, wid, 7)
+            h = h.reshape(8, 8)
+            for py in range(8 - o):
+                for px in range(8 - o):
+                    s = sum(float(h[yy, xx]) ** 2 * 64 for yy in range(py, 8, 8 - o) for xx in range(px, 8, 8 - o))
+                    assert abs(s - 1) <= 3e-7
+    for wid in range(12):
+        h, _ = oracle.dft_window_3d(1, 3, 0, 0, 7, wid)
+        tw = oracle.dft_raw_window(wid, 0, 1, 2.5)
+        assert abs(9.0 * float(h[4]) ** 2 - tw * tw) <= 3e-7
+    assert abs(oracle.dft_window_3d(3, 2, 1, 1, 7, 7)[1] - 4.0) <= 1e-6
+    assert abs(oracle.dft_window_3d(3, 2, 1, 1, 7, 0)[1] - 32.0 / 3.0) <= 1e-4
+    assert math.isnan(oracle.dft_raw_window(4, 0, 8, float(np.finfo(np.float32).max)))
+
+
+def test_temporal_window_normalisation_exact(oracle):
+    # tests/unit/temporal_grid.cpp:24-35: exact binary32 values and wscale
+    for T in range(1, 16):
+        for O in range(T):
+            if O > T // 2 and T % (T - O):
+                continue
+            for wid in (0, 3, 7, 9):
+                h, ws = oracle.dft_window_3d(T, 1, 0, 0, 7, wid, tmode=1, tosize=O)
+                raw = [oracle.dft_raw_window(wid, z, T, 2.5) for z in range(T)]
+                energy = F(0)
+                for z in range(T):
+                    den = sum(raw[k] ** 2 for k in range(z, -1, -(T - O))) + \
+                        sum(raw[k] ** 2 for k in range(z + T - O, T, T - O))
+                    expected = F((raw[z] / math.sqrt(den)) * (1 / math.sqrt(float(T))))
+                    assert h[z] == expected
+                    energy = F(energy + F(expected * expected))
+                assert F(ws) == F(F(1) / energy)
+
+
+# ----------------------------------------------------------------- transforms
+def _dft2(x):
+    return np.fft.rfft2(x.astype(np.float64))
+
+
+@pytest.mark.parametrize("n,tol", [(8, 5e-6), (16, 5e-6), (32, 1.5e-4)])
+def test_rfft2_impulses_against_binary64(oracle, n, tol):
+    # foundation.cpp:112-420: every impulse against a binary64 direct DFT
+    for pos in range(0, n * n, 7):
+        x = np.zeros((n, n), F)
+        x.flat[pos] = 1.0
+        got = oracle.rfft2(x)
+        assert np.abs(got - _dft2(x)).max() <= tol
+
+
+@pytest.mark.parametrize("h,w", [(8, 8), (16, 16), (32, 32), (12, 12), (5, 7), (16, 32), (32, 8)])
+def test_rfft2_random_and_round_trip(oracle, h, w):
+    rng = np.random.default_rng(h * w)
+    x = rng.uniform(-1, 1, (h, w)).astype(F)
+    X = oracle.rfft2(x)
+    ref = _dft2(x)
+    assert np.abs(X - ref).max() <= 3e-5 * max(1.0, np.abs(ref).max() / 16)
+    assert np.abs(oracle.irfft2(X, w) - x).max() <= 2e-6
+
+
+@pytest.mark.parametrize("t", [2, 3, 4, 5, 7])
+def test_rfft3_and_slices(oracle, t):
+    # foundation.cpp:471-612: 3D forward and centre/other slices vs binary64
+    rng = np.random.default_rng(t)
+    v = rng.uniform(-1, 1, (t, 16, 16)).astype(F)
+    X = oracle.rfft3(v)
+    ref = np.fft.rfftn(v.astype(np.float64))
+    assert np.abs(X - ref).max() <= 5e-4
+    for s in range(t):
+        assert np.abs(oracle.irfft3_slice(X, 16, s) - v[s]).max() <= 2e-6
+
+
+def test_fft_conventions_hand_examples(oracle):
+    # docs/knowledge/en/shared/fft-conventions.md: constant -> DC only, unscaled forward
+    x = np.full((8, 8), 2.0, F)
+    X = oracle.rfft2(x)
+    assert X[0, 0] == 128 and np.count_nonzero(X) == 1
+
+
+# -------------------------------------------------------------- gain kernels
+def test_spectral_branch_kats(oracle):
+    # tests/unit/algorithms.cpp:226-275
+    x = oracle.spectral(np.array([10 + 0j]), type=-1, a=64)
+    assert abs(x[0].real - 3.6) < 1e-6
+    x = oracle.spectral(np.array([10 + 0j]), type=-1, a=64, floor=.5)
+    assert abs(x[0].real - 5) < 1e-6
+    x = oracle.spectral(np.array([3 + 4j]), type=0, a=9)
+    assert abs(x[0].real - 1.92) < 1e-6 and abs(x[0].imag - 2.56) < 1e-6
+    x = oracle.spectral(np.array([3 + 4j]), type=0, a=9, exponent=.5)
+    assert abs(x[0].real - 2.4) < 1e-6 and abs(x[0].imag - 3.2) < 1e-6
+    for thr in (np.nextafter(F(25), F(0)), F(25), np.nextafter(F(25), F(30))):
+        x = oracle.spectral(np.array([3 + 4j]), type=1, a=float(thr))
+        assert x[0] == ((3 + 4j) if thr <= 25 else 0)
+    for v in (4.0, 5.0, 10.0, 11.0):
+        x = oracle.spectral(np.array([v + 0j]), type=3, a=2, b=.5, low=25, high=100)
+        assert abs(x[0].real - v * (2 if 5 <= v <= 10 else .5)) < 1e-5
+    x = oracle.spectral(np.array([3 + 4j]), type=4, a=1, low=0, high=100)
+    assert abs(x[0].real - 3 * math.sqrt(.8)) < 1e-6
+    a = F(16) / F(25)
+    for e in (.49994, .49995, .49996, .50004, .50005, .50006, .99994, .99995, .99996, 1.00004, 1.00005, 1.00006, 2.0):
+        ef = F(e)
+        gain = a if abs(ef - F(1)) < F(.00005) else (np.sqrt(a) if abs(ef - F(.5)) < F(.00005) else F(a ** ef))
+        x = oracle.spectral(np.array([3 + 4j]), type=0, a=9, exponent=float(ef))
+        assert abs(x[0].real - F(gain * 3)) <= 1e-6 and abs(x[0].imag - F(gain * 4)) <= 1e-6
+    with pytest.raises(RuntimeError):
+        oracle.spectral(np.array([np.finfo(np.float32).max + 0j]), type=0, a=9, exponent=2.0)
+
+
+def test_enhancement_worked_example(oracle):
+    # docs/knowledge/en/fft3d/noise-enhancement.md:56: S~1.41523, D~0.992366
+    e = dict(sharpen=.5, dehalo=.2, a=16, b=400, c=2500, sharpen_tab=np.ones(1), halo_tab=np.ones(1))
+    x = oracle.spectral(np.array([10 + 0j]), type=-2, enhancement=e)
+    assert abs(x[0].real / 10 - 1.41523 * 0.992366) < 2e-5
+
+
+def test_temporal_filter_identity_and_example(oracle):
+    # algorithms.cpp:80-116
+    for T in (2, 3, 4, 5):
+        frames = [np.array([complex(j * 11 + k * 7 + 1, j * 5 - k * 3) for k in range(8)]) for j in range(T)]
+        out = oracle.fft3d_temporal(frames, T // 2)
+        assert np.abs(out - frames[T // 2]).max() <= 1e-5
+    out = oracle.fft3d_temporal([np.array([1 + 0j]), np.array([2 + 0j]), np.array([6 + 0j])], 1, degrid=1.0,
+                                grid=np.array([1 + 0j]))
+    assert abs(out[0].real - 2.0) <= 1e-5
+
+
+@pytest.mark.parametrize("T", [1, 2, 3, 4, 5])
+def test_temporal_table_against_direct_dft(oracle, T):
+    # fft3d_models.cpp:82-106
+    bins = 17
+    inputs = [np.array([complex(j + k + 2, j - k) for k in range(bins)]) for j in range(T)]
+    p = np.array([(k + 1) * 3 for k in range(bins)], F)
+    out = oracle.fft3d_temporal(inputs, T // 2, noise_table=p, multiplier=float(T), lower=float(F(.1)))
+    for k in range(bins):
+        res = 0j
+        for m in range(T):
+            f = sum(complex(inputs[j][k]) * np.exp(-2j * np.pi * j * m / T) for j in range(T))
+            q = abs(f) ** 2 + 1e-15
+            f *= max((q - T * p[k]) / q, float(F(.1)))
+            res += f * np.exp(2j * np.pi * (T // 2) * m / T) / T
+        assert abs(out[k] - res) <= 2e-5
+
+
+# -------------------------------------------------------------------- tables
+@pytest.mark.parametrize("W,H", [(5, 5), (8, 9), (16, 8), (31, 5), (32, 9)])
+def test_fft3d_profile_closed_form(oracle, W, H):
+    # fft3d_models.cpp:20-34
+    p = oracle.fft3d_profile(W, H, [2, 4, 6, 8])
+    K = W // 2 + 1
+    for y in range(H):
+        for x in range(K):
+            fy = (H - 2 * abs(y - H // 2)) / H
+            fx = x / K
+            f = math.sqrt((fx * fx + fy * fy) / 2)
+            a = math.sqrt(.5) / 4
+            b = 2 * a
+            s = 8 - 2 * f / a if f < a else (6 - 2 * (f - a) / (b - a) if f < b else 2 + 2 * (1 - f) / (1 - b))
+            exp = s * s * W * H
+            assert abs(p[y * K + x] - exp) <= max(.001, exp * 4 * np.finfo(np.float32).eps)
+
+
+@pytest.mark.parametrize("W,H", [(8, 8), (16, 9), (32, 32)])
+def test_enhancement_tables_closed_form(oracle, W, H):
+    # fft3d_models.cpp:35-49
+    sh, ha, abc = oracle.enhancement_tables(W, H, 4.0, sharpen=.4, dehalo=.2)
+    K = W // 2 + 1
+    mx = 0.0
+    d2s = {}
+    for y in range(H):
+        for x in range(K):
+            dy = y if y < H // 2 else H - y
+            d2 = dy * dy / (H // 2 * (H // 2)) + x * x / (W // 2 * (W // 2))
+            d2s[(y, x)] = d2
+            mx = max(mx, math.exp(-.7 * d2 * 4) - math.exp(-d2 * 4))
+    for (y, x), d2 in d2s.items():
+        k = y * K + x
+        assert abs(sh[k] - (1 - math.exp(-d2 / (2 * .3 * .3)))) <= 2e-6
+        assert abs(ha[k] - (math.exp(-.7 * d2 * 4) - math.exp(-d2 * 4)) / mx) <= 2e-6
+    assert abs(abc[0] - 16 * 16 * W * H) <= 1 and abs(abc[1] - 80 * 80 * W * H) <= 16
+
+
+def test_dft_profile_curves(oracle):
+    # profile.cpp: a constant curve reproduces the flat table, separable
+    # products follow the interpolated knots, radial system uses the radius
+    flat = oracle.dft_profile(1, 16, 8.0, 2.0)
+    assert np.all(flat == F(4.0))
+    same = oracle.dft_profile(1, 16, 8.0, 2.0, slocation=[0.0, 8.0, 1.0, 8.0])
+    assert np.allclose(same, 4.0, rtol=1e-6)
+    # two spatial axes: each knot value is raised to 1/2 before interpolation
+    lin = oracle.dft_profile(1, 16, 1.0, 1.0, ssx=[0.0, 0.0, 1.0, 4.0], ssy=[0.0, 1.0, 1.0, 1.0])
+    for y in range(16):
+        for x in range(9):
+            fx = min(x, 16 - x) / 8
+            assert abs(lin[y * 9 + x] - 2 * fx) <= 1e-6
+    rad = oracle.dft_profile(1, 16, 1.0, 1.0, slocation=[0.0, 0.0, 1.0, 1.0], ssystem=1)
+    for y in range(16):
+        for x in range(9):
+            fx, fy = x / 8, min(y, 16 - y) / 8
+            r = math.sqrt((fx * fx + fy * fy) / 2)
+            assert abs(rad[y * 9 + x] - r) <= 1e-6
+
+
+# ---------------------------------------------------------------- Kalman
+GOLDEN_KALMAN = [
+    # tests/unit/phase4_kernels.cpp:83-97 (binary32 golden states)
+    (complex(-22355.0, 1088.8779296875), complex(10412.9765625, -14584.216796875), 178956960.0, 119304656.0,
+     268435456.0, complex(-6833.32763671875, -6335.21923828125), 141281824.0, 74358856.0),
+    (complex(0.4257414937019348, -0.22678810358047485), complex(-0.07621931284666061, 0.021932702511548996),
+     0.020350884646177292, 0.006574865896254778, 0.06299116462469101,
+     complex(0.07409356534481049, -0.05254710465669632), 0.01886279508471489, 0.005648490972816944),
+    (complex(0.4257415533065796, -0.22678810358047485), complex(-0.07621931284666061, 0.021932702511548996),
+     0.020350884646177292, 0.006574865896254778, 0.06299116462469101,
+     complex(0.4257415533065796, -0.22678810358047485), 0.06299116462469101, 0.06299116462469101),
+    (complex(-381535.1875, -69576.8671875), complex(-357165.21875, -62999.546875), 268435456.0, 268435456.0,
+     268435456.0, complex(-373411.875, -67384.421875), 178956960.0, 119304656.0),
+]
+
+
+@pytest.mark.parametrize("case", range(len(GOLDEN_KALMAN)))
+@pytest.mark.parametrize("table", [False, True])
+def test_kalman_golden_states_bit_exact(oracle, case, table):
+    x, last, c, q, r, exp_l, exp_c, exp_q = GOLDEN_KALMAN[case]
+    n = 33
+    L, C, Q = oracle.kalman_step(np.full(n, x, np.complex64), np.full(n, last, np.complex64), np.full(n, c, F),
+                                 np.full(n, q, F), pattern=np.full(n, r, F) if table else None, uniform=r, ratio2=4.0)
+    assert np.all(L == np.complex64(exp_l)) and np.all(C == F(exp_c)) and np.all(Q == F(exp_q))
+
+
+def _kalman_expanded(x, l, c, q, r, ratio2):
+    """Independent restatement of the frozen expanded-component recurrence
+    (tests/unit/kalman_reference.hpp:7-32): separate real/imaginary C and Q."""
+    r, thr = F(r), F(F(r) * F(ratio2))
+    dr, di = F(F(x.real) - F(l.real)), F(F(x.imag) - F(l.imag))
+    if (r == 0) or F(dr * dr) > thr or F(di * di) > thr:
+        return complex(x), (r, r), (r, r)
+    out = []
+    for v, lv, cv, qv in ((x.real, l.real, c[0], q[0]), (x.imag, l.imag, c[1], q[1])):
+        s = F(F(cv) + F(qv))
+        g = F(s / F(s + r))
+        nq = F(F(g * g) * r)
+        nc = F(F(F(1) - g) * s)
+        nl = F(F(g * F(v)) + F(F(F(1) - g) * F(lv)))
+        out.append((nl, nc, nq))
+    return complex(out[0][0], out[1][0]), (out[0][1], out[1][1]), (out[0][2], out[1][2])
+
+
+@pytest.mark.parametrize("signal", [0, 1, 2, 3])
+def test_kalman_long_history_against_expanded_oracle(oracle, signal):
+    # phase4_kernels.cpp:49-78: 512 steps, values adjacent to the strict threshold
+    n = 9
+    L = np.zeros(n, np.complex64)
+    C = np.ones(n, F)
+    Q = np.ones(n, F)
+    sl = [0j] * n
+    sc = [(F(1), F(1))] * n
+    sq = [(F(1), F(1))] * n
+    for frame in range(512):
+        xs = []
+        for i in range(n):
+            v = F(.25)
+            if signal == 1:
+                v = F(F(frame) * F(.002) + F(i) * F(.001))
+            if signal == 2:
+                thr = F(F(sl[i].real) + F(2))
+                v = thr if i % 3 == 0 else np.nextafter(thr, F(np.inf) if i % 3 == 1 else F(-np.inf))
+            if signal == 3:
+                v = F(50) if (frame // 19) % 2 else F(-50)
+            xs.append(complex(v, 0 if signal == 2 else F(v * F(.5))))
+        L, C, Q = oracle.kalman_step(np.array(xs, np.complex64), L, C, Q, uniform=1.0, ratio2=4.0)
+        for i in range(n):
+            sl[i], sc[i], sq[i] = _kalman_expanded(np.complex64(xs[i]), np.complex64(sl[i]), sc[i], sq[i], 1.0, 4.0)
+            assert np.complex64(sl[i]) == L[i]
+            assert sc[i][0] == C[i] and sc[i][1] == C[i] and sq[i][0] == Q[i] and sq[i][1] == Q[i]
+
+
+def test_kalman_two_step_example(oracle):
+    # docs/knowledge/en/fft3d/kalman.md:35-37
+    L, C, Q = oracle.kalman_step(np.array([1 + 0j], np.complex64), np.zeros(1, np.complex64), np.full(1, 2, F),
+                                 np.full(1, 2, F), uniform=2.0, ratio2=4.0)
+    assert abs(L[0].real - 2 / 3) < 1e-6 and abs(C[0] - 4 / 3) < 1e-6 and abs(Q[0] - 8 / 9) < 1e-6
+    L, C, Q = oracle.kalman_step(np.array([1 + 0j], np.complex64), L, C, Q, uniform=2.0, ratio2=4.0)
+    assert abs(L[0].real - 16 / 19) < 1e-6 and abs(C[0] - 20 / 19) < 1e-6 and abs(Q[0] - 200 / 361) < 1e-6
+    L, C, Q = oracle.kalman_step(np.array([3 + 0j], np.complex64), np.zeros(1, np.complex64), np.full(1, 2, F),
+                                 np.full(1, 2, F), uniform=2.0, ratio2=4.0)
+    assert L[0] == 3 and C[0] == 2 and Q[0] == 2
+
+
+# ------------------------------------------------------------------ dither
+def test_dither_hash_kats(oracle):
+    # tests/vapoursynth/phase4_dither.py:39, phase4_kernels.cpp:122
+    for coords, h in (((0, 0, 0, 0, 0), 0x642a1d14), ((0, 1, 0, 0, 0), 0xd32e9dca), ((1, 0, 0, 0, 0), 0xf383f4c5),
+                      ((17, 23, 2, 5, 7), 0x9eb639f9), ((2147483647, 2147483646, 2, 107, 255), 0x5064b117)):
+        assert oracle.dither_hash(*coords) == h
+    assert oracle.dither_hash(17, 23, 2, 5, 7) >> 8 == 10401337
+
+
+def _dither_numpy(a, mode, seed, n, p, oracle):
+    """Independent binary32 restatement of docs/knowledge/en/dfttest/dither.md."""
+    h, w = a.shape
+    out = np.empty((h, w), np.uint8)
+    cur = np.zeros(w, F)
+    scale = F(F(mode - 1) + F(.5))
+    off = F(scale * F(.5))
+    for y in range(h):
+        nxt = np.zeros(w, F)
+        for x in range(w):
+            e = F(a[y, x])
+            if mode == 1:
+                v = F(F(e + cur[x]) + F(.5))
+            else:
+                u = F(F(oracle.dither_hash(seed, n, p, y, x) >> 8) * F(2 ** -24))
+                v = F(F(F(F(e + F(u * scale)) - off) + cur[x]) + F(.5))
+            d = int(np.clip(v, 0, 255))
+            out[y, x] = d
+            err = F(e - F(d))
+            if x:
+                nxt[x - 1] = F(nxt[x - 1] + F(err * F(.1875)))
+            nxt[x] = F(nxt[x] + F(err * F(.3125)))
+            if x + 1 < w:
+                cur[x + 1] = F(cur[x + 1] + F(err * F(.4375)))
+                nxt[x + 1] = F(nxt[x + 1] + F(err * F(.0625)))
+        cur = nxt
+    return out
+
+
+def test_dither_against_independent_restatement(oracle):
+    assert list(oracle.dither(np.array([[10.4, 10.4]], F), 1)[0]) == [10, 11]  # dither.md:26
+    rng = np.random.default_rng(1)
+    for w in (1, 7, 17, 33):
+        a = (rng.integers(0, 256, (5, w)) * F(.375)).astype(F)
+        for mode in (1, 2, 4, 2147483647):
+            for seed in (0, 17):
+                assert np.array_equal(oracle.dither(a, mode, seed, 3, 1), _dither_numpy(a, mode, seed, 3, 1, oracle))
+
+
+# -------------------------------------------------------- temporal lattice
+def test_temporal_lattice_enumeration(oracle):
+    # tests/unit/temporal_grid.cpp:8-23
+    for T in range(1, 16):
+        for O in range(T):
+            if O > T // 2 and T % (T - O):
+                with pytest.raises(RuntimeError):
+                    oracle.temporal_blocks(0, T, T, O)
+                continue
+            for N in (T, T + 1, 3 * T + 7):
+                for n in range(N):
+                    blocks = oracle.temporal_blocks(n, N, T, O)
+                    expected = [s for s in range(-max(T - O, O), n + 1, T - O) if n < s + T]
+                    assert [b[0] for b in blocks] == expected
+                    for start, target, slots in blocks:
+                        assert target == n - start and slots[target] == n
+                        assert slots == [min(max(start + z, 0), N - 1) for z in range(T)]
+
+
+# ------------------------------------------------------------ plane pipelines
+def test_fft3d_4x4_worked_example(oracle):
+    # docs/knowledge/en/fft3d.md:128-150 -> repeated rows [10,13]
+    img = np.tile(np.array([10, 14], np.uint8), (16, 8))
+    out = oracle.fft3d_plane([img], bits=8, bw=4, bh=4, ow=0, oh=0, bt=1, sigma=4, beta=1, degrid=0)
+    assert np.array_equal(out, np.tile(np.array([10, 13], np.uint8), (16, 8)))
+
+
+@pytest.mark.parametrize("bits,floating", [(8, False), (10, False), (16, False), (32, True)])
+def test_identity_paths(oracle, bits, floating):
+    # algorithms.cpp:276-327: FFT3D sigma=0 and DFTTest ftype=2/sigma=1 reproduce the input
+    w, h = 35, 29
+    y, x = np.mgrid[0:h, 0:w]
+    if floating:
+        img = (((x * 11 + y * 7) % 29) / 20 - .25).astype(F)
+    else:
+        img = (((x * 11 + y * 7) % 251) << (bits - 8)).astype(np.uint8 if bits == 8 else np.uint16)
+    for wt in (0, 1, 2):
+        out = oracle.fft3d_plane([img], bits=bits, floating=floating, chroma=True, bw=8, bh=6, ow=4, oh=2,
+                                 wintype=wt, sigma=0, bt=1)
+        if floating:
+            assert np.abs(out - np.clip(img, 0, 1)).max() <= 1e-6
+        else:
+            assert np.array_equal(out, img)
+    for mode in (0, 1):
+        out = oracle.dfttest_plane([img], bits=bits, floating=floating, tbsize=1, sbsize=8 if mode else 3,
+                                   sosize=6 if mode else 12, smode=mode, ftype=2, sigma=1, swin=7, zmean=False)
+        if floating:
+            assert np.abs(out - img).max() <= 1e-6
+        else:
+            assert np.array_equal(out, img)
+
+
+def test_temporal_plan_identities(oracle):
+    # algorithms.cpp:119-197
+    q = [np.array([[v]], np.uint8) for v in (1, 2, 6)]
+    kw = dict(sbsize=1, smode=0, tbsize=3, swin=7, twin=7, ftype=2, sigma=0)
+    assert oracle.dfttest_plane(q, bits=8, zmean=True, **kw)[0, 0] == 3
+    assert oracle.dfttest_plane(q, bits=8, zmean=False, **kw)[0, 0] == 0
+    frames = [np.full((16, 16), v, np.uint8) for v in (10, 20, 30)]
+    out = oracle.fft3d_plane(frames, bits=8, bw=8, bh=8, ow=4, oh=4, bt=3, sigma=0)
+    assert np.all(out == 20)
+    out = oracle.dfttest_plane(frames, bits=8, sbsize=8, sosize=4, smode=1, tbsize=3, swin=7, twin=7, ftype=2,
+                               sigma=1.0, zmean=False)
+    assert np.all(out == 20)
+
+
+@pytest.mark.parametrize("S", [8, 16, 32])
+@pytest.mark.parametrize("T", [1, 3, 5])
+def test_dfttest_constant_gain_scales_centre(oracle, S, T):
+    # algorithms.cpp:572-596: ftype=2, no mean removal
+    w, h = 128, 64
+    y, x = np.mgrid[0:h, 0:w]
+    frames = [(0.2 + ((x * 7 + y * 11 + z * 17) % 101) / 255.0).astype(F) for z in range(T)]
+    for gain in (0.0, 0.7, 1.25):
+        out = oracle.dfttest_plane(frames, bits=32, floating=True, sbsize=S, sosize=S // 2, tbsize=T, ftype=2,
+                                   sigma=gain, zmean=False)
+        assert np.abs(out - frames[T // 2] * F(gain)).max() <= 2e-6
+
+
+def _composition_expected(inputs, W, H, T, degrid, model, n=3):
+    """numpy binary64 restatement of the FFT3D composition (reference
+    tests/vapoursynth/phase3_acceptance.py:10-57; rectangular windows)."""
+    ky, kx = np.indices((H, W // 2 + 1))
+    norm = 1 / (W * H)
+    fy = (H - 2 * np.abs(ky - H // 2)) / H
+    fx = kx / (W // 2 + 1)
+    f = np.sqrt((fx * fx + fy * fy) / 2)
+    a = np.sqrt(.5) / 4
+    b = 2 * a
+    sig = np.where(f < a, 5 + (4 - 5) * f / a, np.where(f < b, 4 + (3 - 4) * (f - a) / (b - a),
+                                                        2 + (3 - 2) * (1 - f) / (1 - b))) / 255
+    analytic = sig * sig / norm
+    dy = np.where(ky < H // 2, ky, H - ky)
+    d2 = dy ** 2 / (H // 2) ** 2 + kx ** 2 / (W // 2) ** 2
+    ws = 1 - np.exp(-d2 / (2 * .3 ** 2))
+    wh = np.exp(-.7 * d2 * 4) - np.exp(-d2 * 4)
+    wh /= wh.max()
+    A = (4 / 255) ** 2 / norm
+    B = (20 / 255) ** 2 / norm
+    C = .1 ** 2 / norm
+
+    def residual(z):
+        mean = np.zeros_like(z)
+        if degrid:
+            mean[0, 0] = z[0, 0].real
+        return z - mean, mean
+
+    def gain(q):
+        return (1 + .4 * ws * np.sqrt(q * B / ((q + A) * (q + B)))) * (q + C) / (q + C + .2 * wh * q)
+
+    power = analytic if model == "analytic" else (2 / 255) ** 2 / norm
+    spatial = np.fft.rfft2(inputs[n][:H, :W].astype(np.float64))
+    if T <= 1:
+        R, M = residual(spatial)
+        q = np.abs(R) ** 2 + 1e-15
+        if T == 1:
+            wiener = np.maximum((q - power) / q, 0)
+            if model == "uniform":
+                result = R * wiener * gain(q) + M
+            else:
+                R, M = residual(R * wiener + M)
+                result = R * gain(np.abs(R) ** 2 + 1e-15) + M
+        else:
+            result = R * gain(q) + M
+    else:
+        spectra = np.stack([np.fft.rfft2(inputs[k][:H, :W].astype(np.float64))
+                            for k in range(n - T // 2, n + (T - 1) // 2 + 1)])
+        temporal = np.fft.fft(spectra, axis=0)
+        removed = np.zeros_like(temporal)
+        if degrid:
+            removed[0, 0, 0] = T * spatial[0, 0].real
+        R = temporal - removed
+        q = np.abs(R) ** 2 + 1e-15
+        spatial = np.fft.ifft(R * np.maximum((q - T * power) / q, 0) + removed, axis=0)[T // 2]
+        R, M = residual(spatial)
+        result = R * gain(np.abs(R) ** 2 + 1e-15) + M
+    return np.clip(np.fft.irfft2(result, s=(H, W)), 0, 1)
+
+
+@pytest.mark.parametrize("W,H", [(4, 4), (8, 8)])
+@pytest.mark.parametrize("degrid", [0, 1])
+@pytest.mark.parametrize("model", ["uniform", "analytic"])
+@pytest.mark.parametrize("T", [-1, 1, 2, 3, 4, 5])
+def test_fft3d_composition_against_numpy(oracle, W, H, degrid, model, T):
+    rng = np.random.default_rng(61)
+    # 64x48 planes; with ow=oh=0 the first visible block is exactly [0,H)x[0,W)
+    inputs = [rng.uniform(0.1, 0.9, (48, 64)).astype(F) for _ in range(7)]
+    kw = dict(bw=W, bh=H, ow=0, oh=0, bt=T, sigma=2, degrid=degrid, sharpen=.4, dehalo=.2, ht=.1)
+    if model == "analytic":
+        kw.update(sigma2=3, sigma3=4, sigma4=5)
+    Tn = max(1, T)
+    slots = [inputs[k] for k in range(3 - Tn // 2, 3 + (Tn - 1) // 2 + 1)]
+    got = oracle.fft3d_plane(slots, bits=32, floating=True, **kw)[:H, :W]
+    want = _composition_expected(inputs, W, H, T, degrid, model)
+    assert np.abs(got - want).max() <= 4e-6
+
+
+def test_dfttest_tmode1_closed_forms(oracle):
+    # tests/vapoursynth/phase5_temporal.py:19-40: B=1 temporal overlap-add closed form
+    rng_frames = [(((n * n * 7 + np.indices((3, 5))[0] * 13 + np.indices((3, 5))[1] * 19) % 113) / 128.).astype(F)
+                  for n in range(31)]
+    for T in (1, 2, 3, 4, 5, 6, 9):
+        for O in range(T):
+            if O > T // 2 and T % (T - O):
+                continue
+            h = np.array([1 / np.sqrt(len(range(z % (T - O), T, T - O))) / np.sqrt(T) for z in range(T)], F)
+            for mean in (False, True):
+                for n in sorted({0, 1, T - 1, T, 29, 30}):
+                    blocks = oracle.temporal_blocks(n, 31, T, O)
+                    slots = [rng_frames[s] for _, _, sl in blocks for s in sl]
+                    targets = [t for _, t, _ in blocks]
+                    got = oracle.dfttest_plane(slots, bits=32, floating=True, targets=targets, tmode=1, tbsize=T,
+                                               tosize=O, smode=0, sbsize=1, swin=7, twin=7, ftype=2,
+                                               sigma=0 if mean else .375, zmean=mean)
+                    expected = np.zeros((3, 5))
+                    for s, t, _ in blocks:
+                        if mean:
+                            value = sum(rng_frames[max(0, min(30, s + z))] * float(h[z]) for z in range(T)) / float(
+                                sum(h))
+                        else:
+                            value = rng_frames[n] * .375
+                        expected += value * (T * float(h[t]) ** 2)
+                    assert np.abs(got - expected).max() < 3e-6, (T, O, mean, n)
