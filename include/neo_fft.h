@@ -144,6 +144,18 @@ nf_status nf_process_batch(const nf_plan* const* plans, int n_jobs, const nf_pla
                            const int* frames, const int* planes, unsigned* device_flags,
                            void* stream);
 
+/* DFTTest tmode=1 with tosize=0: every frame of the lattice block
+ * [s, s+tbsize) has that block as its only covering block (reference
+ * src/algorithms/temporal_grid.hpp:24-40), so one pass over its tbsize
+ * sources (slot z = frame clamp(s+z)) writes the outputs of frames s+z to
+ * dsts[z] (data NULL skips a slice; outputs share one stride). Results are
+ * bit-identical to nf_plan_process of each frame. No counterpart in the
+ * reference, which recomputes the block per output frame (plan.cpp:545-606).
+ * Asynchronous like nf_plan_enqueue. */
+nf_status nf_dfttest_enqueue_volume(const nf_plan* plan, const nf_plane* sources, int n_sources,
+                                    const nf_mut_plane* dsts, int first_frame, int plane,
+                                    unsigned* device_flags, void* stream);
+
 /* FFT3D Kalman mode (bt=0): device state of one plane (reference
  * KalmanState, src/kernels/kalman.hpp:5-14: complex L + float C + float Q,
  * 16 bytes per bin). create = Plan::initial_kalman (plan.cpp:293-300). */
@@ -168,6 +180,55 @@ nf_status nf_kalman_state_download(const nf_kalman_state* state, float* last, fl
  * 1 FFT3D block kernel, 2 FFT3D synthesis kernel, 3 other. */
 nf_status nf_profile_enable(int enable);
 nf_status nf_profile_read(double ms[4], long long counts[4], int reset);
+
+/* ------------------------------------------------------------------------
+ * Filter instances: the host layer above the plan boundary, restating the
+ * reference's Filter<A> (src/plugin/filter.hpp): plane selection
+ * (descriptors.hpp:142-160), temporal slots and end-of-clip handling
+ * (filter.hpp:253-307, 401-445), the DFTTest tmode=1 lattice
+ * (temporal_grid.hpp:24-40), FFT3D ROI (roi.hpp:13-22) and the Kalman replay
+ * with bounded warmup and checkpoints (filter.hpp:152-252,
+ * src/runtime/checkpoints.hpp:32-79). Frames come from a caller callback,
+ * the VapourSynth getFrame model; planes are device memory. */
+typedef enum nf_color_family { NF_COLOR_GRAY = 0, NF_COLOR_YUV = 1, NF_COLOR_RGB = 2 } nf_color_family;
+
+typedef struct nf_video_info {
+  int color_family; /* nf_color_family */
+  int bits, floating;
+  int subsampling_w, subsampling_h; /* log2, YUV chroma only */
+  int plane_count;                  /* 1, 3 or 4 */
+  int width, height, num_frames;
+} nf_video_info;
+
+typedef struct nf_frame {
+  nf_plane planes[4];
+} nf_frame;
+typedef struct nf_mut_frame {
+  nf_mut_plane planes[4];
+} nf_mut_frame;
+
+/* Fills *out with source frame n (device planes valid until the request that
+ * asked for it returns). Returns NF_OK or an error status. */
+typedef nf_status (*nf_frame_source)(void* user, int n, nf_frame* out);
+
+typedef struct nf_filter nf_filter;
+
+/* planes: explicit plane list (NULL = every non-alpha plane, reference
+ * default); unselected planes are copied from frame n. */
+nf_status nf_filter_create_dfttest(const nf_video_info* info, const nf_dfttest_config* config,
+                                   const int* planes, int n_planes, nf_filter** out);
+nf_status nf_filter_create_fft3d(const nf_video_info* info, const nf_fft3d_config* config,
+                                 const int* planes, int n_planes, nf_filter** out);
+void nf_filter_destroy(nf_filter* filter);
+/* Filter::request: the sorted distinct source frames output n reads (Kalman
+ * requests read frames one at a time during replay; n itself is listed).
+ * Returns the count (frames[] filled up to capacity) or -1. */
+int nf_filter_request(const nf_filter* filter, int n, int* frames, int capacity);
+/* Filter::process for output n into dst (synchronous on stream). */
+nf_status nf_filter_get_frame(nf_filter* filter, int n, nf_frame_source source, void* user,
+                              const nf_mut_frame* dst, void* stream);
+/* Frames of the Kalman checkpoints currently retained (diagnostics). */
+int nf_filter_checkpoints(const nf_filter* filter, int* frames, int capacity);
 
 /* KernelInfo (reference vapoursynth.cpp:64-69): writes a NUL-terminated
  * "key=value;..." description of device, arch and kernel variants. */

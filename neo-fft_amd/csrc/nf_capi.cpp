@@ -60,6 +60,8 @@ hipError_t g_lds_status = hipSuccess;
 void ensure_lds_limits() {
   std::call_once(g_lds_once, [] {
     g_lds_status = dfttest_t1_set_lds_limit();
+    if (g_lds_status == hipSuccess) g_lds_status = dfttest_vol_set_lds_limit();
+    if (g_lds_status == hipSuccess) g_lds_status = dither_set_lds_limit();
     if (g_lds_status == hipSuccess) g_lds_status = fft3d_set_lds_limit();
   });
   hip_check(g_lds_status, "hipFuncSetAttribute");
@@ -118,6 +120,12 @@ struct nf_plan {
   const float2* d_grid = nullptr;
   int tile_w = 64, tile_h = 64, nbx_max = 0, ww_max = 0;
   size_t lds = 0;
+  int tmode = 0, tosize = 0;
+  const float* d_ttw = nullptr;
+  // temporal kernel tiling per mode: 0 per-frame ring, 1 per-frame tile (several
+  // covering temporal blocks), 2 all slices of one volume
+  int vol_tile_w[3] = {64, 64, 64}, vol_tile_h[3] = {128, 32, 128}, vol_chunk[3] = {1, 1, 1};
+  size_t vol_lds[3] = {0, 0, 0};
   // FFT3D
   int bw = 0, bh = 0, bt = 1, kalman = 0, denoise = 1, varying = 0, enh_active = 0;
   float base = 0, degrid = 0, lower = 0, sigma_eff = 0, norm = 0, r0 = 0, ratio2 = 0;
@@ -137,6 +145,8 @@ struct nf_kalman_state {
 };
 
 extern "C" const char* nf_last_error(void) { return g_error.c_str(); }
+
+void nf::set_last_error(const std::string& message) { g_error = message; }
 
 extern "C" void nf_dfttest_config_init(nf_dfttest_config* c) {  // descriptors.hpp:200-235
   std::memset(c, 0, sizeof *c);
@@ -248,6 +258,20 @@ void choose_dft_tiling(nf_plan& p) {
   j.tile_w = p.tile_w;
   p.lds = dfttest_t1_lds_bytes(p.B, j);
   require(p.lds <= 160 * 1024, "DFTTest tile does not fit the LDS");
+  if (p.temporal > 1) {
+    const int T = p.temporal;
+    const size_t budget = 76 * 1024;  // two workgroups per CU
+    for (int m = 0; m < 3; ++m) {
+      const int nout = m == 2 ? T : 1;
+      const int acc_rows = m == 1 ? p.vol_tile_h[m] : p.B;
+      const int nbx = (p.vol_tile_w[m] + p.B - 2) / p.S + 1;
+      int c = 1;
+      while (c < nbx && dfttest_vol_lds_bytes(p.B, T, c + 1, p.S, nout, acc_rows, p.vol_tile_w[m]) <= budget) ++c;
+      p.vol_chunk[m] = c;
+      p.vol_lds[m] = dfttest_vol_lds_bytes(p.B, T, c, p.S, nout, acc_rows, p.vol_tile_w[m]);
+      require(p.vol_lds[m] <= 160 * 1024, "DFTTest temporal tile does not fit the LDS");
+    }
+  }
 }
 }  // namespace
 
@@ -273,14 +297,13 @@ extern "C" nf_status nf_dfttest_plan_create(int width, int height, nf_sample_for
     p->S = p->gx.step;
     p->dither = c.dither;
     p->dither_seed = c.dither_seed;
-    // Device coverage of this path: spatial tbsize=1, smode=1, 8/16/32-point blocks.
-    if (c.tbsize != 1) throw Unsupported("DFTTest tbsize>1 is not available on this device path yet");
+    p->tmode = c.tmode;
+    p->tosize = c.tmode ? c.tosize : 0;
+    // Device coverage of this path: smode=1 with 8/16/32-point blocks.
     if (c.smode != 1) throw Unsupported("DFTTest smode=0 is not available on this device path yet");
     if (!(c.sbsize == 8 || c.sbsize == 16 || c.sbsize == 32))
       throw Unsupported("DFTTest sbsize must be 8, 16 or 32 on this device path");
     if (c.n_nlocation) throw Unsupported("DFTTest nlocation noise sampling is not available on this device path yet");
-    if (c.dither > 0 && !fmt.floating && fmt.bits == 8)
-      throw Unsupported("DFTTest dither is not available on this device path yet");
     const int T = c.tbsize, B = c.sbsize;
     // plan.cpp:188-226
     const auto win = dft_window_3d(T, B, c.smode == 0 ? 0 : c.sosize, c.smode, c.swin, c.twin, c.sbeta, c.tbeta,
@@ -303,13 +326,15 @@ extern "C" nf_status nf_dfttest_plan_create(int width, int height, nf_sample_for
     if (c.zmean) {
       std::vector<float> block(win.h.size());
       for (size_t i = 0; i < block.size(); ++i) block[i] = finite(255.0f * win.h[i]);
-      grid = rfft2_host(B, B, block.data());
+      grid = T > 1 ? rfft3_host(T, B, B, block.data()) : rfft2_host(B, B, block.data());
       require(grid[0].real() != 0, "DFTTest zmean requires nonzero template DC");
     }
+    const size_t o_ttw = p->tables.add(temporal_twiddles(T));
     const size_t o_h = p->tables.add(win.h), o_hs = p->tables.add(hsyn);
     const size_t o_pr = primary.empty() ? 0 : p->tables.add(primary);
     const size_t o_g = grid.empty() ? 0 : p->tables.add(grid);
     p->tables.upload();
+    p->d_ttw = p->tables.at<float>(o_ttw);
     p->d_h = p->tables.at<float>(o_h);
     p->d_hsyn = p->tables.at<float>(o_hs);
     g.primary = primary.empty() ? nullptr : p->tables.at<float>(o_pr);
@@ -681,6 +706,82 @@ void enqueue_dft_jobs(const std::vector<const nf_plan*>& plans, const std::vecto
   }
 }
 
+// Temporal DFTTest inputs: distinct frames (pointer + stride) and slot map.
+DftTemporal make_temporal(const nf_plan& p, const nf_plane* sources, int n_sources, const int* targets,
+                          int n_targets, int mode) {
+  DftTemporal q{};
+  const int T = p.temporal;
+  q.T = T;
+  q.tmode1 = p.tmode == 1;
+  q.emit_all = mode == 2;
+  q.nout = mode == 2 ? T : 1;
+  if (p.tmode == 1 && mode != 2) {
+    require(n_targets >= 1 && n_targets <= 15 && n_sources == n_targets * T, "invalid temporal block inputs");
+    for (int i = 0; i < n_targets; ++i) {
+      require(targets[i] >= 0 && targets[i] < T, "invalid target temporal slice");
+      q.targets[i] = targets[i];
+    }
+    q.ntb = n_targets;
+  } else {
+    require(n_targets == 0 || mode == 2, "invalid temporal block inputs");
+    require(n_sources == T, "DFTTest sources size must match plan temporal_size");
+    q.ntb = 1;
+  }
+  require(q.ntb * T <= 225, "too many temporal slots");
+  int nf = 0;
+  for (int i = 0; i < n_sources; ++i) {
+    int f = 0;
+    while (f < nf && !(q.frames[f] == sources[i].data && q.frame_stride[f] == sources[i].stride_bytes)) ++f;
+    if (f == nf) {
+      require(nf < 30, "too many distinct temporal source frames");
+      q.frames[nf] = sources[i].data;
+      q.frame_stride[nf] = sources[i].stride_bytes;
+      ++nf;
+    }
+    q.slot_frame[i] = (unsigned char)f;
+  }
+  q.ttw = p.d_ttw;
+  const int m = mode == 2 ? 2 : (q.ntb > 1 ? 1 : 0);
+  q.chunk = p.vol_chunk[m];
+  return q;
+}
+
+DftJob retile(DftJob j, const nf_plan& p, int m) {
+  j.tile_w = p.vol_tile_w[m];
+  j.tile_h = p.vol_tile_h[m];
+  j.tile_x0 = p.gx.offset / j.tile_w;
+  j.tile_y0 = p.gy.offset / j.tile_h;
+  j.tiles_x = (p.gx.offset + p.gx.length + j.tile_w - 1) / j.tile_w - j.tile_x0;
+  j.tiles_y = (p.gy.offset + p.gy.length + j.tile_h - 1) / j.tile_h - j.tile_y0;
+  return j;
+}
+
+void enqueue_dft_vol_jobs(const std::vector<const nf_plan*>& plans, const std::vector<DftJob>& jobs,
+                          const std::vector<DftTemporal>& tq, unsigned* flags, hipStream_t stream) {
+  ensure_lds_limits();
+  for (size_t first = 0; first < jobs.size(); first += kMaxVolJobs) {
+    DftVolLaunch L{};
+    const size_t n = std::min<size_t>(kMaxVolJobs, jobs.size() - first);
+    int tiles = 0;
+    size_t lds = 0;
+    const int B = plans[first]->B;
+    for (size_t i = 0; i < n; ++i) {
+      const nf_plan& p = *plans[first + i];
+      require(p.B == B, "batched DFTTest jobs must share sbsize");
+      const int m = tq[first + i].emit_all ? 2 : (tq[first + i].ntb > 1 ? 1 : 0);
+      L.jobs[i] = retile(jobs[first + i], p, m);
+      L.tq[i] = tq[first + i];
+      L.tile_start[i] = tiles;
+      tiles += L.jobs[i].tiles_x * L.jobs[i].tiles_y;
+      lds = std::max(lds, p.vol_lds[m]);
+    }
+    L.tile_start[n] = tiles;
+    L.njobs = (int)n;
+    L.flags = flags;
+    timed(0, stream, [&] { hip_check(launch_dfttest_vol(B, L, tiles, lds, stream), "dfttest temporal kernel launch"); });
+  }
+}
+
 void enqueue_fft3d_jobs(const std::vector<const nf_plan*>& plans, std::vector<Fft3dJob> jobs, unsigned* flags,
                         hipStream_t stream, bool synthesize) {
   ensure_lds_limits();
@@ -727,16 +828,38 @@ nf_status enqueue_one(const nf_plan* plan, const nf_plane* sources, int n_source
                       nf_mut_plane dst, int frame, int plane, unsigned* flags, hipStream_t stream) {
   require(plan && sources && n_sources > 0, "sources must not be empty");
   require(flags != nullptr, "device flag word required");
-  (void)targets;
-  (void)frame;
-  (void)plane;
   check_plane(*plan, dst.width, dst.height, dst.stride_bytes, dst.data);
   for (int i = 0; i < n_sources; ++i)
     check_plane(*plan, sources[i].width, sources[i].height, sources[i].stride_bytes, sources[i].data);
   if (plan->algorithm == 0) {
-    require(n_targets == 0, "invalid temporal block inputs");
-    require(n_sources == plan->temporal, "DFTTest sources size must match plan temporal_size");
-    enqueue_dft_jobs({plan}, {make_dft_job(*plan, sources[0], dst)}, flags, stream);
+    DftJob j = make_dft_job(*plan, sources[0], dst);
+    // 8-bit dither (plan.cpp:619-623): the float crop goes through a
+    // stream-ordered buffer into the wavefront scan
+    const bool dither = plan->dither > 0 && plan->storage == kU8;
+    float* acc = nullptr;
+    if (dither) {
+      hip_check(hipMallocAsync((void**)&acc, sizeof(float) * (size_t)plan->width * plan->height, stream),
+                "hipMallocAsync(dither)");
+      j.acc_out = acc;
+      j.acc_stride = plan->width;
+    }
+    if (plan->temporal == 1) {
+      require(n_targets == 0 || (plan->tmode == 1 && n_targets == 1 && targets[0] == 0), "invalid temporal block inputs");
+      require(n_sources == 1, "DFTTest sources size must match plan temporal_size");
+      enqueue_dft_jobs({plan}, {j}, flags, stream);
+    } else {
+      const DftTemporal q = make_temporal(*plan, sources, n_sources, targets, n_targets, 0);
+      enqueue_dft_vol_jobs({plan}, {j}, {q}, flags, stream);
+    }
+    if (dither) {
+      require(frame >= 0 && plane >= 0, "invalid dither coordinates or mode");
+      timed(3, stream, [&] {
+        hip_check(launch_dither(acc, plan->width, plan->width, plan->height, static_cast<uint8_t*>(dst.data),
+                                dst.stride_bytes, plan->dither, plan->dither_seed, frame, plane, stream),
+                  "dither kernel launch");
+      });
+      hip_check(hipFreeAsync(acc, stream), "hipFreeAsync(dither)");
+    }
   } else {
     require(!plan->kalman, "Kalman plans use nf_kalman_advance / nf_kalman_render");
     require(n_sources == plan->temporal || n_sources == 1,
@@ -784,6 +907,7 @@ extern "C" nf_status nf_process_batch(const nf_plan* const* plans, int n_jobs, c
     std::vector<const nf_plan*> pv;
     std::vector<DftJob> dj;
     std::vector<Fft3dJob> fj;
+    std::vector<DftTemporal> tq;
     for (int i = 0; i < n_jobs; ++i) {
       const nf_plan* p = plans[i];
       require(p && p->algorithm == alg, "batched jobs must use one filter");
@@ -793,8 +917,11 @@ extern "C" nf_status nf_process_batch(const nf_plan* const* plans, int n_jobs, c
       for (int k = 0; k < ns; ++k) check_plane(*p, s[k].width, s[k].height, s[k].stride_bytes, s[k].data);
       pv.push_back(p);
       if (alg == 0) {
+        require(p->tmode == 0 || p->temporal == 1, "batched DFTTest jobs take tmode=0 planes (use nf_plan_enqueue)");
+        require(!(p->dither > 0 && p->storage == kU8), "dithered DFTTest planes use nf_plan_enqueue");
         require(ns == p->temporal, "DFTTest sources size must match plan temporal_size");
         dj.push_back(make_dft_job(*p, s[0], dsts[i]));
+        if (p->temporal > 1) tq.push_back(make_temporal(*p, s, ns, nullptr, 0, 0));
       } else {
         require(!p->kalman, "Kalman plans use nf_kalman_advance / nf_kalman_render");
         require(ns == p->temporal || ns == 1, "FFT3D sources size must match plan temporal_size or single-frame fallback");
@@ -808,8 +935,45 @@ extern "C" nf_status nf_process_batch(const nf_plan* const* plans, int n_jobs, c
         fj.push_back(j);
       }
     }
-    if (alg == 0) enqueue_dft_jobs(pv, dj, device_flags, (hipStream_t)stream);
-    else enqueue_fft3d_jobs(pv, fj, device_flags, (hipStream_t)stream, true);
+    if (alg == 0) {
+      const bool temporal = pv[0]->temporal > 1;
+      for (auto* p : pv) require((p->temporal > 1) == temporal, "batched DFTTest jobs must share tbsize");
+      if (temporal) enqueue_dft_vol_jobs(pv, dj, tq, device_flags, (hipStream_t)stream);
+      else enqueue_dft_jobs(pv, dj, device_flags, (hipStream_t)stream);
+    } else {
+      enqueue_fft3d_jobs(pv, fj, device_flags, (hipStream_t)stream, true);
+    }
+    return NF_OK;
+  });
+}
+
+extern "C" nf_status nf_dfttest_enqueue_volume(const nf_plan* plan, const nf_plane* sources, int n_sources,
+                                               const nf_mut_plane* dsts, int first_frame, int plane,
+                                               unsigned* device_flags, void* stream) {
+  return guarded([&]() -> nf_status {
+    require(plan && plan->algorithm == 0 && sources && dsts && device_flags, "null argument");
+    require(plan->tmode == 1 && plan->tosize == 0 && plan->temporal > 1,
+            "volume processing needs a tmode=1, tosize=0, tbsize>1 DFTTest plan");
+    require(!(plan->dither > 0 && plan->storage == kU8), "dithered DFTTest planes use nf_plan_enqueue");
+    (void)first_frame;
+    (void)plane;
+    for (int i = 0; i < n_sources; ++i)
+      check_plane(*plan, sources[i].width, sources[i].height, sources[i].stride_bytes, sources[i].data);
+    DftTemporal q = make_temporal(*plan, sources, n_sources, nullptr, 0, 2);
+    bool any = false;
+    for (int z = 0; z < plan->temporal; ++z) {
+      q.dsts[z] = dsts[z].data;
+      if (dsts[z].data) {
+        check_plane(*plan, dsts[z].width, dsts[z].height, dsts[z].stride_bytes, dsts[z].data);
+        any = true;
+      }
+    }
+    require(any, "no output slice requested");
+    nf_mut_plane first{};
+    for (int z = 0; z < plan->temporal && !first.data; ++z) first = dsts[z];
+    for (int z = 0; z < plan->temporal; ++z)
+      if (dsts[z].data) require(dsts[z].stride_bytes == first.stride_bytes, "volume outputs must share a stride");
+    enqueue_dft_vol_jobs({plan}, {make_dft_job(*plan, sources[0], first)}, {q}, device_flags, (hipStream_t)stream);
     return NF_OK;
   });
 }

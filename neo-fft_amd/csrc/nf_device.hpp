@@ -79,6 +79,33 @@ struct DftLaunch {
   unsigned* flags;
 };
 
+// Temporal (tbsize>1) DFTTest inputs of one job: the logical slots of every
+// covering temporal block map to a table of distinct source frames.
+struct DftTemporal {
+  int T;          // tbsize
+  int ntb;        // covering temporal blocks (tmode=1), else 1
+  int emit_all;   // one volume -> every slice z written to dsts[z]
+  int tmode1;     // synthesis (U*V)*h instead of U*(h*V)
+  int nout;       // outputs written per pass (T if emit_all, else 1)
+  int chunk;      // blocks per LDS chunk along a block row
+  int targets[15];
+  unsigned char slot_frame[225];  // [tb*T + z] -> index into frames
+  const void* frames[30];
+  long long frame_stride[30];
+  void* dsts[15];
+  const float* ttw;  // direct-DFT twiddles of length T: (cos, sin) of 2*pi*e/T
+};
+
+constexpr int kMaxVolJobs = 3;
+
+struct DftVolLaunch {
+  DftJob jobs[kMaxVolJobs];
+  DftTemporal tq[kMaxVolJobs];
+  int tile_start[kMaxVolJobs + 1];
+  int njobs;
+  unsigned* flags;
+};
+
 // One FFT3D plane of one output frame: block stage writes windowed
 // contributions, the synthesis stage sums them in the canonical order.
 struct Fft3dJob {
@@ -127,3 +154,8 @@ struct Fft3dLaunch {
 };
 
 }  // namespace nf
+
+// Kernel arguments travel in the 4 KiB kernarg segment.
+static_assert(sizeof(nf::DftLaunch) <= 4000, "DftLaunch exceeds the kernarg budget");
+static_assert(sizeof(nf::DftVolLaunch) <= 4000, "DftVolLaunch exceeds the kernarg budget");
+static_assert(sizeof(nf::Fft3dLaunch) <= 4000, "Fft3dLaunch exceeds the kernarg budget");

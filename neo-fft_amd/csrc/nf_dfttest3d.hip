@@ -1,0 +1,361 @@
+// nf_dfttest3d.hip -- DFTTest temporal path (tbsize>1, tmode 0 and 1) on gfx950.
+//
+// A workgroup owns an output tile and walks the block rows touching it. Each
+// block row is processed in chunks of consecutive blocks so LDS holds only a
+// chunk's 3D spectra. Per chunk: the T temporal slots' decoded, reflected
+// source rows are staged in LDS; every block goes through the 3D analysis
+// window, spatial row and column transforms per slice, the temporal transform
+// per bin (depth 3/4/5 butterflies or a direct DFT, reference
+// src/spectral/fft_backend.cpp:397-420, 618-656), the gain with 3D mean
+// removal (scalar/spectral.cpp:5-53), the inverse of the target slice only
+// (fft_backend.cpp:526-586) and the synthesis window, and is added to the
+// accumulators in the canonical order: temporal block, block row, block
+// column (plan.cpp:545-605, docs/specs/phase-5/dfttest/kernel-temporal-ola.md).
+//
+// emit_all: for tmode=1 with tosize=0 every frame of a lattice block
+// [s, s+T) has that block as its only covering block, so one pass writes all
+// T output frames from one spectrum. Each slice uses the same operations as a
+// per-frame pass (full inverse butterfly, centre formula for the centre slice
+// of depth 3/5), so results are bit-identical to T separate passes.
+#include "nf_device.hpp"
+#include "nf_fft.hpp"
+#include "nf_gain.hpp"
+#include "nf_launch.hpp"
+
+namespace nf {
+
+namespace {
+
+__device__ __forceinline__ void store_dft_output(const DftJob& J, void* dst, int oyy, int oxx, float a) {
+  char* drow = static_cast<char*>(dst) + (long long)oyy * J.dst_stride;
+  if (J.storage == kF32) {
+    reinterpret_cast<float*>(drow)[oxx] = a * J.oscale;
+  } else {
+    const float v = (a * J.oscale) + 0.5f;
+    const float cl = v < 0.0f ? 0.0f : (J.peak < v ? J.peak : v);
+    if (J.storage == kU8) reinterpret_cast<uint8_t*>(drow)[oxx] = (uint8_t)cl;
+    else reinterpret_cast<uint16_t*>(drow)[oxx] = (uint16_t)cl;
+  }
+}
+
+// Inverse temporal transform of one bin for slice s (fft_backend.cpp:542-548).
+__device__ __forceinline__ Cpx inverse_slice(int T, const Cpx* v, int s, const float* ttw) {
+  if ((T == 3 || T == 5) && s == T / 2) return temporal_inverse_center(T, v);
+  if (T >= 3 && T <= 5) {
+    Cpx w[5];
+    for (int z = 0; z < T; ++z) w[z] = v[z];
+    temporal_butterfly(T, w, false);
+    return w[s];
+  }
+  return dft_term(T, v, s, ttw, false);
+}
+
+}  // namespace
+
+template <int B>
+__global__ __launch_bounds__(256) void dfttest_vol_kernel(const DftVolLaunch L) {
+  extern __shared__ float smem[];
+  constexpr int K = B / 2 + 1;
+  constexpr int BINS2 = B * K;
+  int jid = 0;
+  while (jid + 1 < L.njobs && (int)blockIdx.x >= L.tile_start[jid + 1]) ++jid;
+  const DftJob& J = L.jobs[jid];
+  const DftTemporal& Q = L.tq[jid];
+  const int tile = (int)blockIdx.x - L.tile_start[jid];
+  const int tx = tile % J.tiles_x, ty = tile / J.tiles_x;
+  const int S = J.S, T = Q.T, NBc = Q.chunk, nout = Q.nout;
+  const int tid = threadIdx.x, nthreads = blockDim.x;
+  const bool ring_mode = Q.ntb == 1;
+
+  const int vx0 = J.gx.offset, vx1 = J.gx.offset + J.gx.length;
+  const int vy0 = J.gy.offset, vy1 = J.gy.offset + J.gy.length;
+  const int X0 = max(vx0, (J.tile_x0 + tx) * J.tile_w), X1 = min(vx1, (J.tile_x0 + tx + 1) * J.tile_w);
+  const int Y0 = max(vy0, (J.tile_y0 + ty) * J.tile_h), Y1 = min(vy1, (J.tile_y0 + ty + 1) * J.tile_h);
+  const int bx_lo = ceil_div_int(X0 - B + 1, S), bx_hi = (X1 - 1) / S;
+  const int by_lo = ceil_div_int(Y0 - B + 1, S), by_hi = (Y1 - 1) / S;
+  const int TW = X1 - X0, TH = Y1 - Y0;
+  const int ww_c = (NBc - 1) * S + B;
+  const int acc_rows = ring_mode ? B : J.tile_h;
+
+  float* spec = smem;                                 // [NBc][T][BINS2][2]
+  float* out = spec + NBc * T * BINS2 * 2;            // [NBc][B][B]
+  float* win = out + NBc * B * B;                     // [T][B][ww_c]
+  float* acc = win + T * B * ww_c;                    // [nout][acc_rows][tile_w]
+  float* dc = acc + nout * acc_rows * J.tile_w;       // [NBc][2]
+  float* hw = dc + 2 * NBc;                           // [T][B][B]
+  float* hs = hw + T * B * B;                         // [nout][B][B]
+
+  bool bad = false;
+  for (int i = tid; i < T * B * B; i += nthreads) hw[i] = J.h[i];
+  for (int i = tid; i < nout * acc_rows * J.tile_w; i += nthreads) acc[i] = 0.0f;
+  const float fct = 1.0f / ((float)T * (float)B * (float)B);
+  const float volume = (float)T * (float)B * (float)B;
+  for (int tb = 0; tb < Q.ntb; ++tb) {
+    const int target = Q.tmode1 ? Q.targets[tb] : T / 2;
+    __syncthreads();
+    // synthesis windows of the slices written by this pass
+    for (int i = tid; i < nout * B * B; i += nthreads) {
+      const int o = i / (B * B), e = i - o * (B * B);
+      const int z = Q.emit_all ? o : target;
+      hs[i] = Q.tmode1 ? J.h[z * B * B + e] : J.hsyn[z * B * B + e];
+    }
+    for (int by = by_lo; by <= by_hi; ++by) {
+      const int oy = by * S;
+      for (int cb = bx_lo; cb <= bx_hi; cb += NBc) {
+        const int nb = min(NBc, bx_hi - cb + 1);
+        const int cx0 = cb * S;
+        const int WWc = (nb - 1) * S + B;
+        __syncthreads();
+        // A: all T slots' source rows of this chunk
+        for (int idx = tid; idx < T * B * WWc; idx += nthreads) {
+          const int z = idx / (B * WWc);
+          const int rem = idx - z * (B * WWc);
+          const int r = rem / WWc, c = rem - r * WWc;
+          const int f = Q.slot_frame[tb * T + z];
+          const int sy = reflect(oy + r, J.gy), sx = reflect(cx0 + c, J.gx);
+          const float v = load_sample(Q.frames[f], Q.frame_stride[f], J.storage, sy, sx) * J.dscale;
+          bad |= !isfinite(v);
+          win[(z * B + r) * ww_c + c] = v;
+        }
+        __syncthreads();
+        // B: windowed rows of every slice
+        for (int t = tid; t < T * nb * B; t += nthreads) {
+          const int z = t / (nb * B);
+          const int rem = t - z * (nb * B);
+          const int bi = rem / B, y = rem - bi * B;
+          float re[B], im[B];
+          const float* src = win + (z * B + y) * ww_c + bi * S;
+          const float* hrow = hw + (z * B + y) * B;
+#pragma unroll
+          for (int x = 0; x < B; ++x) {
+            re[x] = src[x] * hrow[x];
+            im[x] = 0.0f;
+          }
+          cfft<B, true>(re, im);
+          float* dst = spec + (((bi * T + z) * BINS2) + y * K) * 2;
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            dst[2 * k] = re[k];
+            dst[2 * k + 1] = (k == B / 2) ? 0.0f : im[k];
+          }
+        }
+        __syncthreads();
+        // C: column transforms of every slice
+        for (int t = tid; t < T * nb * K; t += nthreads) {
+          const int z = t / (nb * K);
+          const int rem = t - z * (nb * K);
+          const int bi = rem / K, k = rem - bi * K;
+          float re[B], im[B];
+          float* col = spec + (((bi * T + z) * BINS2) + k) * 2;
+#pragma unroll
+          for (int y = 0; y < B; ++y) {
+            re[y] = col[y * K * 2];
+            im[y] = col[y * K * 2 + 1];
+          }
+          cfft<B, true>(re, im);
+#pragma unroll
+          for (int y = 0; y < B; ++y) {
+            col[y * K * 2] = re[y];
+            col[y * K * 2 + 1] = im[y];
+          }
+        }
+        __syncthreads();
+        // D: temporal transform per bin
+        for (int t = tid; t < nb * BINS2; t += nthreads) {
+          const int bi = t / BINS2, bin = t - bi * BINS2;
+          Cpx v[15];
+          float* base = spec + ((bi * T) * BINS2 + bin) * 2;
+          for (int z = 0; z < T; ++z) v[z] = Cpx{base[z * BINS2 * 2], base[z * BINS2 * 2 + 1]};
+          if (T >= 3 && T <= 5) {
+            temporal_butterfly(T, v, true);
+          } else {
+            Cpx w[15];
+            for (int m = 0; m < T; ++m) w[m] = dft_term(T, v, m, Q.ttw, true);
+            for (int m = 0; m < T; ++m) v[m] = w[m];
+          }
+          for (int z = 0; z < T; ++z) {
+            base[z * BINS2 * 2] = v[z].r;
+            base[z * BINS2 * 2 + 1] = v[z].i;
+          }
+          if (bin == 0) {
+            dc[2 * bi] = v[0].r;
+            dc[2 * bi + 1] = v[0].i;
+          }
+        }
+        __syncthreads();
+        // E: gain over the 3D spectrum, inverse temporal transform
+        for (int t = tid; t < nb * BINS2; t += nthreads) {
+          const int bi = t / BINS2, bin = t - bi * BINS2;
+          const bool use_grid = J.grid && J.mean_scale != 0.0f && J.grid[0].x != 0.0f;
+          const float gs = use_grid ? (J.mean_scale * dc[2 * bi]) / J.grid[0].x : 0.0f;
+          Cpx v[15];
+          float* base = spec + ((bi * T) * BINS2 + bin) * 2;
+          for (int z = 0; z < T; ++z) {
+            float re = base[z * BINS2 * 2], im = base[z * BINS2 * 2 + 1];
+            const int k3 = z * BINS2 + bin;
+            float mr = 0.0f, mi = 0.0f;
+            if (J.grid) {
+              const float2 gv = J.grid[k3];
+              mr = gs * gv.x;
+              mi = gs * gv.y;
+            }
+            apply_gain(J.g, k3, re, im, mr, mi, bad);
+            v[z] = Cpx{re, im};
+          }
+          if (Q.emit_all) {
+            Cpx w[15];
+            for (int z = 0; z < T; ++z) w[z] = inverse_slice(T, v, z, Q.ttw);
+            for (int z = 0; z < T; ++z) {
+              base[z * BINS2 * 2] = w[z].r;
+              base[z * BINS2 * 2 + 1] = w[z].i;
+            }
+          } else {
+            const Cpx r = inverse_slice(T, v, target, Q.ttw);
+            base[0] = r.r;
+            base[1] = r.i;
+          }
+        }
+        for (int o = 0; o < nout; ++o) {
+          if (Q.emit_all && Q.dsts[o] == nullptr) continue;
+          const int zs = Q.emit_all ? o : 0;
+          __syncthreads();
+          // F: inverse column transforms of the slice
+          for (int t = tid; t < nb * K; t += nthreads) {
+            const int bi = t / K, k = t - bi * K;
+            float re[B], im[B];
+            float* col = spec + (((bi * T + zs) * BINS2) + k) * 2;
+#pragma unroll
+            for (int y = 0; y < B; ++y) {
+              re[y] = col[y * K * 2];
+              im[y] = col[y * K * 2 + 1];
+            }
+            cfft<B, false>(re, im);
+#pragma unroll
+            for (int y = 0; y < B; ++y) {
+              col[y * K * 2] = re[y];
+              col[y * K * 2 + 1] = im[y];
+            }
+          }
+          __syncthreads();
+          // G: inverse rows, normalisation, synthesis window
+          for (int t = tid; t < nb * B; t += nthreads) {
+            const int bi = t / B, y = t - bi * B;
+            float re[B], im[B];
+            const float* row = spec + (((bi * T + zs) * BINS2) + y * K) * 2;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+              re[k] = row[2 * k];
+              im[k] = row[2 * k + 1];
+            }
+#pragma unroll
+            for (int k = K; k < B; ++k) {
+              re[k] = re[B - k];
+              im[k] = 0.0f - im[B - k];
+            }
+            cfft<B, false>(re, im);
+            float* ob = out + (bi * B + y) * B;
+            const float* hsrow = hs + (o * B + y) * B;
+#pragma unroll
+            for (int x = 0; x < B; ++x) {
+              float v = re[x] * fct;
+              bad |= !isfinite(v);
+              if (Q.tmode1) v = volume * v;
+              ob[x] = v * hsrow[x];
+            }
+          }
+          __syncthreads();
+          // H: overlap-add of this chunk, block columns ascending
+          {
+            const int r0 = max(oy, Y0), r1 = min(oy + B, Y1);
+            const int xa = max(X0, cx0), xb = min(X1, cx0 + WWc);
+            const int nr = r1 - r0, ncol = xb - xa;
+            float* A = acc + o * acc_rows * J.tile_w;
+            for (int idx = tid; idx < nr * ncol; idx += nthreads) {
+              const int rr = idx / ncol;
+              const int x = xa + (idx - rr * ncol);
+              const int y = r0 + rr;
+              float* cell = A + (ring_mode ? (y % B) : (y - Y0)) * J.tile_w + (x - X0);
+              float sum = *cell;
+              const int b_first = max(cb, ceil_div_int(x - B + 1, S));
+              const int b_last = min(cb + nb - 1, x / S);
+              for (int bx = b_first; bx <= b_last; ++bx)
+                sum = sum + out[((bx - cb) * B + (y - oy)) * B + (x - bx * S)];
+              *cell = sum;
+            }
+          }
+        }
+      }
+      if (ring_mode) {
+        __syncthreads();
+        // rows [oy, oy+S) are final
+        const int r0 = max(oy, Y0), r1 = min(oy + S, Y1);
+        const int nr = r1 - r0;
+        for (int o = 0; o < nout; ++o) {
+          void* dst = Q.emit_all ? Q.dsts[o] : J.dst;
+          float* A = acc + o * acc_rows * J.tile_w;
+          for (int idx = tid; idx < nr * TW; idx += nthreads) {
+            const int rr = idx / TW;
+            const int x = X0 + (idx - rr * TW);
+            const int y = r0 + rr;
+            float* cell = A + (y % B) * J.tile_w + (x - X0);
+            const float a = *cell;
+            *cell = 0.0f;
+            if (dst == nullptr) continue;
+            bad |= !isfinite(a);
+            if (J.acc_out) J.acc_out[(long long)(y - vy0) * J.acc_stride + (x - vx0)] = a;
+            else store_dft_output(J, dst, y - vy0, x - vx0, a);
+          }
+        }
+      }
+    }
+  }
+  if (!ring_mode) {
+    __syncthreads();
+    for (int idx = tid; idx < TH * TW; idx += nthreads) {
+      const int rr = idx / TW;
+      const int x = X0 + (idx - rr * TW);
+      const int y = Y0 + rr;
+      const float a = acc[rr * J.tile_w + (x - X0)];
+      bad |= !isfinite(a);
+      if (J.acc_out) J.acc_out[(long long)(y - vy0) * J.acc_stride + (x - vx0)] = a;
+      else store_dft_output(J, J.dst, y - vy0, x - vx0, a);
+    }
+  }
+  if (bad) atomicOr(L.flags, NF_FLAG_NONFINITE);
+}
+
+size_t dfttest_vol_lds_bytes(int B, int T, int chunk, int S, int nout, int acc_rows, int tile_w) {
+  const int K = B / 2 + 1;
+  const size_t ww_c = (size_t)(chunk - 1) * S + B;
+  return sizeof(float) * ((size_t)chunk * T * B * K * 2 + (size_t)chunk * B * B + (size_t)T * B * ww_c +
+                          (size_t)nout * acc_rows * tile_w + 2 * (size_t)chunk + (size_t)T * B * B +
+                          (size_t)nout * B * B);
+}
+
+hipError_t launch_dfttest_vol(int B, const DftVolLaunch& L, int total_tiles, size_t lds, hipStream_t stream) {
+  dim3 grid(total_tiles), block(256);
+  switch (B) {
+    case 8:
+      hipLaunchKernelGGL(dfttest_vol_kernel<8>, grid, block, lds, stream, L);
+      break;
+    case 16:
+      hipLaunchKernelGGL(dfttest_vol_kernel<16>, grid, block, lds, stream, L);
+      break;
+    case 32:
+      hipLaunchKernelGGL(dfttest_vol_kernel<32>, grid, block, lds, stream, L);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t dfttest_vol_set_lds_limit() {
+  hipError_t e = hipFuncSetAttribute((const void*)dfttest_vol_kernel<8>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)dfttest_vol_kernel<16>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void*)dfttest_vol_kernel<32>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+
+}  // namespace nf

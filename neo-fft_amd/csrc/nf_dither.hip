@@ -1,0 +1,100 @@
+// nf_dither.hip -- DFTTest 8-bit Floyd-Steinberg output on gfx950.
+//
+// Reference: src/kernels/scalar/dither.cpp:11-41 (row-major scan, error
+// E - D with the clipped D, weights 7/16 right, 3/16 below-left, 5/16 below,
+// 1/16 below-right) and the coordinate hash noise of rows.cpp:6-8. The scan is
+// a skewed wavefront: one workgroup of up to 1024 lanes handles a pass of
+// rows, lane r owns row r and processes pixel x = t - 2r at step t. The
+// value consumed at (y, x) is rebuilt in the reference's binary32 order:
+//   next(y-1, x) = ((0 + e(y-1,x-1)*1/16) + e(y-1,x)*5/16) + e(y-1,x+1)*3/16
+//   cur(y, x)    = next(y-1, x) + e(y, x-1)*7/16
+// where the three previous-row errors reach lane r from lane r-1 through LDS
+// (one barrier per step), and absent neighbours drop their term.
+#include "nf_device.hpp"
+#include "nf_launch.hpp"
+
+namespace nf {
+
+namespace {
+__device__ __forceinline__ uint32_t mix32(uint32_t v) {
+  v ^= v >> 16;
+  v *= 0x7feb352du;
+  v ^= v >> 15;
+  v *= 0x846ca68bu;
+  return v ^ (v >> 16);
+}
+__device__ __forceinline__ uint32_t dither_hash(uint32_t seed, uint32_t n, uint32_t p, uint32_t y, uint32_t x) {
+  return mix32(mix32(mix32(mix32(mix32(seed ^ 0xa511e9b3u) ^ n) ^ p) ^ y) ^ x);
+}
+}  // namespace
+
+__global__ __launch_bounds__(1024) void dither_kernel(const float* acc, long long acc_stride, int w, int h,
+                                                      uint8_t* dst, long long dst_stride, int mode, int seed, int frame,
+                                                      int plane) {
+  extern __shared__ float lds[];
+  float* xfer = lds;               // [2][1024] error produced this step, per lane
+  float* last_row = lds + 2048;    // [w] errors of the previous pass's last row
+  const int r = threadIdx.x, nl = blockDim.x;
+  const float scale = (float)(mode - 1) + .5f, off = scale * .5f;
+  for (int y0 = 0; y0 < h; y0 += nl) {
+    const int y = y0 + r;
+    const int rows = min(nl, h - y0);
+    const bool active_row = r < rows;
+    // e(y-1, x-1), e(y-1, x), e(y-1, x+1) for the pixel about to be processed
+    float pm = 0.0f, p0 = 0.0f, pp = 0.0f, own = 0.0f;
+    const int steps = w + 2 * (rows - 1) + 2;
+    for (int t = 0; t < steps; ++t) {
+      const int x = t - 2 * r;
+      float err = 0.0f;
+      if (active_row && x >= 0 && x < w) {
+        const float e = acc[(long long)y * acc_stride + x];
+        float nxt = 0.0f;
+        if (y > 0) {
+          if (x > 0) nxt = nxt + pm * .0625f;
+          nxt = nxt + p0 * .3125f;
+          if (x + 1 < w) nxt = nxt + pp * .1875f;
+        }
+        float cur = nxt;
+        if (x > 0) cur = cur + own * .4375f;
+        float v;
+        if (mode == 1) {
+          v = (e + cur) + .5f;
+        } else {
+          const float u = (float)(dither_hash((uint32_t)seed, (uint32_t)frame, (uint32_t)plane, (uint32_t)y,
+                                              (uint32_t)x) >> 8) * 0x1p-24f;
+          v = (((e + u * scale) - off) + cur) + .5f;
+        }
+        const uint8_t d = (uint8_t)(v <= 0.0f ? 0.0f : (v >= 255.0f ? 255.0f : v));
+        dst[(long long)y * dst_stride + x] = d;
+        err = e - (float)d;
+        own = err;
+        if (r == rows - 1) last_row[x] = err;
+      }
+      xfer[(t & 1) * 1024 + r] = err;
+      __syncthreads();
+      // the previous row's error at x+2 (lane r-1 processed x+2 at step t)
+      float incoming;
+      if (r > 0) incoming = xfer[(t & 1) * 1024 + r - 1];
+      else incoming = (y0 > 0 && x + 2 >= 0 && x + 2 < w) ? last_row[x + 2] : 0.0f;
+      pm = p0;
+      p0 = pp;
+      pp = incoming;
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_dither(const float* acc, long long acc_stride, int w, int h, uint8_t* dst, long long dst_stride,
+                         int mode, int seed, int frame, int plane, hipStream_t stream) {
+  const int lanes = h < 1024 ? ((h + 63) / 64) * 64 : 1024;
+  const size_t lds = sizeof(float) * (2048 + (size_t)w);
+  hipLaunchKernelGGL(dither_kernel, dim3(1), dim3(lanes), lds, stream, acc, acc_stride, w, h, dst, dst_stride, mode,
+                     seed, frame, plane);
+  return hipGetLastError();
+}
+
+hipError_t dither_set_lds_limit() {
+  return hipFuncSetAttribute((const void*)dither_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+
+}  // namespace nf

@@ -210,6 +210,84 @@ NF_HD void cfft(float* r, float* i) {
   else fft32<FWD>(r, i);
 }
 
+// Temporal-axis transforms of DFTTest's 3D spectrum, same operation order as
+// the reference's dense small-volume butterflies (src/spectral/
+// fft_backend.cpp:397-420) for depths 3, 4, 5 and its centre-slice inverse
+// (:526-540). cscale multiplies both parts by a float constant, crot
+// multiplies by -i (forward) or +i (inverse).
+struct Cpx {
+  float r, i;
+};
+NF_HD Cpx cadd(Cpx a, Cpx b) { return {a.r + b.r, a.i + b.i}; }
+NF_HD Cpx csub(Cpx a, Cpx b) { return {a.r - b.r, a.i - b.i}; }
+NF_HD Cpx cscale(Cpx a, float s) { return {a.r * s, a.i * s}; }
+NF_HD Cpx crot(Cpx z, bool fwd) { return fwd ? Cpx{z.i, -z.r} : Cpx{-z.i, z.r}; }
+
+struct TemporalK {
+  static constexpr float k3 = 0.86602540378443864676f;
+  static constexpr float k5a = 0.30901699437494742410f, k5b = 0.80901699437494742410f;
+  static constexpr float k5c = 0.95105651629515357212f, k5d = 0.58778525229247312917f;
+};
+
+NF_HD void temporal_butterfly(int n, Cpx* v, bool fwd) {
+  if (n == 3) {
+    const Cpx sum = cadd(v[1], v[2]), diff = csub(v[1], v[2]);
+    const Cpx center = csub(v[0], cscale(sum, 0.5f));
+    const Cpx side = crot(cscale(diff, TemporalK::k3), fwd);
+    v[0] = cadd(v[0], sum);
+    v[1] = cadd(center, side);
+    v[2] = csub(center, side);
+  } else if (n == 4) {
+    const Cpx es = cadd(v[0], v[2]), ed = csub(v[0], v[2]);
+    const Cpx os = cadd(v[1], v[3]), od = crot(csub(v[1], v[3]), fwd);
+    v[0] = cadd(es, os);
+    v[2] = csub(es, os);
+    v[1] = cadd(ed, od);
+    v[3] = csub(ed, od);
+  } else {
+    const Cpx a = cadd(v[1], v[4]), b = cadd(v[2], v[3]), c = csub(v[1], v[4]), d = csub(v[2], v[3]);
+    const Cpx base1 = csub(cadd(v[0], cscale(a, TemporalK::k5a)), cscale(b, TemporalK::k5b));
+    const Cpx base2 = cadd(csub(v[0], cscale(a, TemporalK::k5b)), cscale(b, TemporalK::k5a));
+    const Cpx side1 = crot(cadd(cscale(c, TemporalK::k5c), cscale(d, TemporalK::k5d)), fwd);
+    const Cpx side2 = crot(csub(cscale(c, TemporalK::k5d), cscale(d, TemporalK::k5c)), fwd);
+    v[0] = cadd(cadd(v[0], a), b);
+    v[1] = cadd(base1, side1);
+    v[4] = csub(base1, side1);
+    v[2] = cadd(base2, side2);
+    v[3] = csub(base2, side2);
+  }
+}
+
+// Centre slice of the depth-3/5 inverse with the reference's operation order.
+NF_HD Cpx temporal_inverse_center(int n, const Cpx* v) {
+  if (n == 3) {
+    const Cpx sum = cadd(v[1], v[2]), diff = csub(v[1], v[2]);
+    const Cpx center = csub(v[0], cscale(sum, 0.5f));
+    const Cpx side = cscale(diff, TemporalK::k3);
+    return cadd(center, Cpx{-side.i, side.r});
+  }
+  const Cpx a = cadd(v[1], v[4]), b = cadd(v[2], v[3]), c = csub(v[1], v[4]), d = csub(v[2], v[3]);
+  const Cpx base = cadd(csub(v[0], cscale(a, TemporalK::k5b)), cscale(b, TemporalK::k5a));
+  const Cpx side = csub(cscale(c, TemporalK::k5d), cscale(d, TemporalK::k5c));
+  return cadd(base, Cpx{-side.i, side.r});
+}
+
+// Direct DFT term k of length n: sum_j x[j] * (c_e, +-s_e), e = j*k mod n,
+// with tw[2e] = float(cos(2 pi e / n)), tw[2e+1] = float(sin(2 pi e / n)).
+NF_HD Cpx dft_term(int n, const Cpx* x, int k, const float* tw, bool fwd) {
+  float sr = 0.0f, si = 0.0f;
+  for (int j = 0; j < n; ++j) {
+    const int e = (j * k) % n;
+    const float c = tw[2 * e];
+    const float s = fwd ? -tw[2 * e + 1] : tw[2 * e + 1];
+    const float pr = x[j].r * c - x[j].i * s;
+    const float pi = x[j].r * s + x[j].i * c;
+    sr = sr + pr;
+    si = si + pi;
+  }
+  return {sr, si};
+}
+
 // Host-side dispatch by runtime length (plan creation only).
 inline bool cfft_supported(int n) { return n == 8 || n == 16 || n == 32; }
 inline void cfft_host(int n, float* r, float* i, bool fwd) {

@@ -7,6 +7,14 @@ size_t dfttest_t1_lds_bytes(int B, const DftJob& j);
 hipError_t launch_dfttest_t1(int B, const DftLaunch& L, int total_tiles, size_t lds, hipStream_t stream);
 hipError_t dfttest_t1_set_lds_limit();
 
+size_t dfttest_vol_lds_bytes(int B, int T, int chunk, int S, int nout, int acc_rows, int tile_w);
+hipError_t launch_dfttest_vol(int B, const DftVolLaunch& L, int total_tiles, size_t lds, hipStream_t stream);
+hipError_t dfttest_vol_set_lds_limit();
+
+hipError_t launch_dither(const float* acc, long long acc_stride, int w, int h, uint8_t* dst, long long dst_stride,
+                         int mode, int seed, int frame, int plane, hipStream_t stream);
+hipError_t dither_set_lds_limit();
+
 size_t fft3d_block_lds_bytes(int BW, int BH, int T, int NB, int Sx);
 hipError_t launch_fft3d_block(int BW, int BH, const Fft3dLaunch& L, int total_groups, size_t lds, hipStream_t stream);
 hipError_t launch_fft3d_synthesis(const Fft3dLaunch& L, hipStream_t stream);

@@ -20,6 +20,9 @@ struct Unsupported : std::runtime_error {
 inline void require(bool ok, const std::string& message) {
   if (!ok) throw InvalidArgument(message);
 }
+// Thread-local message returned by nf_last_error().
+void set_last_error(const std::string& message);
+
 // reference checked.hpp:37-41 (creation-time use)
 float finite(float v);
 
@@ -56,5 +59,6 @@ std::vector<float> dft_profile(const Curves& c, int T, int S, float sigma, float
 // Real transforms of one block (same butterflies as the kernels).
 std::vector<std::complex<float>> rfft2_host(int h, int w, const float* in);
 std::vector<std::complex<float>> rfft3_host(int t, int h, int w, const float* in);
+std::vector<float> temporal_twiddles(int n);
 
 }  // namespace nf

@@ -74,6 +74,22 @@ class MutPlane(ctypes.Structure):
     _fields_ = Plane._fields_
 
 
+class VideoInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("color_family", "bits", "floating", "subsampling_w", "subsampling_h",
+                                            "plane_count", "width", "height", "num_frames")]
+
+
+class Frame(ctypes.Structure):
+    _fields_ = [("planes", Plane * 4)]
+
+
+class MutFrame(ctypes.Structure):
+    _fields_ = [("planes", MutPlane * 4)]
+
+
+FrameSource = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Frame))
+COLOR_FAMILIES = {"gray": 0, "yuv": 1, "rgb": 2}
+
 # Every exported symbol of include/neo_fft.h with its ctypes signature.
 _SIGNATURES = {
     "nf_last_error": (ctypes.c_char_p, []),
@@ -93,6 +109,9 @@ _SIGNATURES = {
     "nf_process_batch": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(Plane), c_int_p,
                                         c_int_p, ctypes.POINTER(MutPlane), c_int_p, c_int_p, ctypes.c_void_p,
                                         ctypes.c_void_p]),
+    "nf_dfttest_enqueue_volume": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Plane), ctypes.c_int,
+                                                 ctypes.POINTER(MutPlane), ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_void_p, ctypes.c_void_p]),
     "nf_kalman_state_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     "nf_kalman_state_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "nf_kalman_state_destroy": (None, [ctypes.c_void_p]),
@@ -101,6 +120,15 @@ _SIGNATURES = {
     "nf_kalman_render": (ctypes.c_int, [ctypes.c_void_p, Plane, MutPlane, ctypes.c_void_p, ctypes.c_void_p]),
     "nf_kalman_state_download": (ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p, c_float_p]),
     "nf_kernel_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
+    "nf_filter_create_dfttest": (ctypes.c_int, [ctypes.POINTER(VideoInfo), ctypes.POINTER(DFTTestConfig), c_int_p,
+                                                ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "nf_filter_create_fft3d": (ctypes.c_int, [ctypes.POINTER(VideoInfo), ctypes.POINTER(FFT3DConfig), c_int_p,
+                                              ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "nf_filter_destroy": (None, [ctypes.c_void_p]),
+    "nf_filter_request": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_int_p, ctypes.c_int]),
+    "nf_filter_get_frame": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, FrameSource, ctypes.c_void_p,
+                                           ctypes.POINTER(MutFrame), ctypes.c_void_p]),
+    "nf_filter_checkpoints": (ctypes.c_int, [ctypes.c_void_p, c_int_p, ctypes.c_int]),
     "nf_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "nf_profile_read": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong),
                                        ctypes.c_int]),
@@ -264,6 +292,15 @@ class Plan:
                                      ctypes.c_void_p(flags.data_ptr()), _stream_handle(stream)))
 
 
+def dfttest_volume(plan, sources, dsts, flags, first_frame=0, plane=0, stream=None):
+    """tmode=1/tosize=0: one pass over a lattice block's sources writes every
+    slice z to dsts[z] (None skips it)."""
+    srcs = (Plane * len(sources))(*[_plane(s) for s in sources])
+    outs = (MutPlane * len(dsts))(*[MutPlane(None, 0, 0, 0) if d is None else _plane(d, MutPlane) for d in dsts])
+    _check(lib().nf_dfttest_enqueue_volume(plan.handle, srcs, len(sources), outs, first_frame, plane,
+                                           ctypes.c_void_p(flags.data_ptr()), _stream_handle(stream)))
+
+
 def _fmt(bits, floating, chroma):
     return SampleFormat(int(bits), int(bool(floating)), int(bool(chroma)))
 
@@ -361,3 +398,110 @@ def kernel_info():
     buf = ctypes.create_string_buffer(512)
     _check(lib().nf_kernel_info(buf, 512))
     return dict(kv.split("=", 1) for kv in buf.value.decode().split(";") if "=" in kv)
+
+
+class Clip:
+    """A planar clip whose frames are lists of 2D device tensors (one per
+    plane). frames: list of frames, or a callable n -> frame."""
+
+    def __init__(self, frames, num_frames=None, color_family="yuv", bits=8, floating=False, subsampling=(1, 1),
+                 width=None, height=None, plane_count=None):
+        self._frames = frames
+        self.num_frames = num_frames if num_frames is not None else len(frames)
+        first = self.get_frame(0)
+        self.color_family = color_family
+        self.bits = 32 if floating else bits
+        self.floating = floating
+        self.subsampling = subsampling if color_family == "yuv" else (0, 0)
+        self.plane_count = plane_count or len(first)
+        self.height, self.width = (height, width) if width else tuple(first[0].shape)
+
+    def get_frame(self, n):
+        if callable(self._frames):
+            return self._frames(n)
+        return self._frames[n]
+
+    def info(self):
+        return VideoInfo(COLOR_FAMILIES[self.color_family], self.bits, int(self.floating), self.subsampling[0],
+                         self.subsampling[1], self.plane_count, self.width, self.height, self.num_frames)
+
+
+class FilteredClip:
+    """Output of DFTTest / FFT3D: frames are produced on request, like a
+    VapourSynth node (reference src/plugin/filter.hpp request/process)."""
+
+    def __init__(self, source, handle):
+        self.source = source
+        self._h = handle
+        self.num_frames = source.num_frames
+        self._keep = []
+
+        def fetch(user, n, out):
+            try:
+                fr = self.source.get_frame(n)
+                self._keep.append(fr)
+                for p, t in enumerate(fr):
+                    out[0].planes[p] = _plane(t)
+                return NF_OK
+            except Exception:  # reported to the filter as a failed source request
+                return NF_EINVAL
+
+        self._cb = FrameSource(fetch)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().nf_filter_destroy(h)
+            self._h = None
+
+    def requests(self, n):
+        buf = (ctypes.c_int * 256)()
+        k = lib().nf_filter_request(self._h, n, buf, 256)
+        if k < 0:
+            raise InvalidArgument(NF_EINVAL, "frame index outside clip")
+        return list(buf[:k])
+
+    def checkpoints(self):
+        buf = (ctypes.c_int * 8)()
+        k = lib().nf_filter_checkpoints(self._h, buf, 8)
+        return list(buf[:k])
+
+    def get_frame(self, n, out=None, stream=None):
+        import torch
+        src0 = self.source.get_frame(min(n, self.num_frames - 1))
+        if out is None:
+            out = [torch.empty_like(t) for t in src0]
+        mf = MutFrame()
+        for p, t in enumerate(out):
+            mf.planes[p] = _plane(t, MutPlane)
+        self._keep = []
+        try:
+            _check(lib().nf_filter_get_frame(self._h, n, self._cb, None, ctypes.byref(mf), _stream_handle(stream)))
+        finally:
+            self._keep = []
+        return out
+
+
+def _planes_arg(planes):
+    if planes is None:
+        return None, 0
+    arr = (ctypes.c_int * max(1, len(planes)))(*planes)
+    return arr, len(planes)
+
+
+def DFTTest(clip, planes=None, **params):
+    """core.neo_fft.DFTTest equivalent (reference docs/api/en/dfttest.md)."""
+    cfg = dfttest_config(**params)
+    pl, npl = _planes_arg(planes)
+    h = ctypes.c_void_p()
+    _check(lib().nf_filter_create_dfttest(ctypes.byref(clip.info()), ctypes.byref(cfg), pl, npl, ctypes.byref(h)))
+    return FilteredClip(clip, h)
+
+
+def FFT3D(clip, planes=None, **params):
+    """core.neo_fft.FFT3D equivalent (reference docs/api/en/fft3d.md)."""
+    cfg = fft3d_config(**params)
+    pl, npl = _planes_arg(planes)
+    h = ctypes.c_void_p()
+    _check(lib().nf_filter_create_fft3d(ctypes.byref(clip.info()), ctypes.byref(cfg), pl, npl, ctypes.byref(h)))
+    return FilteredClip(clip, h)

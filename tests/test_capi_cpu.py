@@ -14,6 +14,7 @@ HEADER = os.path.join(ROOT, "include", "neo_fft.h")
 def header_functions():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = "\n".join(line for line in text.splitlines() if not line.lstrip().startswith("typedef"))
     return sorted(set(re.findall(r"\b(nf_[a-z0-9_]+)\s*\(", text)))
 
 

@@ -1,0 +1,156 @@
+"""Filter-level GPU tests: the C++ host layer (nf_filter_*) driving the
+kernels for whole clips -- temporal slot selection and clip-end handling,
+plane selection, FFT3D ROI and the Kalman replay/checkpoint policy -- against
+the oracle driven by an independent Python restatement of the reference's
+request semantics (src/plugin/filter.hpp, src/runtime/checkpoints.hpp)."""
+import numpy as np
+import pytest
+
+from gpu_util import to_device, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+def _yuv_clip(n, w, h, bits, seed):
+    rng = np.random.default_rng(seed)
+    dt = np.uint8 if bits == 8 else np.uint16
+    peak = (1 << bits) - 1
+    frames = []
+    base = [rng.integers(0, peak + 1, size=s) for s in ((h, w), (h // 2, w // 2), (h // 2, w // 2))]
+    for _ in range(n):
+        frames.append([np.clip(b + rng.integers(-peak // 20, peak // 20 + 1, size=b.shape), 0, peak).astype(dt)
+                       for b in base])
+    return frames
+
+
+def _device_clip(nf, frames, bits):
+    dev = [[to_device(p) for p in fr] for fr in frames]
+    return nf.Clip(dev, color_family="yuv", bits=bits, subsampling=(1, 1))
+
+
+def _dft_slots(n, N, T, tmode, tosize, oracle):
+    if tmode == 1:
+        blocks = oracle.temporal_blocks(n, N, T, tosize)
+        return [s for _, _, sl in blocks for s in sl], [t for _, t, _ in blocks]
+    c = T // 2
+    return [min(max(n + j - c, 0), N - 1) for j in range(T)], None
+
+
+@pytest.mark.parametrize("T,tmode,tosize", [(1, 0, 0), (3, 0, 0), (5, 1, 0), (4, 1, 2)])
+def test_dfttest_clip(nf, oracle, T, tmode, tosize):
+    N, w, h, bits = 9, 64, 48, 10
+    frames = _yuv_clip(N, w, h, bits, 5 + T)
+    params = dict(tbsize=T, tmode=tmode, tosize=tosize, sbsize=16, sosize=12, sigma=6.0)
+    out = nf.DFTTest(_device_clip(nf, frames, bits), **params)
+    for n in (0, 1, 4, N - 2, N - 1):
+        got = out.get_frame(n)
+        slots, targets = _dft_slots(n, N, T, tmode, tosize, oracle)
+        assert out.requests(n) == sorted(set(slots))
+        for p in range(3):
+            ref = oracle.dfttest_plane([frames[s][p] for s in slots], bits=bits, targets=targets, **params)
+            assert np.array_equal(to_host(got[p], np.uint16), ref), (n, p)
+
+
+def test_dfttest_plane_selection_copies(nf, oracle):
+    frames = _yuv_clip(3, 64, 48, 8, 2)
+    out = nf.DFTTest(_device_clip(nf, frames, 8), planes=[0], tbsize=1)
+    got = out.get_frame(1)
+    assert np.array_equal(to_host(got[0], np.uint8), oracle.dfttest_plane([frames[1][0]], bits=8, tbsize=1))
+    for p in (1, 2):
+        assert np.array_equal(to_host(got[p], np.uint8), frames[1][p])
+
+
+@pytest.mark.parametrize("bt", [1, 3, 5])
+def test_fft3d_clip_with_end_fallback(nf, oracle, bt):
+    N, w, h, bits = 7, 96, 64, 16
+    frames = _yuv_clip(N, w, h, bits, bt)
+    params = dict(bt=bt, sigma=2.0, bw=16, bh=16)
+    out = nf.FFT3D(_device_clip(nf, frames, bits), **params)
+    left, right = bt // 2, (bt - 1) // 2
+    for n in range(N):
+        got = out.get_frame(n)
+        fallback = bt <= 1 or n < left or N - 1 - n < right
+        slots = [n] if fallback else list(range(n - bt // 2, n - bt // 2 + bt))
+        for p in range(3):
+            ref = oracle.fft3d_plane([frames[s][p] for s in slots], bits=bits, chroma=p > 0, **params)
+            assert np.array_equal(to_host(got[p], np.uint16), ref), (n, p)
+
+
+def test_fft3d_roi(nf, oracle):
+    N, w, h, bits = 3, 96, 64, 8
+    frames = _yuv_clip(N, w, h, bits, 9)
+    params = dict(bt=3, sigma=3.0, bw=16, bh=16)
+    out = nf.FFT3D(_device_clip(nf, frames, bits), l=8, t=4, r=16, b=12, **params)
+    got = out.get_frame(1)
+    for p in range(3):
+        a = 2 if p else 1
+        L, Tt, R, Bm = 8 // a, 4 // a, 16 // a, 12 // a
+        ph, pw = frames[1][p].shape
+        roi = [frames[j][p][Tt:ph - Bm, L:pw - R] for j in range(3)]
+        ref = frames[1][p].copy()
+        ref[Tt:ph - Bm, L:pw - R] = oracle.fft3d_plane(roi, bits=bits, chroma=p > 0, **params)
+        assert np.array_equal(to_host(got[p], np.uint8), ref), p
+
+
+class _KalmanPolicy:
+    """Independent restatement of the reference's Kalman request policy:
+    bounded warmup (filter.hpp:164-177) and the checkpoint cache
+    (checkpoints.hpp:32-79: four LRU entries, first published wins)."""
+
+    def __init__(self, oracle, params, dims, bits, warmup):
+        self.oracle, self.params, self.dims, self.bits, self.warmup = oracle, params, dims, bits, warmup
+        self.entries = []  # [frame, states, age]
+
+    def _touch(self, sel):
+        for e in self.entries:
+            if e is not sel and e[2] < sel[2]:
+                e[2] += 1
+        sel[2] = 0
+
+    def render(self, frames, n):
+        if n == 0:
+            return [p.copy() for p in frames[0]]
+        earliest = n - self.warmup - 1 if n > self.warmup else 0
+        cands = [e for e in self.entries if earliest <= e[0] <= n]
+        start = max(cands, key=lambda e: e[0]) if cands else None
+        if start:
+            self._touch(start)
+        replay = start[0] if start else earliest
+        if replay < n:
+            states = [s.copy() for s in start[1]] if start else [
+                self.oracle.KalmanState(self.params, w, h, bits=self.bits) for (h, w) in self.dims]
+            for k in range(replay + 1, n + 1):
+                for p, st in enumerate(states):
+                    st.advance(frames[k][p], chroma=p > 0)
+        else:
+            states = start[1]
+        outs = [st.render(chroma=p > 0) for p, st in enumerate(states)]
+        if replay < n and not any(e[0] == n for e in self.entries):
+            if len(self.entries) == 4:
+                self.entries.remove(max(self.entries, key=lambda e: e[2]))
+            e = [n, states, 4]
+            self.entries.append(e)
+            self._touch(e)
+        elif replay < n:
+            self._touch(next(e for e in self.entries if e[0] == n))
+        return outs
+
+
+@pytest.mark.parametrize("order", ["sequential", "seek"])
+def test_kalman_replay_policy(nf, oracle, order):
+    N, w, h, bits = 14, 64, 48, 10
+    frames = _yuv_clip(N, w, h, bits, 21)
+    params = dict(bt=0, sigma=2.0, bw=16, bh=16, kratio=2.0)
+    warmup = 3
+    out = nf.FFT3D(_device_clip(nf, frames, bits), kalman_warmup=warmup, **params)
+    policy = _KalmanPolicy(oracle, params, [p.shape for p in frames[0]], bits, warmup)
+    if order == "sequential":
+        seq = list(range(N))
+    else:
+        seq = [int(v) for v in np.random.default_rng(4).permutation(N)] + [3, 4, 5, 12, 2]
+    for n in seq:
+        got = out.get_frame(n)
+        ref = policy.render(frames, n)
+        for p in range(3):
+            assert np.array_equal(to_host(got[p], np.uint16), ref[p]), (order, n, p)
+    assert out.checkpoints() == sorted(e[0] for e in policy.entries)

@@ -214,3 +214,85 @@ def test_batch_matches_single(nf, oracle):
     assert int(flags.item()) == 0
     for (_, _, dst), ref in zip(jobs, refs):
         assert np.array_equal(to_host(dst, np.uint16), ref)
+
+
+# ------------------------------------------------------- DFTTest temporal
+def _clip_frames(n, h, w, bits, seed):
+    rng = np.random.default_rng(seed)
+    base = _rand_plane(rng, h, w, bits, False).astype(np.int64)
+    peak = (1 << bits) - 1
+    return [np.clip(base + rng.integers(-20, 21, size=(h, w)), 0, peak).astype(np.uint8 if bits == 8 else np.uint16)
+            for _ in range(n)]
+
+
+@pytest.mark.parametrize("T", [3, 5, 7])
+@pytest.mark.parametrize("B,O", [(8, 4), (16, 12), (32, 16)])
+def test_dfttest_tmode0(nf, oracle, T, B, O):
+    h, w = 50, 77
+    frames = _clip_frames(T, h, w, 8, T * 10 + B)
+    params = dict(tbsize=T, tmode=0, sbsize=B, sosize=O, ftype=0, sigma=8.0, twin=3)
+    plan = nf.dfttest_plan(w, h, bits=8, **params)
+    ref = oracle.dfttest_plane(frames, bits=8, **params)
+    got = _run(plan, frames)
+    assert_parity(got, ref, what=f"T{T} B{B}")
+    assert np.array_equal(got, ref), compare(got, ref)
+
+
+@pytest.mark.parametrize("T,O", [(2, 0), (4, 2), (5, 0), (5, 2), (6, 4)])
+def test_dfttest_tmode1(nf, oracle, T, O):
+    N = 12
+    h, w = 40, 56
+    clip = _clip_frames(N, h, w, 10, T * 7 + O)
+    params = dict(tbsize=T, tmode=1, tosize=O, sbsize=16, sosize=12, ftype=0, sigma=6.0, twin=0)
+    plan = nf.dfttest_plan(w, h, bits=10, **params)
+    for n in (0, 3, N - 1):
+        blocks = oracle.temporal_blocks(n, N, T, O)
+        slots = [clip[s] for _, _, sl in blocks for s in sl]
+        targets = [t for _, t, _ in blocks]
+        ref = oracle.dfttest_plane(slots, bits=10, targets=targets, **params)
+        got = _run(plan, slots, targets=targets)
+        assert np.array_equal(got, ref), (n, compare(got, ref))
+
+
+def test_dfttest_volume_matches_per_frame(nf, oracle):
+    import torch
+    T, N = 5, 11
+    h, w = 64, 96
+    clip = _clip_frames(N, h, w, 16, 3)
+    params = dict(tbsize=T, tmode=1, tosize=0, sbsize=16, sosize=12, ftype=0, sigma=8.0)
+    plan = nf.dfttest_plan(w, h, bits=16, **params)
+    flags = torch.zeros(1, dtype=torch.int32, device="cuda")
+    dev = [to_device(f) for f in clip]
+    for start in (-5, 0, 5, 10):
+        slots = [max(0, min(N - 1, start + z)) for z in range(T)]
+        outs = [empty_like_np(clip[0]) if 0 <= start + z < N else None for z in range(T)]
+        nf.dfttest_volume(plan, [dev[s] for s in slots], outs, flags)
+        torch.cuda.synchronize()
+        assert int(flags.item()) == 0
+        for z in range(T):
+            n = start + z
+            if not 0 <= n < N:
+                continue
+            ref = oracle.dfttest_plane([clip[s] for s in slots], bits=16, targets=[z], **params)
+            assert np.array_equal(to_host(outs[z], np.uint16), ref), (start, z)
+
+
+@pytest.mark.parametrize("mode", [1, 2, 4, 2147483647])
+@pytest.mark.parametrize("h,w", [(45, 70), (1100, 40)])
+def test_dfttest_dither(nf, oracle, mode, h, w):
+    rng = np.random.default_rng(mode % 97 + h)
+    img = _rand_plane(rng, h, w, 8, False)
+    params = dict(tbsize=1, sbsize=16, sosize=12, ftype=0, sigma=8.0, dither=mode, dither_seed=17)
+    plan = nf.dfttest_plan(w, h, bits=8, **params)
+    ref = oracle.dfttest_plane([img], bits=8, frame=3, plane=1, **params)
+    got = _run(plan, [img], frame=3, plane=1)
+    assert np.array_equal(got, ref), compare(got, ref)
+
+
+def test_dfttest_dither_temporal(nf, oracle):
+    frames = _clip_frames(5, 40, 56, 8, 77)
+    params = dict(tbsize=5, tmode=1, tosize=0, sbsize=16, sosize=12, ftype=0, sigma=8.0, dither=2, dither_seed=0)
+    plan = nf.dfttest_plan(56, 40, bits=8, **params)
+    ref = oracle.dfttest_plane(frames, bits=8, targets=[2], frame=7, plane=0, **params)
+    got = _run(plan, frames, targets=[2], frame=7, plane=0)
+    assert np.array_equal(got, ref), compare(got, ref)
