@@ -156,6 +156,22 @@ nf_status nf_dfttest_enqueue_volume(const nf_plan* plan, const nf_plane* sources
                                     const nf_mut_plane* dsts, int first_frame, int plane,
                                     unsigned* device_flags, void* stream);
 
+/* Differentiable DFTTest Wiener path (tbsize=1, ftype=0, f0beta=1, float32
+ * planes). No counterpart in the reference: for tuning the per-frequency
+ * noise curve. table: per-bin noise power a_k on the device (sbsize *
+ * (sbsize/2+1) floats, the calibrated curve the reference builds in
+ * profile.cpp:167-215), or NULL for the plan's own. forward = nf_plan_enqueue
+ * with that table. backward recomputes the block spectra and writes the
+ * gradient of a loss with respect to the source plane (grad_source, may have
+ * data NULL) and to the table (grad_table, device, may be NULL), given the
+ * loss gradient with respect to the output plane (grad_out). Deterministic;
+ * asynchronous like nf_plan_enqueue. */
+nf_status nf_dfttest_wiener_forward(const nf_plan* plan, nf_plane source, nf_mut_plane dst,
+                                    const float* table, unsigned* device_flags, void* stream);
+nf_status nf_dfttest_wiener_backward(const nf_plan* plan, nf_plane source, nf_plane grad_out,
+                                     nf_mut_plane grad_source, const float* table, float* grad_table,
+                                     unsigned* device_flags, void* stream);
+
 /* FFT3D Kalman mode (bt=0): device state of one plane (reference
  * KalmanState, src/kernels/kalman.hpp:5-14: complex L + float C + float Q,
  * 16 bytes per bin). create = Plan::initial_kalman (plan.cpp:293-300). */

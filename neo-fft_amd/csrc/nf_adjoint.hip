@@ -1,0 +1,325 @@
+// nf_adjoint.hip -- adjoint of the DFTTest spatial Wiener path on gfx950:
+// gradients of a loss on the float output plane with respect to the input
+// plane and to the per-bin noise table a_k (the calibrated sigma curve).
+//
+// Forward path per block (reference plan.cpp:553-602, scalar/spectral.cpp:5-53,
+// ftype=0 with exponent 1): x = q*h, X = rfft2(x), g = mean*X0/G0, m = g*G,
+// R = X - m, p = |R|^2, W = max((p-a)/(p+eps), 0), Y = W*R + m,
+// y = fct * C2R(Y), output accumulation += y*h_syn, out = acc * oscale.
+//
+// Adjoint per block (spectra are recomputed, never stored):
+//   gy      = h_syn * gacc                       (gacc = d loss / d acc)
+//   gY      = fct * w_kx * rfft2(gy)             (C2R adjoint; w=1 on the DC and
+//                                                  Nyquist columns, 2 elsewhere)
+//   gR      = W*gY + (dW/dp * 2<gY,R>) R,  ga_k += dW/da <gY,R>
+//   gX      = gR,  gX0.re += sum_k <gY-gR, G_k> * mean / G0.re
+//   gx      = C2R_unscaled(gX / w_kx),  gq += gx*h (canonical block order)
+// The padded-plane gradient is then folded through the one-reflection map
+// (reference geometry.hpp:45-48) in ascending cover order, and the per-tile
+// table gradients are summed in tile order: results are deterministic.
+#include "nf_device.hpp"
+#include "nf_fft.hpp"
+#include "nf_gain.hpp"
+#include "nf_launch.hpp"
+
+namespace nf {
+
+template <int B>
+__global__ __launch_bounds__(256) void dfttest_adjoint_kernel(const DftJob J, const float* grad_out,
+                                                               long long gout_stride, float* gpad, float* ga_part,
+                                                               unsigned* flags) {
+  extern __shared__ float smem[];
+  constexpr int K = B / 2 + 1;
+  constexpr int BINS = B * K;
+  const int tx = blockIdx.x % J.tiles_x, ty = blockIdx.x / J.tiles_x;
+  const int S = J.S, tid = threadIdx.x, nthreads = blockDim.x;
+  const int X0 = tx * J.tile_w, X1 = min(J.gx.cover, X0 + J.tile_w);
+  const int Y0 = ty * J.tile_h, Y1 = min(J.gy.cover, Y0 + J.tile_h);
+  const int bx_lo = max(0, ceil_div_int(X0 - B + 1, S)), bx_hi = min(J.gx.count - 1, (X1 - 1) / S);
+  const int by_lo = max(0, ceil_div_int(Y0 - B + 1, S)), by_hi = min(J.gy.count - 1, (Y1 - 1) / S);
+  const int nbx = bx_hi - bx_lo + 1;
+  const int ww = (nbx - 1) * S + B;
+  const int cx0 = bx_lo * S;
+  const int vx0 = J.gx.offset, vx1 = J.gx.offset + J.gx.length;
+  const int vy0 = J.gy.offset, vy1 = J.gy.offset + J.gy.length;
+
+  float* spec = smem;                            // [nbx_max][BINS][2] forward spectra X
+  float* gsp = spec + J.nbx_max * BINS * 2;      // [nbx_max][BINS][2] adjoint spectra
+  float* win = gsp + J.nbx_max * BINS * 2;       // [B][ww_max] decoded source
+  float* gwin = win + B * J.ww_max;              // [B][ww_max] gacc
+  float* gout = gwin + B * J.ww_max;             // [nbx_max][B][B] block input gradients
+  float* acc = gout + J.nbx_max * B * B;         // [tile_h][tile_w]
+  float* dc = acc + J.tile_h * J.tile_w;         // [nbx_max][2]
+  float* dgc = dc + 2 * J.nbx_max;               // [nbx_max][K] column partials of d/dg
+  float* gab = dgc + J.nbx_max * K;              // [nbx_max][BINS] per-block table gradients
+  float* gat = gab + J.nbx_max * BINS;           // [BINS] tile table gradient
+  float* hw = gat + BINS;                        // [B*B]
+  float* hs = hw + B * B;                        // [B*B]
+
+  bool bad = false;
+  for (int i = tid; i < B * B; i += nthreads) {
+    hw[i] = J.h[i];
+    hs[i] = J.hsyn[i];
+  }
+  for (int i = tid; i < J.tile_h * J.tile_w; i += nthreads) acc[i] = 0.0f;
+  for (int i = tid; i < BINS; i += nthreads) gat[i] = 0.0f;
+  const float fct = 1.0f / ((float)B * (float)B);
+  const float g0 = J.grid ? J.grid[0].x : 1.0f;
+  __syncthreads();
+
+  for (int by = by_lo; by <= by_hi && nbx > 0; ++by) {
+    const int oy = by * S;
+    for (int idx = tid; idx < B * ww; idx += nthreads) {
+      const int r = idx / ww, c = idx - r * ww;
+      const int cy = oy + r, cx = cx0 + c;
+      win[r * J.ww_max + c] = load_sample(J.src, J.src_stride, J.storage, reflect(cy, J.gy), reflect(cx, J.gx)) * J.dscale;
+      const bool vis = cy >= vy0 && cy < vy1 && cx >= vx0 && cx < vx1;
+      gwin[r * J.ww_max + c] = vis ? grad_out[(long long)(cy - vy0) * gout_stride + (cx - vx0)] * J.oscale : 0.0f;
+    }
+    __syncthreads();
+    // forward rows of x = q*h, adjoint rows of gy = gacc*h_syn
+    for (int t = tid; t < 2 * nbx * B; t += nthreads) {
+      const int which = t / (nbx * B);
+      const int rem = t - which * (nbx * B);
+      const int bi = rem / B, y = rem - bi * B;
+      float re[B], im[B];
+      const float* src = (which ? gwin : win) + y * J.ww_max + bi * S;
+      const float* wrow = (which ? hs : hw) + y * B;
+#pragma unroll
+      for (int x = 0; x < B; ++x) {
+        re[x] = src[x] * wrow[x];
+        im[x] = 0.0f;
+      }
+      cfft<B, true>(re, im);
+      float* dst = (which ? gsp : spec) + (bi * BINS + y * K) * 2;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        dst[2 * k] = re[k];
+        dst[2 * k + 1] = (k == B / 2) ? 0.0f : im[k];
+      }
+    }
+    __syncthreads();
+    for (int t = tid; t < 2 * nbx * K; t += nthreads) {
+      const int which = t / (nbx * K);
+      const int rem = t - which * (nbx * K);
+      const int bi = rem / K, k = rem - bi * K;
+      float re[B], im[B];
+      float* col = (which ? gsp : spec) + (bi * BINS + k) * 2;
+#pragma unroll
+      for (int y = 0; y < B; ++y) {
+        re[y] = col[y * K * 2];
+        im[y] = col[y * K * 2 + 1];
+      }
+      cfft<B, true>(re, im);
+      const float wk = (k == 0 || k == B / 2) ? 1.0f : 2.0f;
+#pragma unroll
+      for (int y = 0; y < B; ++y) {
+        col[y * K * 2] = which ? (fct * wk) * re[y] : re[y];
+        col[y * K * 2 + 1] = which ? (fct * wk) * im[y] : im[y];
+      }
+      if (!which && k == 0) {
+        dc[2 * bi] = re[0];
+        dc[2 * bi + 1] = im[0];
+      }
+    }
+    __syncthreads();
+    // gain adjoint per column; d/dg column partials and per-block table gradients
+    for (int t = tid; t < nbx * K; t += nthreads) {
+      const int bi = t / K, k = t - bi * K;
+      const float gs = J.grid ? (J.mean_scale * dc[2 * bi]) / g0 : 0.0f;
+      float dg = 0.0f;
+      for (int y = 0; y < B; ++y) {
+        const int bin = y * K + k;
+        float mr = 0.0f, mi = 0.0f, Gr = 0.0f, Gi = 0.0f;
+        if (J.grid) {
+          Gr = J.grid[bin].x;
+          Gi = J.grid[bin].y;
+          mr = gs * Gr;
+          mi = gs * Gi;
+        }
+        const float a = J.g.primary ? J.g.primary[bin] : J.g.a;
+        float* X = spec + (bi * BINS + bin) * 2;
+        float* gY = gsp + (bi * BINS + bin) * 2;
+        const float Rr = X[0] - mr, Ri = X[1] - mi;
+        const float p = Rr * Rr + Ri * Ri;
+        const float q = p + 1e-15f;
+        const float Wl = (p - a) / q;
+        const bool on = Wl > 0.0f;
+        const float W = on ? Wl : 0.0f;
+        const float dot = gY[0] * Rr + gY[1] * Ri;
+        const float dWdp = on ? (a + 1e-15f) / (q * q) : 0.0f;
+        const float dWda = on ? -1.0f / q : 0.0f;
+        const float gRr = W * gY[0] + (2.0f * dWdp * dot) * Rr;
+        const float gRi = W * gY[1] + (2.0f * dWdp * dot) * Ri;
+        dg = dg + ((gY[0] - gRr) * Gr + (gY[1] - gRi) * Gi);
+        gab[bi * BINS + bin] = dWda * dot;
+        gY[0] = gRr;
+        gY[1] = gRi;
+        bad |= !isfinite(gRr) || !isfinite(gRi);
+      }
+      dgc[bi * K + k] = dg;
+    }
+    __syncthreads();
+    // g = mean*X0.re/G0.re feeds every bin's mean: fold d/dg into gX0.re,
+    // and accumulate table gradients of the blocks this tile owns
+    for (int bi = tid; bi < nbx; bi += nthreads) {
+      float dg = 0.0f;
+      for (int k = 0; k < K; ++k) dg = dg + dgc[bi * K + k];
+      if (J.grid) gsp[(bi * BINS) * 2] = gsp[(bi * BINS) * 2] + dg * (J.mean_scale / g0);
+    }
+    for (int bin = tid; bin < BINS; bin += nthreads) {
+      float s = gat[bin];
+      for (int bi = 0; bi < nbx; ++bi) {
+        const int bxo = (bx_lo + bi) * S;
+        if (bxo >= X0 && bxo < X1 && oy >= Y0 && oy < Y1) s = s + gab[bi * BINS + bin];
+      }
+      gat[bin] = s;
+    }
+    __syncthreads();
+    // adjoint of the forward transform: unscaled inverse of gX / w
+    for (int t = tid; t < nbx * K; t += nthreads) {
+      const int bi = t / K, k = t - bi * K;
+      const float inv_w = (k == 0 || k == B / 2) ? 1.0f : 0.5f;
+      float re[B], im[B];
+      float* col = gsp + (bi * BINS + k) * 2;
+#pragma unroll
+      for (int y = 0; y < B; ++y) {
+        re[y] = col[y * K * 2] * inv_w;
+        im[y] = col[y * K * 2 + 1] * inv_w;
+      }
+      cfft<B, false>(re, im);
+#pragma unroll
+      for (int y = 0; y < B; ++y) {
+        col[y * K * 2] = re[y];
+        col[y * K * 2 + 1] = im[y];
+      }
+    }
+    __syncthreads();
+    for (int t = tid; t < nbx * B; t += nthreads) {
+      const int bi = t / B, y = t - bi * B;
+      float re[B], im[B];
+      const float* row = gsp + (bi * BINS + y * K) * 2;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        re[k] = row[2 * k];
+        im[k] = row[2 * k + 1];
+      }
+#pragma unroll
+      for (int k = K; k < B; ++k) {
+        re[k] = re[B - k];
+        im[k] = 0.0f - im[B - k];
+      }
+      cfft<B, false>(re, im);
+      float* o = gout + (bi * B + y) * B;
+#pragma unroll
+      for (int x = 0; x < B; ++x) o[x] = re[x] * hw[y * B + x];
+    }
+    __syncthreads();
+    {
+      const int r0 = max(oy, Y0), r1 = min(oy + B, Y1);
+      const int nr = r1 - r0, TW = X1 - X0;
+      for (int idx = tid; idx < nr * TW; idx += nthreads) {
+        const int rr = idx / TW;
+        const int x = X0 + (idx - rr * TW);
+        const int y = r0 + rr;
+        float sum = acc[(y - Y0) * J.tile_w + (x - X0)];
+        const int b_first = max(bx_lo, ceil_div_int(x - B + 1, S)), b_last = min(bx_hi, x / S);
+        for (int bx = b_first; bx <= b_last; ++bx) sum = sum + gout[((bx - bx_lo) * B + (y - oy)) * B + (x - bx * S)];
+        acc[(y - Y0) * J.tile_w + (x - X0)] = sum;
+      }
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < (Y1 - Y0) * (X1 - X0); idx += nthreads) {
+    const int rr = idx / (X1 - X0), c = idx - rr * (X1 - X0);
+    gpad[(long long)(Y0 + rr) * J.gx.cover + X0 + c] = acc[rr * J.tile_w + c];
+  }
+  for (int bin = tid; bin < BINS; bin += nthreads) ga_part[(long long)blockIdx.x * BINS + bin] = gat[bin];
+  if (bad) atomicOr(flags, NF_FLAG_NONFINITE);
+}
+
+// Fold the padded-plane gradient through the one-reflection map: source
+// sample s receives the cover positions c with reflect(c) = s, ascending c.
+__global__ void reflect_fold_kernel(const float* gpad, Axis gx, Axis gy, float dscale, float* gsrc, long long gstride) {
+  const int sx = blockIdx.x * blockDim.x + threadIdx.x, sy = blockIdx.y;
+  if (sx >= gx.length || sy >= gy.length) return;
+  int cys[3], ncy = 0, cxs[3], ncx = 0;
+  const Axis* ax[2] = {&gy, &gx};
+  const int s[2] = {sy, sx};
+  for (int d = 0; d < 2; ++d) {
+    const Axis& a = *ax[d];
+    int* out = d ? cxs : cys;
+    int& n = d ? ncx : ncy;
+    const int cands[3] = {a.offset - s[d], a.offset + s[d], a.offset + 2 * a.length - 2 - s[d]};
+    for (int i = 0; i < 3; ++i) {
+      const int c = cands[i];
+      if (c < 0 || c >= a.cover || reflect(c, a) != s[d]) continue;
+      bool dup = false;
+      for (int k = 0; k < n; ++k) dup |= out[k] == c;
+      if (!dup) out[n++] = c;
+    }
+  }
+  float sum = 0.0f;
+  for (int i = 0; i < ncy; ++i)
+    for (int j = 0; j < ncx; ++j) sum = sum + gpad[(long long)cys[i] * gx.cover + cxs[j]];
+  gsrc[(long long)sy * gstride + sx] = sum * dscale;
+}
+
+__global__ void table_reduce_kernel(const float* ga_part, int tiles, int bins, float* grad_table) {
+  const int bin = blockIdx.x * blockDim.x + threadIdx.x;
+  if (bin >= bins) return;
+  float s = 0.0f;
+  for (int t = 0; t < tiles; ++t) s = s + ga_part[(long long)t * bins + bin];
+  grad_table[bin] = s;
+}
+
+size_t dfttest_adjoint_lds_bytes(int B, int nbx_max, int ww_max, int tile_w, int tile_h) {
+  const int K = B / 2 + 1, BINS = B * K;
+  return sizeof(float) * ((size_t)2 * nbx_max * BINS * 2 + 2 * (size_t)B * ww_max + (size_t)nbx_max * B * B +
+                          (size_t)tile_w * tile_h + 2 * (size_t)nbx_max + (size_t)nbx_max * K +
+                          (size_t)nbx_max * BINS + BINS + 2 * (size_t)B * B);
+}
+
+hipError_t launch_dfttest_adjoint(int B, const DftJob& J, int tiles, size_t lds, const float* grad_out,
+                                  long long gout_stride, float* gpad, float* ga_part, float* gsrc, long long gstride,
+                                  float* grad_table, unsigned* flags, hipStream_t stream) {
+  dim3 grid(tiles), block(256);
+  switch (B) {
+    case 8:
+      hipLaunchKernelGGL(dfttest_adjoint_kernel<8>, grid, block, lds, stream, J, grad_out, gout_stride, gpad, ga_part, flags);
+      break;
+    case 16:
+      hipLaunchKernelGGL(dfttest_adjoint_kernel<16>, grid, block, lds, stream, J, grad_out, gout_stride, gpad, ga_part, flags);
+      break;
+    case 32:
+      hipLaunchKernelGGL(dfttest_adjoint_kernel<32>, grid, block, lds, stream, J, grad_out, gout_stride, gpad, ga_part, flags);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (gsrc) {
+    dim3 g2((J.gx.length + 255) / 256, J.gy.length);
+    hipLaunchKernelGGL(reflect_fold_kernel, g2, dim3(256), 0, stream, gpad, J.gx, J.gy, J.dscale, gsrc, gstride);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (grad_table) {
+    const int bins = B * (B / 2 + 1);
+    hipLaunchKernelGGL(table_reduce_kernel, dim3((bins + 127) / 128), dim3(128), 0, stream, ga_part, tiles, bins,
+                       grad_table);
+    e = hipGetLastError();
+  }
+  return e;
+}
+
+hipError_t dfttest_adjoint_set_lds_limit() {
+  hipError_t e = hipFuncSetAttribute((const void*)dfttest_adjoint_kernel<8>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)dfttest_adjoint_kernel<16>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void*)dfttest_adjoint_kernel<32>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+
+}  // namespace nf

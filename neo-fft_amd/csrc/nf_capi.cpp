@@ -62,6 +62,7 @@ void ensure_lds_limits() {
     g_lds_status = dfttest_t1_set_lds_limit();
     if (g_lds_status == hipSuccess) g_lds_status = dfttest_vol_set_lds_limit();
     if (g_lds_status == hipSuccess) g_lds_status = dither_set_lds_limit();
+    if (g_lds_status == hipSuccess) g_lds_status = dfttest_adjoint_set_lds_limit();
     if (g_lds_status == hipSuccess) g_lds_status = fft3d_set_lds_limit();
   });
   hip_check(g_lds_status, "hipFuncSetAttribute");
@@ -974,6 +975,71 @@ extern "C" nf_status nf_dfttest_enqueue_volume(const nf_plan* plan, const nf_pla
     for (int z = 0; z < plan->temporal; ++z)
       if (dsts[z].data) require(dsts[z].stride_bytes == first.stride_bytes, "volume outputs must share a stride");
     enqueue_dft_vol_jobs({plan}, {make_dft_job(*plan, sources[0], first)}, {q}, device_flags, (hipStream_t)stream);
+    return NF_OK;
+  });
+}
+
+// ------------------------------------------------- differentiable Wiener path
+namespace {
+void require_wiener(const nf_plan* plan) {
+  require(plan && plan->algorithm == 0, "not a DFTTest plan");
+  require(plan->temporal == 1 && plan->g.type == 0 && std::fabs(plan->g.exponent - 1.0f) < 0.00005f,
+          "the differentiable path needs tbsize=1, ftype=0 and f0beta=1");
+  require(plan->storage == kF32, "the differentiable path needs float32 planes");
+}
+}  // namespace
+
+extern "C" nf_status nf_dfttest_wiener_forward(const nf_plan* plan, nf_plane source, nf_mut_plane dst,
+                                               const float* table, unsigned* device_flags, void* stream) {
+  return guarded([&]() -> nf_status {
+    require_wiener(plan);
+    require(device_flags != nullptr, "device flag word required");
+    check_plane(*plan, source.width, source.height, source.stride_bytes, source.data);
+    check_plane(*plan, dst.width, dst.height, dst.stride_bytes, dst.data);
+    DftJob j = make_dft_job(*plan, source, dst);
+    if (table) j.g.primary = table;
+    enqueue_dft_jobs({plan}, {j}, device_flags, (hipStream_t)stream);
+    return NF_OK;
+  });
+}
+
+extern "C" nf_status nf_dfttest_wiener_backward(const nf_plan* plan, nf_plane source, nf_plane grad_out,
+                                                nf_mut_plane grad_source, const float* table, float* grad_table,
+                                                unsigned* device_flags, void* stream) {
+  return guarded([&]() -> nf_status {
+    require_wiener(plan);
+    require(device_flags != nullptr, "device flag word required");
+    check_plane(*plan, source.width, source.height, source.stride_bytes, source.data);
+    check_plane(*plan, grad_out.width, grad_out.height, grad_out.stride_bytes, grad_out.data);
+    if (grad_source.data)
+      check_plane(*plan, grad_source.width, grad_source.height, grad_source.stride_bytes, grad_source.data);
+    ensure_lds_limits();
+    const hipStream_t st = (hipStream_t)stream;
+    nf_mut_plane none{};
+    DftJob j = make_dft_job(*plan, source, none);
+    if (table) j.g.primary = table;
+    j.tile_w = 32;
+    j.tile_h = 32;
+    j.tiles_x = (plan->gx.cover + j.tile_w - 1) / j.tile_w;
+    j.tiles_y = (plan->gy.cover + j.tile_h - 1) / j.tile_h;
+    j.nbx_max = (j.tile_w + plan->B - 2) / plan->S + 1;
+    j.ww_max = (j.nbx_max - 1) * plan->S + plan->B;
+    const size_t lds = dfttest_adjoint_lds_bytes(plan->B, j.nbx_max, j.ww_max, j.tile_w, j.tile_h);
+    require(lds <= 160 * 1024, "DFTTest adjoint tile does not fit the LDS");
+    const int tiles = j.tiles_x * j.tiles_y;
+    const int bins = plan->B * (plan->B / 2 + 1);
+    float *gpad = nullptr, *gpart = nullptr;
+    hip_check(hipMallocAsync((void**)&gpad, sizeof(float) * (size_t)plan->gx.cover * plan->gy.cover, st),
+              "hipMallocAsync(adjoint)");
+    hip_check(hipMallocAsync((void**)&gpart, sizeof(float) * (size_t)tiles * bins, st), "hipMallocAsync(adjoint)");
+    timed(3, st, [&] {
+      hip_check(launch_dfttest_adjoint(plan->B, j, tiles, lds, static_cast<const float*>(grad_out.data),
+                                       grad_out.stride_bytes / 4, gpad, gpart, static_cast<float*>(grad_source.data),
+                                       grad_source.stride_bytes / 4, grad_table, device_flags, st),
+                "dfttest adjoint launch");
+    });
+    hip_check(hipFreeAsync(gpad, st), "hipFreeAsync");
+    hip_check(hipFreeAsync(gpart, st), "hipFreeAsync");
     return NF_OK;
   });
 }

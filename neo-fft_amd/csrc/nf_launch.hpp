@@ -15,6 +15,12 @@ hipError_t launch_dither(const float* acc, long long acc_stride, int w, int h, u
                          int mode, int seed, int frame, int plane, hipStream_t stream);
 hipError_t dither_set_lds_limit();
 
+size_t dfttest_adjoint_lds_bytes(int B, int nbx_max, int ww_max, int tile_w, int tile_h);
+hipError_t launch_dfttest_adjoint(int B, const DftJob& J, int tiles, size_t lds, const float* grad_out,
+                                  long long gout_stride, float* gpad, float* ga_part, float* gsrc, long long gstride,
+                                  float* grad_table, unsigned* flags, hipStream_t stream);
+hipError_t dfttest_adjoint_set_lds_limit();
+
 size_t fft3d_block_lds_bytes(int BW, int BH, int T, int NB, int Sx);
 hipError_t launch_fft3d_block(int BW, int BH, const Fft3dLaunch& L, int total_groups, size_t lds, hipStream_t stream);
 hipError_t launch_fft3d_synthesis(const Fft3dLaunch& L, hipStream_t stream);

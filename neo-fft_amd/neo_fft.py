@@ -112,6 +112,10 @@ _SIGNATURES = {
     "nf_dfttest_enqueue_volume": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Plane), ctypes.c_int,
                                                  ctypes.POINTER(MutPlane), ctypes.c_int, ctypes.c_int,
                                                  ctypes.c_void_p, ctypes.c_void_p]),
+    "nf_dfttest_wiener_forward": (ctypes.c_int, [ctypes.c_void_p, Plane, MutPlane, ctypes.c_void_p, ctypes.c_void_p,
+                                                 ctypes.c_void_p]),
+    "nf_dfttest_wiener_backward": (ctypes.c_int, [ctypes.c_void_p, Plane, Plane, MutPlane, ctypes.c_void_p,
+                                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "nf_kalman_state_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     "nf_kalman_state_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "nf_kalman_state_destroy": (None, [ctypes.c_void_p]),
@@ -299,6 +303,24 @@ def dfttest_volume(plan, sources, dsts, flags, first_frame=0, plane=0, stream=No
     outs = (MutPlane * len(dsts))(*[MutPlane(None, 0, 0, 0) if d is None else _plane(d, MutPlane) for d in dsts])
     _check(lib().nf_dfttest_enqueue_volume(plan.handle, srcs, len(sources), outs, first_frame, plane,
                                            ctypes.c_void_p(flags.data_ptr()), _stream_handle(stream)))
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def wiener_forward(plan, source, dst, flags, table=None, stream=None):
+    """DFTTest Wiener (tbsize=1, ftype=0) with an optional device noise table."""
+    _check(lib().nf_dfttest_wiener_forward(plan.handle, _plane(source), _plane(dst, MutPlane), _ptr(table),
+                                           ctypes.c_void_p(flags.data_ptr()), _stream_handle(stream)))
+
+
+def wiener_backward(plan, source, grad_out, flags, grad_source=None, table=None, grad_table=None, stream=None):
+    """Adjoint of wiener_forward: gradients w.r.t. the source plane and the table."""
+    gs = MutPlane(None, 0, 0, 0) if grad_source is None else _plane(grad_source, MutPlane)
+    _check(lib().nf_dfttest_wiener_backward(plan.handle, _plane(source), _plane(grad_out), gs, _ptr(table),
+                                            _ptr(grad_table), ctypes.c_void_p(flags.data_ptr()),
+                                            _stream_handle(stream)))
 
 
 def _fmt(bits, floating, chroma):

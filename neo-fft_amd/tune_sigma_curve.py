@@ -1,0 +1,141 @@
+#!/usr/bin/env python3
+"""Tune DFTTest's per-frequency sigma curve by gradient descent on the GPU.
+
+The DFTTest Wiener path (tbsize=1, ftype=0) is wrapped in a
+torch.autograd.Function whose forward and backward are the gfx950 kernels
+behind nf_dfttest_wiener_forward / nf_dfttest_wiener_backward. The curve is a
+list of (position, value) knots with fixed positions; the per-bin noise table
+is built from it in torch with the reference's calibration
+(src/algorithms/profile.cpp:139-215: knot values raised to 1/dims, linear
+interpolation over the folded frequency, separable product, divided by the
+window scale), so autograd chains the table gradient back to the knots.
+
+Usage: python neo-fft_amd/tune_sigma_curve.py [--steps 20] [--knots 16]
+Data: 16 seeded 256x256 float32 patches of random sinusoid sums in [0,1]
+plus Gaussian noise sigma=0.05; loss = MSE against the clean patches.
+"""
+import argparse
+import os
+import sys
+
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+import neo_fft  # noqa: E402
+
+
+def folded_frequency(n, device, dtype):
+    i = torch.arange(n, device=device, dtype=dtype)
+    return torch.minimum(i, n - i) / float(n // 2)
+
+
+def curve_table(positions, values, block, divisor, dims=2):
+    """Per-bin noise power a[y, x] of a shared (slocation) curve, ssystem=0,
+    tbsize=1: knot values ** (1/dims), piecewise-linear in the folded
+    frequency of each axis, product over axes, divided by `divisor`."""
+    v = values.clamp_min(0) ** (1.0 / dims)
+    pos = positions.to(values.dtype)
+
+    def interp(f):
+        idx = torch.searchsorted(pos, f.contiguous()).clamp(1, len(pos) - 1)
+        lo, hi = pos[idx - 1], pos[idx]
+        t = (f - lo) / (hi - lo)
+        return v[idx - 1] * (1 - t) + v[idx] * t
+
+    fy = folded_frequency(block, values.device, values.dtype)
+    fx = fy[: block // 2 + 1]
+    return (interp(fy)[:, None] * interp(fx)[None, :] / divisor).reshape(-1)
+
+
+class DFTTestWiener(torch.autograd.Function):
+    """y = DFTTest(x; a) on one float32 plane, a = per-bin noise table."""
+
+    @staticmethod
+    def forward(ctx, x, table, plan):
+        flags = torch.zeros(1, dtype=torch.int32, device=x.device)
+        x = x.contiguous()
+        t32 = table.detach().float().contiguous()
+        y = torch.empty_like(x)
+        neo_fft.wiener_forward(plan, x, y, flags, table=t32, stream=torch.cuda.current_stream())
+        ctx.save_for_backward(x, t32)
+        ctx.plan = plan
+        return y
+
+    @staticmethod
+    def backward(ctx, gy):
+        x, t32 = ctx.saved_tensors
+        flags = torch.zeros(1, dtype=torch.int32, device=x.device)
+        gx = torch.empty_like(x)
+        gt = torch.empty_like(t32)
+        neo_fft.wiener_backward(ctx.plan, x, gy.contiguous().float(), flags, grad_source=gx, table=t32,
+                                grad_table=gt, stream=torch.cuda.current_stream())
+        return gx, gt, None
+
+
+def wscale_of(block=16, overlap=12):
+    """Window scale of the plan (Hann, tbsize=1): 1 / sum(h^2)."""
+    import math
+    raw = [0.5 - 0.5 * math.cos(2 * math.pi * (j + 0.5) / block) for j in range(block)]
+    step = block - overlap
+    sw = []
+    for j in range(block):
+        d = sum(raw[k] ** 2 for k in range(j, -1, -step)) + sum(raw[k] ** 2 for k in range(j + step, block, step))
+        sw.append(raw[j] / math.sqrt(d))
+    energy = 0.0
+    for y in range(block):
+        for x in range(block):
+            h = float(torch.tensor(sw[y] * sw[x] / block, dtype=torch.float32))
+            energy += h * h
+    return 1.0 / energy
+
+
+def synthetic_patches(n=16, size=256, sigma=0.05, seed=5, device="cuda"):
+    g = torch.Generator().manual_seed(seed)
+    yy, xx = torch.meshgrid(torch.arange(size, dtype=torch.float64), torch.arange(size, dtype=torch.float64),
+                            indexing="ij")
+    clean, noisy = [], []
+    for _ in range(n):
+        img = torch.zeros(size, size, dtype=torch.float64)
+        for _ in range(6):
+            th = torch.rand(1, generator=g).item() * 3.14159
+            per = 6 + torch.rand(1, generator=g).item() * 50
+            ph = torch.rand(1, generator=g).item() * 6.283
+            img += torch.sin((xx * torch.cos(torch.tensor(th)) + yy * torch.sin(torch.tensor(th))) * 6.283 / per + ph)
+        img = (img - img.min()) / (img.max() - img.min())
+        clean.append(img.float())
+        noisy.append((img + sigma * torch.randn(size, size, generator=g, dtype=torch.float64)).float())
+    return torch.stack(clean).to(device), torch.stack(noisy).to(device)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--knots", type=int, default=16)
+    ap.add_argument("--lr", type=float, default=0.3)
+    args = ap.parse_args()
+    clean, noisy = synthetic_patches()
+    n, size = clean.shape[0], clean.shape[1]
+    plan = neo_fft.dfttest_plan(size, size, bits=32, floating=True, tbsize=1, sbsize=16, sosize=12, ftype=0,
+                                sigma=8.0)
+    divisor = wscale_of()
+    positions = torch.linspace(0, 1, args.knots, dtype=torch.float64, device="cuda")
+    log_values = torch.full((args.knots,), float(torch.log(torch.tensor(8.0))), dtype=torch.float64, device="cuda",
+                            requires_grad=True)
+    opt = torch.optim.Adam([log_values], lr=args.lr)
+    for step in range(args.steps):
+        opt.zero_grad()
+        table = curve_table(positions, log_values.exp(), 16, divisor)
+        loss = 0
+        for i in range(n):
+            out = DFTTestWiener.apply(noisy[i], table.float(), plan)
+            loss = loss + ((out - clean[i]) ** 2).mean()
+        loss = loss / n
+        loss.backward()
+        opt.step()
+        print(f"step {step:3d}  mse {loss.item():.6e}  curve {[round(v, 3) for v in log_values.exp().tolist()]}")
+
+
+if __name__ == "__main__":
+    main()

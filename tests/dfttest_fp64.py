@@ -1,0 +1,69 @@
+"""binary64 restatement of the DFTTest spatial Wiener path on float planes
+(reference plan.cpp:170-226 and :538-640 with ftype=0, exponent 1, zmean),
+used as the finite-difference reference for the adjoint kernel. Windows and
+geometry follow windows.cpp:97-152 and geometry.hpp:26-44 without binary32
+rounding."""
+import math
+
+import numpy as np
+
+
+def _axis(length, block, overlap):
+    step = block - overlap
+    cover = block * ((length + block - 1) // block) + 2 * max(step, overlap)
+    return dict(step=step, cover=cover, count=(cover - block) // step + 1, offset=(cover - length) // 2)
+
+
+def _hann_window(block, overlap):
+    raw = np.array([0.5 - 0.5 * math.cos(2 * math.pi * (j + 0.5) / block) for j in range(block)])
+    step = block - overlap
+    sw = np.empty(block)
+    for j in range(block):
+        d = sum(raw[k] ** 2 for k in range(j, -1, -step)) + sum(raw[k] ** 2 for k in range(j + step, block, step))
+        sw[j] = raw[j] / math.sqrt(d)
+    return np.outer(sw, sw) / block
+
+
+def _reflect_index(n, a, length):
+    j = n - a["offset"]
+    return np.where(j < 0, -j, np.where(j < length, j, 2 * length - 2 - j))
+
+
+def forward(img, table, block=16, overlap=12, zmean=True):
+    """img: float64 [h, w] (float plane values); table: per-bin a [block*(block/2+1)]."""
+    h, w = img.shape
+    ax, ay = _axis(w, block, overlap), _axis(h, block, overlap)
+    hwin = _hann_window(block, overlap)
+    hsyn = hwin * block * block
+    q = img[np.ix_(_reflect_index(np.arange(ay["cover"]), ay, h), _reflect_index(np.arange(ax["cover"]), ax, w))] * 255
+    S = ax["step"]
+    blocks = np.lib.stride_tricks.sliding_window_view(q, (block, block))[::S, ::S]
+    X = np.fft.rfft2(blocks * hwin)
+    a = np.asarray(table, np.float64).reshape(block, block // 2 + 1)
+    if zmean:
+        G = np.fft.rfft2(255 * hwin)
+        m = (X[..., :1, :1].real / G[0, 0].real) * G
+    else:
+        m = np.zeros_like(X)
+    R = X - m
+    p = np.abs(R) ** 2
+    W = np.maximum((p - a) / (p + 1e-15), 0)
+    y = np.fft.irfft2(W * R + m, s=(block, block)) * hsyn
+    acc = np.zeros((ay["cover"], ax["cover"]))
+    for by in range(ay["count"]):
+        for bx in range(ax["count"]):
+            acc[by * S:by * S + block, bx * S:bx * S + block] += y[by, bx]
+    return acc[ay["offset"]:ay["offset"] + h, ax["offset"]:ax["offset"] + w] / 255
+
+
+def curve_table(positions, values, block, divisor, dims=2):
+    v = np.maximum(np.asarray(values, np.float64), 0) ** (1.0 / dims)
+    pos = np.asarray(positions, np.float64)
+
+    def interp(f):
+        return np.interp(f, pos, v)
+
+    i = np.arange(block)
+    fy = np.minimum(i, block - i) / (block // 2)
+    fx = fy[: block // 2 + 1]
+    return (interp(fy)[:, None] * interp(fx)[None, :] / divisor).reshape(-1)

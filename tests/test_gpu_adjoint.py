@@ -1,0 +1,102 @@
+"""Adjoint of the DFTTest Wiener path (nf_dfttest_wiener_backward) against
+binary64 central differences (h = 1e-4) of a binary64 forward, with respect
+to input pixels, to the per-bin noise table, and -- through the curve map of
+the tuning script's autograd Function -- to a 16-knot sigma curve."""
+import numpy as np
+import pytest
+
+import dfttest_fp64 as F64
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def setup(nf):
+    import torch
+    rng = np.random.default_rng(3)
+    size = 48
+    yy, xx = np.mgrid[0:size, 0:size]
+    clean = 0.5 + 0.3 * np.sin(xx * 0.4 + yy * 0.15) + 0.1 * np.cos(yy * 0.7)
+    noisy = (clean + rng.normal(0, 0.05, clean.shape)).astype(np.float32)
+    plan = nf.dfttest_plan(size, size, bits=32, floating=True, tbsize=1, sbsize=16, sosize=12, ftype=0, sigma=8.0)
+    table = rng.uniform(0.5, 6.0, 16 * 9).astype(np.float32)
+    return dict(torch=torch, size=size, clean=clean, noisy=noisy, plan=plan, table=table)
+
+
+def _gpu_grads(nf, s):
+    torch = s["torch"]
+    x = torch.from_numpy(s["noisy"]).cuda()
+    t = torch.from_numpy(s["table"]).cuda()
+    flags = torch.zeros(1, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    nf.wiener_forward(s["plan"], x, y, flags, table=t)
+    gout = 2 * (y.double() - torch.from_numpy(s["clean"]).cuda()).float()
+    gx = torch.empty_like(x)
+    gt = torch.empty_like(t)
+    nf.wiener_backward(s["plan"], x, gout.contiguous(), flags, grad_source=gx, table=t, grad_table=gt)
+    torch.cuda.synchronize()
+    assert int(flags.item()) == 0
+    return y.cpu().numpy(), gx.cpu().numpy().astype(np.float64), gt.cpu().numpy().astype(np.float64)
+
+
+def _loss64(img, table, clean):
+    return float(((F64.forward(img, table) - clean) ** 2).sum())
+
+
+def test_forward_matches_binary64(nf, setup):
+    y, _, _ = _gpu_grads(nf, setup)
+    ref = F64.forward(setup["noisy"].astype(np.float64), setup["table"].astype(np.float64))
+    assert np.abs(y - ref).max() <= 2e-5
+
+
+def test_input_gradient_against_central_differences(nf, setup):
+    _, gx, _ = _gpu_grads(nf, setup)
+    img = setup["noisy"].astype(np.float64)
+    tab = setup["table"].astype(np.float64)
+    h = 1e-4
+    scale = np.abs(gx).max()
+    for (py, px) in [(0, 0), (1, 30), (24, 24), (47, 13), (10, 47), (33, 2)]:
+        e = np.zeros_like(img)
+        e[py, px] = h
+        fd = (_loss64(img + e, tab, setup["clean"]) - _loss64(img - e, tab, setup["clean"])) / (2 * h)
+        assert abs(gx[py, px] - fd) <= 2e-3 * scale + 1e-3 * abs(fd), (py, px, gx[py, px], fd)
+
+
+def test_table_gradient_against_central_differences(nf, setup):
+    _, _, gt = _gpu_grads(nf, setup)
+    img = setup["noisy"].astype(np.float64)
+    tab = setup["table"].astype(np.float64)
+    h = 1e-4
+    scale = np.abs(gt).max()
+    for k in [0, 1, 5, 9, 40, 77, 100, 143]:
+        e = np.zeros_like(tab)
+        e[k] = h
+        fd = (_loss64(img, tab + e, setup["clean"]) - _loss64(img, tab - e, setup["clean"])) / (2 * h)
+        assert abs(gt[k] - fd) <= 2e-3 * scale + 1e-3 * abs(fd), (k, gt[k], fd)
+
+
+def test_curve_gradient_through_autograd(nf, setup):
+    """16-knot curve -> table (torch) -> kernel forward/backward (autograd
+    Function of the tuning script) against binary64 differences of the curve."""
+    torch = setup["torch"]
+    import tune_sigma_curve as tune
+    divisor = 5.0
+    pos = np.linspace(0, 1, 16)
+    vals = np.linspace(2.0, 20.0, 16) ** 1.0
+    x = torch.from_numpy(setup["noisy"]).cuda()
+    v = torch.tensor(vals, dtype=torch.float64, device="cuda", requires_grad=True)
+    table = tune.curve_table(torch.tensor(pos, device="cuda"), v, 16, divisor)
+    out = tune.DFTTestWiener.apply(x, table.float(), setup["plan"])
+    loss = ((out.double() - torch.from_numpy(setup["clean"]).cuda()) ** 2).sum()
+    loss.backward()
+    g = v.grad.cpu().numpy()
+    img = setup["noisy"].astype(np.float64)
+    h = 1e-4
+    scale = np.abs(g).max()
+    for k in range(16):
+        e = np.zeros(16)
+        e[k] = h
+        lp = _loss64(img, F64.curve_table(pos, vals + e, 16, divisor), setup["clean"])
+        lm = _loss64(img, F64.curve_table(pos, vals - e, 16, divisor), setup["clean"])
+        fd = (lp - lm) / (2 * h)
+        assert abs(g[k] - fd) <= 3e-3 * scale + 2e-3 * abs(fd), (k, g[k], fd)
