@@ -38,13 +38,29 @@ def parse():
     ap.add_argument("--cpu-frames", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-dfttest", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run 5 device copies of this many MiB (counter calibration) and exit")
     return ap.parse_args()
 
 
+def copy_calibration(mib):
+    """A plain copy kernel of known size: HBM counters read on it give the
+    scale between counter units and bytes (read mib MiB, write mib MiB)."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each_way": n * 4, "copies": 5}))
+    return 0
+
+
 def fft3d_flops_per_frame(plans):
-    """Model of SURVEY-style FP32 work: real N-point FFT = 2.5 N log2 N per
-    transform axis pass, T forward + 1 inverse 2D transforms per block,
-    temporal filter (T^2*8 + T*20) per bin, window/synthesis 3 per sample."""
+    """FP32 work model: real N-point FFT = 2.5 N log2 N per transform axis
+    pass, T forward + 1 inverse 2D transforms per block, temporal filter
+    (T^2*8 + T*20) per bin, window/synthesis 3 per sample."""
     import math
     total = 0.0
     for p in plans:
@@ -63,6 +79,8 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
         return 2
+    if args.copy_mib:
+        return copy_calibration(args.copy_mib)
     import numpy as np
     import torch
     import neo_fft as nf

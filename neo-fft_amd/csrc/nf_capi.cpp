@@ -260,18 +260,31 @@ void choose_dft_tiling(nf_plan& p) {
   p.lds = dfttest_t1_lds_bytes(p.B, j);
   require(p.lds <= 160 * 1024, "DFTTest tile does not fit the LDS");
   if (p.temporal > 1) {
+    // Three launch shapes: 0 = one covering temporal block (ring
+    // accumulator), 1 = several covering blocks (tile accumulator), 2 = all
+    // slices of a lattice block. Each takes the widest tile whose chunk of
+    // one block fits the LDS, then grows the chunk within two workgroups
+    // per CU. A shape that cannot fit is left unavailable (vol_lds = 0).
     const int T = p.temporal;
-    const size_t budget = 76 * 1024;  // two workgroups per CU
+    const size_t budget = 76 * 1024;
     for (int m = 0; m < 3; ++m) {
       const int nout = m == 2 ? T : 1;
-      const int acc_rows = m == 1 ? p.vol_tile_h[m] : p.B;
-      const int nbx = (p.vol_tile_w[m] + p.B - 2) / p.S + 1;
-      int c = 1;
-      while (c < nbx && dfttest_vol_lds_bytes(p.B, T, c + 1, p.S, nout, acc_rows, p.vol_tile_w[m]) <= budget) ++c;
-      p.vol_chunk[m] = c;
-      p.vol_lds[m] = dfttest_vol_lds_bytes(p.B, T, c, p.S, nout, acc_rows, p.vol_tile_w[m]);
-      require(p.vol_lds[m] <= 160 * 1024, "DFTTest temporal tile does not fit the LDS");
+      p.vol_lds[m] = 0;
+      for (int tw : {64, 48, 32, 16}) {
+        const int acc_rows = m == 1 ? p.vol_tile_h[m] : p.B;
+        if (dfttest_vol_lds_bytes(p.B, T, 1, p.S, nout, acc_rows, tw) > 160 * 1024) continue;
+        p.vol_tile_w[m] = tw;
+        const int nbx = (tw + p.B - 2) / p.S + 1;
+        int c = 1;
+        while (c < nbx && dfttest_vol_lds_bytes(p.B, T, c + 1, p.S, nout, acc_rows, tw) <= budget) ++c;
+        p.vol_chunk[m] = c;
+        p.vol_lds[m] = dfttest_vol_lds_bytes(p.B, T, c, p.S, nout, acc_rows, tw);
+        break;
+      }
     }
+    const bool single = p.tmode == 0 || p.tosize == 0;
+    if (!(single ? p.vol_lds[0] : p.vol_lds[1]))
+      throw Unsupported("DFTTest temporal block does not fit the LDS at this sbsize/tbsize");
   }
 }
 }  // namespace
@@ -766,10 +779,13 @@ void enqueue_dft_vol_jobs(const std::vector<const nf_plan*>& plans, const std::v
     int tiles = 0;
     size_t lds = 0;
     const int B = plans[first]->B;
+    int max_t = 0;
     for (size_t i = 0; i < n; ++i) {
       const nf_plan& p = *plans[first + i];
       require(p.B == B, "batched DFTTest jobs must share sbsize");
+      max_t = std::max(max_t, tq[first + i].T);
       const int m = tq[first + i].emit_all ? 2 : (tq[first + i].ntb > 1 ? 1 : 0);
+      if (!p.vol_lds[m]) throw Unsupported("DFTTest temporal block does not fit the LDS at this sbsize/tbsize");
       L.jobs[i] = retile(jobs[first + i], p, m);
       L.tq[i] = tq[first + i];
       L.tile_start[i] = tiles;
@@ -779,7 +795,7 @@ void enqueue_dft_vol_jobs(const std::vector<const nf_plan*>& plans, const std::v
     L.tile_start[n] = tiles;
     L.njobs = (int)n;
     L.flags = flags;
-    timed(0, stream, [&] { hip_check(launch_dfttest_vol(B, L, tiles, lds, stream), "dfttest temporal kernel launch"); });
+    timed(0, stream, [&] { hip_check(launch_dfttest_vol(B, max_t, L, tiles, lds, stream), "dfttest temporal kernel launch"); });
   }
 }
 

@@ -39,20 +39,107 @@ __device__ __forceinline__ void store_dft_output(const DftJob& J, void* dst, int
 }
 
 // Inverse temporal transform of one bin for slice s (fft_backend.cpp:542-548).
+template <int NT>
 __device__ __forceinline__ Cpx inverse_slice(int T, const Cpx* v, int s, const float* ttw) {
   if ((T == 3 || T == 5) && s == T / 2) return temporal_inverse_center(T, v);
   if (T >= 3 && T <= 5) {
-    Cpx w[5];
-    for (int z = 0; z < T; ++z) w[z] = v[z];
+    Cpx w[NT > 5 ? 5 : NT];
+#pragma unroll
+    for (int z = 0; z < (NT > 5 ? 5 : NT); ++z)
+      if (z < T) w[z] = v[z];
     temporal_butterfly(T, w, false);
-    return w[s];
+    Cpx r = w[0];
+#pragma unroll
+    for (int z = 1; z < (NT > 5 ? 5 : NT); ++z)
+      if (z == s) r = w[z];
+    return r;
   }
   return dft_term(T, v, s, ttw, false);
 }
 
+// Stages D and E of one chunk. TT > 0 fixes the temporal depth at compile
+// time (T == TT); TT == 0 handles any depth up to 15.
+template <int B, int TT>
+__device__ __forceinline__ void temporal_stages(const DftJob& J, const DftTemporal& Q, float* spec, float* dc, int nb,
+                                                int Tdyn, int target, int tid, int nthreads, bool& bad) {
+  constexpr int K = B / 2 + 1;
+  constexpr int BINS2 = B * K;
+  constexpr int NT = TT > 0 ? TT : 15;
+  const int T = TT > 0 ? TT : Tdyn;
+  // D: forward temporal transform per bin
+  for (int t = tid; t < nb * BINS2; t += nthreads) {
+    const int bi = t / BINS2, bin = t - bi * BINS2;
+    Cpx v[NT];
+    float* base = spec + ((bi * T) * BINS2 + bin) * 2;
+#pragma unroll
+    for (int z = 0; z < NT; ++z)
+      if (z < T) v[z] = Cpx{base[z * BINS2 * 2], base[z * BINS2 * 2 + 1]};
+    if (T >= 3 && T <= 5) {
+      temporal_butterfly(T, v, true);
+    } else {
+      Cpx w[NT];
+#pragma unroll
+      for (int m = 0; m < NT; ++m)
+        if (m < T) w[m] = dft_term(T, v, m, Q.ttw, true);
+#pragma unroll
+      for (int m = 0; m < NT; ++m)
+        if (m < T) v[m] = w[m];
+    }
+#pragma unroll
+    for (int z = 0; z < NT; ++z)
+      if (z < T) {
+        base[z * BINS2 * 2] = v[z].r;
+        base[z * BINS2 * 2 + 1] = v[z].i;
+      }
+    if (bin == 0) {
+      dc[2 * bi] = v[0].r;
+      dc[2 * bi + 1] = v[0].i;
+    }
+  }
+  __syncthreads();
+  // E: gain with 3D mean removal, inverse temporal transform
+  const bool use_grid = J.grid && J.mean_scale != 0.0f && J.grid[0].x != 0.0f;
+  for (int t = tid; t < nb * BINS2; t += nthreads) {
+    const int bi = t / BINS2, bin = t - bi * BINS2;
+    const float gs = use_grid ? (J.mean_scale * dc[2 * bi]) / J.grid[0].x : 0.0f;
+    Cpx v[NT];
+    float* base = spec + ((bi * T) * BINS2 + bin) * 2;
+#pragma unroll
+    for (int z = 0; z < NT; ++z) {
+      if (z >= T) continue;
+      float re = base[z * BINS2 * 2], im = base[z * BINS2 * 2 + 1];
+      const int k3 = z * BINS2 + bin;
+      float mr = 0.0f, mi = 0.0f;
+      if (J.grid) {
+        const float2 gv = J.grid[k3];
+        mr = gs * gv.x;
+        mi = gs * gv.y;
+      }
+      apply_gain(J.g, k3, re, im, mr, mi, bad);
+      v[z] = Cpx{re, im};
+    }
+    if (Q.emit_all) {
+      Cpx w[NT];
+#pragma unroll
+      for (int z = 0; z < NT; ++z)
+        if (z < T) w[z] = inverse_slice<NT>(T, v, z, Q.ttw);
+#pragma unroll
+      for (int z = 0; z < NT; ++z)
+        if (z < T) {
+          base[z * BINS2 * 2] = w[z].r;
+          base[z * BINS2 * 2 + 1] = w[z].i;
+        }
+    } else {
+      const Cpx r = inverse_slice<NT>(T, v, target, Q.ttw);
+      base[0] = r.r;
+      base[1] = r.i;
+    }
+  }
+}
+
 }  // namespace
 
-template <int B>
+template <int B, bool kAnyDepth>
 __global__ __launch_bounds__(256) void dfttest_vol_kernel(const DftVolLaunch L) {
   extern __shared__ float smem[];
   constexpr int K = B / 2 + 1;
@@ -160,59 +247,20 @@ __global__ __launch_bounds__(256) void dfttest_vol_kernel(const DftVolLaunch L) 
           }
         }
         __syncthreads();
-        // D: temporal transform per bin
-        for (int t = tid; t < nb * BINS2; t += nthreads) {
-          const int bi = t / BINS2, bin = t - bi * BINS2;
-          Cpx v[15];
-          float* base = spec + ((bi * T) * BINS2 + bin) * 2;
-          for (int z = 0; z < T; ++z) v[z] = Cpx{base[z * BINS2 * 2], base[z * BINS2 * 2 + 1]};
-          if (T >= 3 && T <= 5) {
-            temporal_butterfly(T, v, true);
-          } else {
-            Cpx w[15];
-            for (int m = 0; m < T; ++m) w[m] = dft_term(T, v, m, Q.ttw, true);
-            for (int m = 0; m < T; ++m) v[m] = w[m];
-          }
-          for (int z = 0; z < T; ++z) {
-            base[z * BINS2 * 2] = v[z].r;
-            base[z * BINS2 * 2 + 1] = v[z].i;
-          }
-          if (bin == 0) {
-            dc[2 * bi] = v[0].r;
-            dc[2 * bi + 1] = v[0].i;
-          }
-        }
-        __syncthreads();
-        // E: gain over the 3D spectrum, inverse temporal transform
-        for (int t = tid; t < nb * BINS2; t += nthreads) {
-          const int bi = t / BINS2, bin = t - bi * BINS2;
-          const bool use_grid = J.grid && J.mean_scale != 0.0f && J.grid[0].x != 0.0f;
-          const float gs = use_grid ? (J.mean_scale * dc[2 * bi]) / J.grid[0].x : 0.0f;
-          Cpx v[15];
-          float* base = spec + ((bi * T) * BINS2 + bin) * 2;
-          for (int z = 0; z < T; ++z) {
-            float re = base[z * BINS2 * 2], im = base[z * BINS2 * 2 + 1];
-            const int k3 = z * BINS2 + bin;
-            float mr = 0.0f, mi = 0.0f;
-            if (J.grid) {
-              const float2 gv = J.grid[k3];
-              mr = gs * gv.x;
-              mi = gs * gv.y;
-            }
-            apply_gain(J.g, k3, re, im, mr, mi, bad);
-            v[z] = Cpx{re, im};
-          }
-          if (Q.emit_all) {
-            Cpx w[15];
-            for (int z = 0; z < T; ++z) w[z] = inverse_slice(T, v, z, Q.ttw);
-            for (int z = 0; z < T; ++z) {
-              base[z * BINS2 * 2] = w[z].r;
-              base[z * BINS2 * 2 + 1] = w[z].i;
-            }
-          } else {
-            const Cpx r = inverse_slice(T, v, target, Q.ttw);
-            base[0] = r.r;
-            base[1] = r.i;
+        // D: temporal transform per bin; E: gain over the 3D spectrum and
+        // inverse temporal transform. Depths 2..7 are compile-time so the
+        // per-bin vectors stay in registers; the kAnyDepth instance takes
+        // depths 8..15 (the host picks the instance from the depth).
+        if (kAnyDepth) {
+          temporal_stages<B, 0>(J, Q, spec, dc, nb, T, target, tid, nthreads, bad);
+        } else {
+          switch (T) {
+            case 2: temporal_stages<B, 2>(J, Q, spec, dc, nb, T, target, tid, nthreads, bad); break;
+            case 3: temporal_stages<B, 3>(J, Q, spec, dc, nb, T, target, tid, nthreads, bad); break;
+            case 4: temporal_stages<B, 4>(J, Q, spec, dc, nb, T, target, tid, nthreads, bad); break;
+            case 5: temporal_stages<B, 5>(J, Q, spec, dc, nb, T, target, tid, nthreads, bad); break;
+            case 6: temporal_stages<B, 6>(J, Q, spec, dc, nb, T, target, tid, nthreads, bad); break;
+            default: temporal_stages<B, 7>(J, Q, spec, dc, nb, T, target, tid, nthreads, bad); break;
           }
         }
         for (int o = 0; o < nout; ++o) {
@@ -332,30 +380,38 @@ size_t dfttest_vol_lds_bytes(int B, int T, int chunk, int S, int nout, int acc_r
                           (size_t)nout * B * B);
 }
 
-hipError_t launch_dfttest_vol(int B, const DftVolLaunch& L, int total_tiles, size_t lds, hipStream_t stream) {
+hipError_t launch_dfttest_vol(int B, int max_depth, const DftVolLaunch& L, int total_tiles, size_t lds, hipStream_t stream) {
   dim3 grid(total_tiles), block(256);
+  const bool any = max_depth > 7;
+#define NF_VOL_LAUNCH(BB)                                                                   \
+  if (any) hipLaunchKernelGGL((dfttest_vol_kernel<BB, true>), grid, block, lds, stream, L); \
+  else hipLaunchKernelGGL((dfttest_vol_kernel<BB, false>), grid, block, lds, stream, L);
   switch (B) {
     case 8:
-      hipLaunchKernelGGL(dfttest_vol_kernel<8>, grid, block, lds, stream, L);
+      NF_VOL_LAUNCH(8)
       break;
     case 16:
-      hipLaunchKernelGGL(dfttest_vol_kernel<16>, grid, block, lds, stream, L);
+      NF_VOL_LAUNCH(16)
       break;
     case 32:
-      hipLaunchKernelGGL(dfttest_vol_kernel<32>, grid, block, lds, stream, L);
+      NF_VOL_LAUNCH(32)
       break;
     default:
       return hipErrorInvalidValue;
   }
+#undef NF_VOL_LAUNCH
   return hipGetLastError();
 }
 
 hipError_t dfttest_vol_set_lds_limit() {
-  hipError_t e = hipFuncSetAttribute((const void*)dfttest_vol_kernel<8>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void*)dfttest_vol_kernel<16>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void*)dfttest_vol_kernel<32>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  const void* fns[] = {(const void*)dfttest_vol_kernel<8, false>,  (const void*)dfttest_vol_kernel<8, true>,
+                       (const void*)dfttest_vol_kernel<16, false>, (const void*)dfttest_vol_kernel<16, true>,
+                       (const void*)dfttest_vol_kernel<32, false>, (const void*)dfttest_vol_kernel<32, true>};
+  for (const void* f : fns) {
+    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace nf

@@ -8,7 +8,7 @@ hipError_t launch_dfttest_t1(int B, const DftLaunch& L, int total_tiles, size_t 
 hipError_t dfttest_t1_set_lds_limit();
 
 size_t dfttest_vol_lds_bytes(int B, int T, int chunk, int S, int nout, int acc_rows, int tile_w);
-hipError_t launch_dfttest_vol(int B, const DftVolLaunch& L, int total_tiles, size_t lds, hipStream_t stream);
+hipError_t launch_dfttest_vol(int B, int max_depth, const DftVolLaunch& L, int total_tiles, size_t lds, hipStream_t stream);
 hipError_t dfttest_vol_set_lds_limit();
 
 hipError_t launch_dither(const float* acc, long long acc_stride, int w, int h, uint8_t* dst, long long dst_stride,
