@@ -250,9 +250,9 @@ void choose_dft_tiling(nf_plan& p) {
   const char* tw = std::getenv("NF_DFT_TILE_W");
   const char* th = std::getenv("NF_DFT_TILE_H");
   p.tile_w = tw ? std::max(8, std::atoi(tw)) : 64;
-  p.tile_h = th ? std::max(8, std::atoi(th)) : 64;
+  p.tile_h = th ? std::max(8, std::atoi(th)) : 128;
   p.nbx_max = (p.tile_w + p.B - 2) / p.S + 1;
-  p.ww_max = (p.nbx_max - 1) * p.S + p.B;
+  p.ww_max = odd_stride((p.nbx_max - 1) * p.S + p.B);
   DftJob j{};
   j.nbx_max = p.nbx_max;
   j.ww_max = p.ww_max;
@@ -507,7 +507,7 @@ extern "C" nf_status nf_fft3d_plan_create(int width, int height, nf_sample_forma
     p->lower = (c.beta - 1.0f) / c.beta;
     p->base = (!fmt.floating && fmt.chroma) ? float(1 << (fmt.bits - 1)) : 0.0f;
     p->peak = fmt.floating ? 1.0f : float((1 << fmt.bits) - 1);
-    p->NB = c.bw == 32 ? 4 : c.bw == 16 ? 8 : 16;
+    p->NB = c.bw == 32 ? 2 : c.bw == 16 ? 4 : 8;
     if (const char* nb = std::getenv("NF_FFT3D_NB")) p->NB = std::max(1, std::atoi(nb));
     p->groups_x = (p->gx.count + p->NB - 1) / p->NB;
     p->lds = fft3d_block_lds_bytes(c.bw, c.bh, p->temporal, p->NB, p->gx.step);

@@ -108,6 +108,13 @@ struct DftVolLaunch {
 
 // One FFT3D plane of one output frame: block stage writes windowed
 // contributions, the synthesis stage sums them in the canonical order.
+// LDS layout helpers. Spectra are kept as separate real and imaginary planes
+// with a row stride of K = B/2+1 (odd for B = 8, 16, 32) and a block stride
+// padded by one row, and staged source rows use an odd row stride, so the
+// per-lane row and column walks of the transforms spread over the LDS banks.
+__host__ __device__ constexpr int spec_block_stride(int B) { return (B + 1) * (B / 2 + 1); }
+__host__ __device__ constexpr int odd_stride(int n) { return n | 1; }
+
 struct Fft3dJob {
   Axis gx, gy;
   int storage;

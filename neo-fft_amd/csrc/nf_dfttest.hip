@@ -40,19 +40,25 @@ __global__ __launch_bounds__(256) void dfttest_t1_kernel(const DftLaunch L) {
   const int cx0 = bx_lo * S;
   const int TW = X1 - X0;
 
-  // LDS carve-up (floats)
-  float* spec = smem;                                  // [nbx_max][B][K][2]
-  float* out = spec + J.nbx_max * B * K * 2;           // [nbx_max][B][B]
-  float* win = out + J.nbx_max * B * B;                // [B][ww_max]
+  // LDS carve-up (floats). ww_max is an odd row stride; spectra are real and
+  // imaginary planes with block stride BS; block outputs and windows use a
+  // row stride of B + 1.
+  constexpr int BS = spec_block_stride(B);
+  constexpr int OS = B + 1;
+  float* sre = smem;                                   // [nbx_max][BS]
+  float* sim = sre + J.nbx_max * BS;                   // [nbx_max][BS]
+  float* out = sim + J.nbx_max * BS;                   // [nbx_max][B][OS]
+  float* win = out + J.nbx_max * B * OS;               // [B][ww_max]
   float* ring = win + B * J.ww_max;                    // [B][tile_w]
   float* dc = ring + B * J.tile_w;                     // [nbx_max][2]
-  float* hw = dc + 2 * J.nbx_max;                      // [B*B]
-  float* hs = hw + B * B;                              // [B*B]
+  float* hw = dc + 2 * J.nbx_max;                      // [B][OS]
+  float* hs = hw + B * OS;                             // [B][OS]
 
   bool bad = false;
   for (int i = tid; i < B * B; i += nthreads) {
-    hw[i] = J.h[i];
-    hs[i] = J.hsyn[i];
+    const int y = i / B, x = i - y * B;
+    hw[y * OS + x] = J.h[i];
+    hs[y * OS + x] = J.hsyn[i];
   }
   for (int i = tid; i < B * J.tile_w; i += nthreads) ring[i] = 0.0f;
   const float fct = 1.0f / ((float)B * (float)B);
@@ -76,15 +82,16 @@ __global__ __launch_bounds__(256) void dfttest_t1_kernel(const DftLaunch L) {
       const float* src = win + y * J.ww_max + bi * S;
 #pragma unroll
       for (int x = 0; x < B; ++x) {
-        re[x] = src[x] * hw[y * B + x];
+        re[x] = src[x] * hw[y * OS + x];
         im[x] = 0.0f;
       }
       cfft<B, true>(re, im);
-      float* dst = spec + ((bi * B + y) * K) * 2;
+      float* dr = sre + bi * BS + y * K;
+      float* di = sim + bi * BS + y * K;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        dst[2 * k] = re[k];
-        dst[2 * k + 1] = (k == B / 2) ? 0.0f : im[k];
+        dr[k] = re[k];
+        di[k] = (k == B / 2) ? 0.0f : im[k];
       }
     }
     __syncthreads();
@@ -92,17 +99,18 @@ __global__ __launch_bounds__(256) void dfttest_t1_kernel(const DftLaunch L) {
     for (int t = tid; t < nbx * K; t += nthreads) {
       const int bi = t / K, k = t - bi * K;
       float re[B], im[B];
-      float* col = spec + (bi * B * K + k) * 2;
+      float* cr = sre + bi * BS + k;
+      float* ci = sim + bi * BS + k;
 #pragma unroll
       for (int y = 0; y < B; ++y) {
-        re[y] = col[y * K * 2];
-        im[y] = col[y * K * 2 + 1];
+        re[y] = cr[y * K];
+        im[y] = ci[y * K];
       }
       cfft<B, true>(re, im);
 #pragma unroll
       for (int y = 0; y < B; ++y) {
-        col[y * K * 2] = re[y];
-        col[y * K * 2 + 1] = im[y];
+        cr[y * K] = re[y];
+        ci[y * K] = im[y];
       }
       if (k == 0) {
         dc[2 * bi] = re[0];
@@ -115,11 +123,12 @@ __global__ __launch_bounds__(256) void dfttest_t1_kernel(const DftLaunch L) {
       const int bi = t / K, k = t - bi * K;
       const float gs = J.grid ? (J.mean_scale * dc[2 * bi]) / J.grid[0].x : 0.0f;
       float re[B], im[B];
-      float* col = spec + (bi * B * K + k) * 2;
+      float* cr = sre + bi * BS + k;
+      float* ci = sim + bi * BS + k;
 #pragma unroll
       for (int y = 0; y < B; ++y) {
-        re[y] = col[y * K * 2];
-        im[y] = col[y * K * 2 + 1];
+        re[y] = cr[y * K];
+        im[y] = ci[y * K];
         const int bin = y * K + k;
         float mr = 0.0f, mi = 0.0f;
         if (J.grid) {
@@ -132,8 +141,8 @@ __global__ __launch_bounds__(256) void dfttest_t1_kernel(const DftLaunch L) {
       cfft<B, false>(re, im);
 #pragma unroll
       for (int y = 0; y < B; ++y) {
-        col[y * K * 2] = re[y];
-        col[y * K * 2 + 1] = im[y];
+        cr[y * K] = re[y];
+        ci[y * K] = im[y];
       }
     }
     __syncthreads();
@@ -141,11 +150,12 @@ __global__ __launch_bounds__(256) void dfttest_t1_kernel(const DftLaunch L) {
     for (int t = tid; t < nbx * B; t += nthreads) {
       const int bi = t / B, y = t - bi * B;
       float re[B], im[B];
-      const float* row = spec + ((bi * B + y) * K) * 2;
+      const float* rr = sre + bi * BS + y * K;
+      const float* ri = sim + bi * BS + y * K;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        re[k] = row[2 * k];
-        im[k] = row[2 * k + 1];
+        re[k] = rr[k];
+        im[k] = ri[k];
       }
 #pragma unroll
       for (int k = K; k < B; ++k) {
@@ -153,12 +163,12 @@ __global__ __launch_bounds__(256) void dfttest_t1_kernel(const DftLaunch L) {
         im[k] = 0.0f - im[B - k];
       }
       cfft<B, false>(re, im);
-      float* o = out + (bi * B + y) * B;
+      float* o = out + (bi * B + y) * OS;
 #pragma unroll
       for (int x = 0; x < B; ++x) {
         const float v = re[x] * fct;
         bad |= !isfinite(v);
-        o[x] = v * hs[y * B + x];
+        o[x] = v * hs[y * OS + x];
       }
     }
     __syncthreads();
@@ -176,7 +186,7 @@ __global__ __launch_bounds__(256) void dfttest_t1_kernel(const DftLaunch L) {
         const int b_last = min(bx_hi, x / S);
         for (int bx = b_first; bx <= b_last; ++bx) {
           const int bi = bx - bx_lo;
-          sum = sum + out[(bi * B + (y - oy)) * B + (x - bx * S)];
+          sum = sum + out[(bi * B + (y - oy)) * OS + (x - bx * S)];
         }
         *cell = sum;
       }
@@ -218,8 +228,10 @@ __global__ __launch_bounds__(256) void dfttest_t1_kernel(const DftLaunch L) {
 
 size_t dfttest_t1_lds_bytes(int B, const DftJob& j) {
   const int K = B / 2 + 1;
-  return sizeof(float) * ((size_t)j.nbx_max * B * K * 2 + (size_t)j.nbx_max * B * B + (size_t)B * j.ww_max +
-                          (size_t)B * j.tile_w + 2 * (size_t)j.nbx_max + 2 * (size_t)B * B);
+  (void)K;
+  return sizeof(float) * ((size_t)j.nbx_max * spec_block_stride(B) * 2 + (size_t)j.nbx_max * B * (B + 1) +
+                          (size_t)B * j.ww_max + (size_t)B * j.tile_w + 2 * (size_t)j.nbx_max +
+                          2 * (size_t)B * (B + 1));
 }
 
 hipError_t launch_dfttest_t1(int B, const DftLaunch& L, int total_tiles, size_t lds, hipStream_t stream) {

@@ -60,20 +60,22 @@ __device__ __forceinline__ Cpx inverse_slice(int T, const Cpx* v, int s, const f
 // Stages D and E of one chunk. TT > 0 fixes the temporal depth at compile
 // time (T == TT); TT == 0 handles any depth up to 15.
 template <int B, int TT>
-__device__ __forceinline__ void temporal_stages(const DftJob& J, const DftTemporal& Q, float* spec, float* dc, int nb,
+__device__ __forceinline__ void temporal_stages(const DftJob& J, const DftTemporal& Q, float* sre, float* sim, float* dc, int nb,
                                                 int Tdyn, int target, int tid, int nthreads, bool& bad) {
   constexpr int K = B / 2 + 1;
   constexpr int BINS2 = B * K;
+  constexpr int BS = spec_block_stride(B);
   constexpr int NT = TT > 0 ? TT : 15;
   const int T = TT > 0 ? TT : Tdyn;
   // D: forward temporal transform per bin
   for (int t = tid; t < nb * BINS2; t += nthreads) {
     const int bi = t / BINS2, bin = t - bi * BINS2;
     Cpx v[NT];
-    float* base = spec + ((bi * T) * BINS2 + bin) * 2;
+    float* br = sre + (bi * T) * BS + bin;
+    float* bim = sim + (bi * T) * BS + bin;
 #pragma unroll
     for (int z = 0; z < NT; ++z)
-      if (z < T) v[z] = Cpx{base[z * BINS2 * 2], base[z * BINS2 * 2 + 1]};
+      if (z < T) v[z] = Cpx{br[z * BS], bim[z * BS]};
     if (T >= 3 && T <= 5) {
       temporal_butterfly(T, v, true);
     } else {
@@ -88,8 +90,8 @@ __device__ __forceinline__ void temporal_stages(const DftJob& J, const DftTempor
 #pragma unroll
     for (int z = 0; z < NT; ++z)
       if (z < T) {
-        base[z * BINS2 * 2] = v[z].r;
-        base[z * BINS2 * 2 + 1] = v[z].i;
+        br[z * BS] = v[z].r;
+        bim[z * BS] = v[z].i;
       }
     if (bin == 0) {
       dc[2 * bi] = v[0].r;
@@ -103,11 +105,12 @@ __device__ __forceinline__ void temporal_stages(const DftJob& J, const DftTempor
     const int bi = t / BINS2, bin = t - bi * BINS2;
     const float gs = use_grid ? (J.mean_scale * dc[2 * bi]) / J.grid[0].x : 0.0f;
     Cpx v[NT];
-    float* base = spec + ((bi * T) * BINS2 + bin) * 2;
+    float* br = sre + (bi * T) * BS + bin;
+    float* bim = sim + (bi * T) * BS + bin;
 #pragma unroll
     for (int z = 0; z < NT; ++z) {
       if (z >= T) continue;
-      float re = base[z * BINS2 * 2], im = base[z * BINS2 * 2 + 1];
+      float re = br[z * BS], im = bim[z * BS];
       const int k3 = z * BINS2 + bin;
       float mr = 0.0f, mi = 0.0f;
       if (J.grid) {
@@ -126,13 +129,13 @@ __device__ __forceinline__ void temporal_stages(const DftJob& J, const DftTempor
 #pragma unroll
       for (int z = 0; z < NT; ++z)
         if (z < T) {
-          base[z * BINS2 * 2] = w[z].r;
-          base[z * BINS2 * 2 + 1] = w[z].i;
+          br[z * BS] = w[z].r;
+          bim[z * BS] = w[z].i;
         }
     } else {
       const Cpx r = inverse_slice<NT>(T, v, target, Q.ttw);
-      base[0] = r.r;
-      base[1] = r.i;
+      br[0] = r.r;
+      bim[0] = r.i;
     }
   }
 }
@@ -143,7 +146,6 @@ template <int B, bool kAnyDepth>
 __global__ __launch_bounds__(256) void dfttest_vol_kernel(const DftVolLaunch L) {
   extern __shared__ float smem[];
   constexpr int K = B / 2 + 1;
-  constexpr int BINS2 = B * K;
   int jid = 0;
   while (jid + 1 < L.njobs && (int)blockIdx.x >= L.tile_start[jid + 1]) ++jid;
   const DftJob& J = L.jobs[jid];
@@ -161,19 +163,22 @@ __global__ __launch_bounds__(256) void dfttest_vol_kernel(const DftVolLaunch L) 
   const int bx_lo = ceil_div_int(X0 - B + 1, S), bx_hi = (X1 - 1) / S;
   const int by_lo = ceil_div_int(Y0 - B + 1, S), by_hi = (Y1 - 1) / S;
   const int TW = X1 - X0, TH = Y1 - Y0;
-  const int ww_c = (NBc - 1) * S + B;
+  const int ww_c = odd_stride((NBc - 1) * S + B);
+  constexpr int BS = spec_block_stride(B);
+  constexpr int OS = B + 1;
   const int acc_rows = ring_mode ? B : J.tile_h;
 
-  float* spec = smem;                                 // [NBc][T][BINS2][2]
-  float* out = spec + NBc * T * BINS2 * 2;            // [NBc][B][B]
-  float* win = out + NBc * B * B;                     // [T][B][ww_c]
+  float* sre = smem;                                  // [NBc][T][BS] real parts
+  float* sim = sre + NBc * T * BS;                    // [NBc][T][BS] imaginary parts
+  float* out = sim + NBc * T * BS;                    // [NBc][B][OS]
+  float* win = out + NBc * B * OS;                    // [T][B][ww_c]
   float* acc = win + T * B * ww_c;                    // [nout][acc_rows][tile_w]
   float* dc = acc + nout * acc_rows * J.tile_w;       // [NBc][2]
-  float* hw = dc + 2 * NBc;                           // [T][B][B]
-  float* hs = hw + T * B * B;                         // [nout][B][B]
+  float* hw = dc + 2 * NBc;                           // [T][B][OS]
+  float* hs = hw + T * B * OS;                        // [nout][B][OS]
 
   bool bad = false;
-  for (int i = tid; i < T * B * B; i += nthreads) hw[i] = J.h[i];
+  for (int i = tid; i < T * B * B; i += nthreads) hw[(i / B) * OS + i % B] = J.h[i];
   for (int i = tid; i < nout * acc_rows * J.tile_w; i += nthreads) acc[i] = 0.0f;
   const float fct = 1.0f / ((float)T * (float)B * (float)B);
   const float volume = (float)T * (float)B * (float)B;
@@ -184,7 +189,7 @@ __global__ __launch_bounds__(256) void dfttest_vol_kernel(const DftVolLaunch L) 
     for (int i = tid; i < nout * B * B; i += nthreads) {
       const int o = i / (B * B), e = i - o * (B * B);
       const int z = Q.emit_all ? o : target;
-      hs[i] = Q.tmode1 ? J.h[z * B * B + e] : J.hsyn[z * B * B + e];
+      hs[(o * B + e / B) * OS + e % B] = Q.tmode1 ? J.h[z * B * B + e] : J.hsyn[z * B * B + e];
     }
     for (int by = by_lo; by <= by_hi; ++by) {
       const int oy = by * S;
@@ -212,18 +217,19 @@ __global__ __launch_bounds__(256) void dfttest_vol_kernel(const DftVolLaunch L) 
           const int bi = rem / B, y = rem - bi * B;
           float re[B], im[B];
           const float* src = win + (z * B + y) * ww_c + bi * S;
-          const float* hrow = hw + (z * B + y) * B;
+          const float* hrow = hw + (z * B + y) * OS;
 #pragma unroll
           for (int x = 0; x < B; ++x) {
             re[x] = src[x] * hrow[x];
             im[x] = 0.0f;
           }
           cfft<B, true>(re, im);
-          float* dst = spec + (((bi * T + z) * BINS2) + y * K) * 2;
+          float* dr = sre + (bi * T + z) * BS + y * K;
+          float* di = sim + (bi * T + z) * BS + y * K;
 #pragma unroll
           for (int k = 0; k < K; ++k) {
-            dst[2 * k] = re[k];
-            dst[2 * k + 1] = (k == B / 2) ? 0.0f : im[k];
+            dr[k] = re[k];
+            di[k] = (k == B / 2) ? 0.0f : im[k];
           }
         }
         __syncthreads();
@@ -233,17 +239,18 @@ __global__ __launch_bounds__(256) void dfttest_vol_kernel(const DftVolLaunch L) 
           const int rem = t - z * (nb * K);
           const int bi = rem / K, k = rem - bi * K;
           float re[B], im[B];
-          float* col = spec + (((bi * T + z) * BINS2) + k) * 2;
+          float* cr = sre + (bi * T + z) * BS + k;
+          float* ci = sim + (bi * T + z) * BS + k;
 #pragma unroll
           for (int y = 0; y < B; ++y) {
-            re[y] = col[y * K * 2];
-            im[y] = col[y * K * 2 + 1];
+            re[y] = cr[y * K];
+            im[y] = ci[y * K];
           }
           cfft<B, true>(re, im);
 #pragma unroll
           for (int y = 0; y < B; ++y) {
-            col[y * K * 2] = re[y];
-            col[y * K * 2 + 1] = im[y];
+            cr[y * K] = re[y];
+            ci[y * K] = im[y];
           }
         }
         __syncthreads();
@@ -252,15 +259,15 @@ __global__ __launch_bounds__(256) void dfttest_vol_kernel(const DftVolLaunch L) 
         // per-bin vectors stay in registers; the kAnyDepth instance takes
         // depths 8..15 (the host picks the instance from the depth).
         if (kAnyDepth) {
-          temporal_stages<B, 0>(J, Q, spec, dc, nb, T, target, tid, nthreads, bad);
+          temporal_stages<B, 0>(J, Q, sre, sim, dc, nb, T, target, tid, nthreads, bad);
         } else {
           switch (T) {
-            case 2: temporal_stages<B, 2>(J, Q, spec, dc, nb, T, target, tid, nthreads, bad); break;
-            case 3: temporal_stages<B, 3>(J, Q, spec, dc, nb, T, target, tid, nthreads, bad); break;
-            case 4: temporal_stages<B, 4>(J, Q, spec, dc, nb, T, target, tid, nthreads, bad); break;
-            case 5: temporal_stages<B, 5>(J, Q, spec, dc, nb, T, target, tid, nthreads, bad); break;
-            case 6: temporal_stages<B, 6>(J, Q, spec, dc, nb, T, target, tid, nthreads, bad); break;
-            default: temporal_stages<B, 7>(J, Q, spec, dc, nb, T, target, tid, nthreads, bad); break;
+            case 2: temporal_stages<B, 2>(J, Q, sre, sim, dc, nb, T, target, tid, nthreads, bad); break;
+            case 3: temporal_stages<B, 3>(J, Q, sre, sim, dc, nb, T, target, tid, nthreads, bad); break;
+            case 4: temporal_stages<B, 4>(J, Q, sre, sim, dc, nb, T, target, tid, nthreads, bad); break;
+            case 5: temporal_stages<B, 5>(J, Q, sre, sim, dc, nb, T, target, tid, nthreads, bad); break;
+            case 6: temporal_stages<B, 6>(J, Q, sre, sim, dc, nb, T, target, tid, nthreads, bad); break;
+            default: temporal_stages<B, 7>(J, Q, sre, sim, dc, nb, T, target, tid, nthreads, bad); break;
           }
         }
         for (int o = 0; o < nout; ++o) {
@@ -271,17 +278,18 @@ __global__ __launch_bounds__(256) void dfttest_vol_kernel(const DftVolLaunch L) 
           for (int t = tid; t < nb * K; t += nthreads) {
             const int bi = t / K, k = t - bi * K;
             float re[B], im[B];
-            float* col = spec + (((bi * T + zs) * BINS2) + k) * 2;
+            float* cr = sre + (bi * T + zs) * BS + k;
+            float* ci = sim + (bi * T + zs) * BS + k;
 #pragma unroll
             for (int y = 0; y < B; ++y) {
-              re[y] = col[y * K * 2];
-              im[y] = col[y * K * 2 + 1];
+              re[y] = cr[y * K];
+              im[y] = ci[y * K];
             }
             cfft<B, false>(re, im);
 #pragma unroll
             for (int y = 0; y < B; ++y) {
-              col[y * K * 2] = re[y];
-              col[y * K * 2 + 1] = im[y];
+              cr[y * K] = re[y];
+              ci[y * K] = im[y];
             }
           }
           __syncthreads();
@@ -289,11 +297,12 @@ __global__ __launch_bounds__(256) void dfttest_vol_kernel(const DftVolLaunch L) 
           for (int t = tid; t < nb * B; t += nthreads) {
             const int bi = t / B, y = t - bi * B;
             float re[B], im[B];
-            const float* row = spec + (((bi * T + zs) * BINS2) + y * K) * 2;
+            const float* rr = sre + (bi * T + zs) * BS + y * K;
+            const float* ri = sim + (bi * T + zs) * BS + y * K;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-              re[k] = row[2 * k];
-              im[k] = row[2 * k + 1];
+              re[k] = rr[k];
+              im[k] = ri[k];
             }
 #pragma unroll
             for (int k = K; k < B; ++k) {
@@ -301,8 +310,8 @@ __global__ __launch_bounds__(256) void dfttest_vol_kernel(const DftVolLaunch L) 
               im[k] = 0.0f - im[B - k];
             }
             cfft<B, false>(re, im);
-            float* ob = out + (bi * B + y) * B;
-            const float* hsrow = hs + (o * B + y) * B;
+            float* ob = out + (bi * B + y) * OS;
+            const float* hsrow = hs + (o * B + y) * OS;
 #pragma unroll
             for (int x = 0; x < B; ++x) {
               float v = re[x] * fct;
@@ -327,7 +336,7 @@ __global__ __launch_bounds__(256) void dfttest_vol_kernel(const DftVolLaunch L) 
               const int b_first = max(cb, ceil_div_int(x - B + 1, S));
               const int b_last = min(cb + nb - 1, x / S);
               for (int bx = b_first; bx <= b_last; ++bx)
-                sum = sum + out[((bx - cb) * B + (y - oy)) * B + (x - bx * S)];
+                sum = sum + out[((bx - cb) * B + (y - oy)) * OS + (x - bx * S)];
               *cell = sum;
             }
           }
@@ -373,11 +382,11 @@ __global__ __launch_bounds__(256) void dfttest_vol_kernel(const DftVolLaunch L) 
 }
 
 size_t dfttest_vol_lds_bytes(int B, int T, int chunk, int S, int nout, int acc_rows, int tile_w) {
-  const int K = B / 2 + 1;
-  const size_t ww_c = (size_t)(chunk - 1) * S + B;
-  return sizeof(float) * ((size_t)chunk * T * B * K * 2 + (size_t)chunk * B * B + (size_t)T * B * ww_c +
-                          (size_t)nout * acc_rows * tile_w + 2 * (size_t)chunk + (size_t)T * B * B +
-                          (size_t)nout * B * B);
+  const size_t ww_c = (size_t)odd_stride((chunk - 1) * S + B);
+  const size_t OS = (size_t)B + 1;
+  return sizeof(float) * ((size_t)chunk * T * spec_block_stride(B) * 2 + (size_t)chunk * B * OS + (size_t)T * B * ww_c +
+                          (size_t)nout * acc_rows * tile_w + 2 * (size_t)chunk + (size_t)T * B * OS +
+                          (size_t)nout * B * OS);
 }
 
 hipError_t launch_dfttest_vol(int B, int max_depth, const DftVolLaunch& L, int total_tiles, size_t lds, hipStream_t stream) {

@@ -23,10 +23,12 @@ namespace nf {
 
 namespace {
 
-// spectral.cpp:162-232 for one bin; x[j] are the T raw spectra.
+// spectral.cpp:162-232 for one bin; x[j] are the T raw spectra. The depth
+// is a compile-time constant so the T x T products unroll over twiddles held
+// in scalar registers.
+template <int T>
 __device__ __forceinline__ void temporal_bin(const Fft3dJob& J, int k, const float* xr, const float* xi,
                                              float g_ratio, float& out_r, float& out_i, bool& bad) {
-  const int T = J.T;
   float Mr = 0.0f, Mi = 0.0f;
   if (J.grid) {
     const float2 gv = J.grid[k];
@@ -34,9 +36,11 @@ __device__ __forceinline__ void temporal_bin(const Fft3dJob& J, int k, const flo
     Mi = g_ratio * gv.y;
   }
   const float gTr = Mr * (float)T, gTi = Mi * (float)T;
-  float Fr[5], Fi[5];
+  float Fr[T], Fi[T];
+#pragma unroll
   for (int m = 0; m < T; ++m) {
     float sr = 0.0f, si = 0.0f;
+#pragma unroll
     for (int j = 0; j < T; ++j) {
       const float wr = J.ftw[2 * (m * 5 + j)], wi = J.ftw[2 * (m * 5 + j) + 1];
       const float pr = xr[j] * wr - xi[j] * wi;
@@ -51,6 +55,7 @@ __device__ __forceinline__ void temporal_bin(const Fft3dJob& J, int k, const flo
   Fi[0] = Fi[0] - gTi;
   const float noise = J.g.primary ? J.g.primary[k] * J.noise_mult : J.noise;
   bad |= !isfinite(noise);
+#pragma unroll
   for (int m = 0; m < T; ++m) {
     const float power = Fr[m] * Fr[m] + Fi[m] * Fi[m];
     bad |= !isfinite(power);
@@ -62,6 +67,7 @@ __device__ __forceinline__ void temporal_bin(const Fft3dJob& J, int k, const flo
   Fr[0] = Fr[0] + gTr;
   Fi[0] = Fi[0] + gTi;
   float yr = 0.0f, yi = 0.0f;
+#pragma unroll
   for (int m = 0; m < T; ++m) {
     const float wr = J.itw[2 * m], wi = J.itw[2 * m + 1];
     const float pr = Fr[m] * wr - Fi[m] * wi;
@@ -72,6 +78,33 @@ __device__ __forceinline__ void temporal_bin(const Fft3dJob& J, int k, const flo
   const float invT = 1.0f / (float)T;
   out_r = yr * invT;
   out_i = yi * invT;
+}
+
+// Temporal Wiener over every bin of the group's blocks into slot 0.
+template <int T, int BINS, int BS>
+__device__ __forceinline__ void temporal_pass(const Fft3dJob& J, float* sre, float* sim, float* dc, int nb, int NB,
+                                              int tid, int nthreads, bool& bad) {
+  const int c = J.c;
+  for (int t = tid; t < nb * BINS; t += nthreads) {
+    const int bi = t / BINS, k = t - bi * BINS;
+    float xr[T], xi[T];
+#pragma unroll
+    for (int j = 0; j < T; ++j) {
+      xr[j] = sre[(j * NB + bi) * BS + k];
+      xi[j] = sim[(j * NB + bi) * BS + k];
+    }
+    const float dcc = sre[(c * NB + bi) * BS];
+    const float g_ratio = (J.grid && J.degrid != 0.0f && J.grid[0].x != 0.0f) ? (J.degrid * dcc) / J.grid[0].x : 0.0f;
+    bad |= !isfinite(g_ratio);
+    float orr, oi;
+    temporal_bin<T>(J, k, xr, xi, g_ratio, orr, oi, bad);
+    sre[bi * BS + k] = orr;
+    sim[bi * BS + k] = oi;
+    if (k == 0) {
+      dc[2 * bi] = orr;
+      dc[2 * bi + 1] = oi;
+    }
+  }
 }
 
 // kalman.cpp:3-24 for one bin.
@@ -108,6 +141,8 @@ __global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
   extern __shared__ float smem[];
   constexpr int K = BW / 2 + 1;
   constexpr int BINS = BH * K;
+  constexpr int BS = spec_block_stride(BW);  // == (BH + 1) * K for square blocks
+  static_assert(BW == BH, "square blocks");
   int jid = 0;
   while (jid + 1 < L.njobs && (int)blockIdx.x >= L.group_start[jid + 1]) ++jid;
   const Fft3dJob& J = L.jobs[jid];
@@ -119,13 +154,14 @@ __global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
   const int Sx = J.gx.step, Sy = J.gy.step;
   const int oy = by * Sy, ox0 = bx0 * Sx;
   const int WW = (nb - 1) * Sx + BW;
-  const int ww_max = (NB - 1) * Sx + BW;
+  const int ww = odd_stride((NB - 1) * Sx + BW);
   const int tid = threadIdx.x, nthreads = blockDim.x;
   const int Tst = J.mode == 0 ? J.T : 1;  // spectra held in LDS
 
-  float* spec = smem;                          // [Tst][NB][BINS][2]
-  float* win = spec + Tst * NB * BINS * 2;     // [BH][ww_max]
-  float* dc = win + BH * ww_max;               // [NB][2]
+  float* sre = smem;                     // [Tst][NB][BS] real parts
+  float* sim = sre + Tst * NB * BS;      // [Tst][NB][BS] imaginary parts
+  float* win = sim + Tst * NB * BS;      // [BH][ww]
+  float* dc = win + BH * ww;             // [NB][2]
   bool bad = false;
 
   if (J.mode != 2) {
@@ -136,14 +172,15 @@ __global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
         const int sy = reflect(oy + r, J.gy), sx = reflect(ox0 + c, J.gx);
         const float v = load_sample(J.src[j], J.src_stride[j], J.storage, sy, sx) - J.base;
         bad |= !isfinite(v);
-        win[r * ww_max + c] = v;
+        win[r * ww + c] = v;
       }
       __syncthreads();
-      float* sj = spec + j * NB * BINS * 2;
+      float* jre = sre + j * NB * BS;
+      float* jim = sim + j * NB * BS;
       for (int t = tid; t < nb * BH; t += nthreads) {
         const int bi = t / BH, y = t - bi * BH;
         float re[BW], im[BW];
-        const float* src = win + y * ww_max + bi * Sx;
+        const float* src = win + y * ww + bi * Sx;
         const float wy = J.wya[y];
 #pragma unroll
         for (int x = 0; x < BW; ++x) {
@@ -151,28 +188,30 @@ __global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
           im[x] = 0.0f;
         }
         cfft<BW, true>(re, im);
-        float* dst = sj + (bi * BINS + y * K) * 2;
+        float* dr = jre + bi * BS + y * K;
+        float* di = jim + bi * BS + y * K;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-          dst[2 * k] = re[k];
-          dst[2 * k + 1] = (k == BW / 2) ? 0.0f : im[k];
+          dr[k] = re[k];
+          di[k] = (k == BW / 2) ? 0.0f : im[k];
         }
       }
       __syncthreads();
       for (int t = tid; t < nb * K; t += nthreads) {
         const int bi = t / K, k = t - bi * K;
         float re[BH], im[BH];
-        float* col = sj + (bi * BINS + k) * 2;
+        float* cr = jre + bi * BS + k;
+        float* ci = jim + bi * BS + k;
 #pragma unroll
         for (int y = 0; y < BH; ++y) {
-          re[y] = col[y * K * 2];
-          im[y] = col[y * K * 2 + 1];
+          re[y] = cr[y * K];
+          im[y] = ci[y * K];
         }
         cfft<BH, true>(re, im);
 #pragma unroll
         for (int y = 0; y < BH; ++y) {
-          col[y * K * 2] = re[y];
-          col[y * K * 2 + 1] = im[y];
+          cr[y * K] = re[y];
+          ci[y * K] = im[y];
         }
         if (k == 0 && j == 0) {
           dc[2 * bi] = re[0];
@@ -189,7 +228,7 @@ __global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
     for (int t = tid; t < nb * BINS; t += nthreads) {
       const int bi = t / BINS, k = t - bi * BINS;
       const long long s = (block_base + bi) * BINS + k;
-      kalman_bin(J, s, k, spec[(bi * BINS + k) * 2], spec[(bi * BINS + k) * 2 + 1], bad);
+      kalman_bin(J, s, k, sre[bi * BS + k], sim[bi * BS + k], bad);
     }
     if (bad) atomicOr(L.flags, NF_FLAG_NONFINITE);
     return;
@@ -200,8 +239,8 @@ __global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
     for (int t = tid; t < nb * BINS; t += nthreads) {
       const int bi = t / BINS, k = t - bi * BINS;
       const float2 l = J.k_last[(block_base + bi) * BINS + k];
-      spec[(bi * BINS + k) * 2] = l.x;
-      spec[(bi * BINS + k) * 2 + 1] = l.y;
+      sre[bi * BS + k] = l.x;
+      sim[bi * BS + k] = l.y;
       if (k == 0) {
         dc[2 * bi] = l.x;
         dc[2 * bi + 1] = l.y;
@@ -210,25 +249,11 @@ __global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
     __syncthreads();
   } else if (J.T > 1) {
     // temporal Wiener per bin into slot 0 (spectral.cpp:162-232)
-    const int c = J.c;
-    for (int t = tid; t < nb * BINS; t += nthreads) {
-      const int bi = t / BINS, k = t - bi * BINS;
-      float xr[5], xi[5];
-      for (int j = 0; j < J.T; ++j) {
-        xr[j] = spec[((j * NB + bi) * BINS + k) * 2];
-        xi[j] = spec[((j * NB + bi) * BINS + k) * 2 + 1];
-      }
-      const float dcc = spec[((c * NB + bi) * BINS) * 2];
-      const float g_ratio = (J.grid && J.degrid != 0.0f && J.grid[0].x != 0.0f) ? (J.degrid * dcc) / J.grid[0].x : 0.0f;
-      bad |= !isfinite(g_ratio);
-      float orr, oi;
-      temporal_bin(J, k, xr, xi, g_ratio, orr, oi, bad);
-      spec[(bi * BINS + k) * 2] = orr;
-      spec[(bi * BINS + k) * 2 + 1] = oi;
-      if (k == 0) {
-        dc[2 * bi] = orr;
-        dc[2 * bi + 1] = oi;
-      }
+    switch (J.T) {
+      case 2: temporal_pass<2, BINS, BS>(J, sre, sim, dc, nb, NB, tid, nthreads, bad); break;
+      case 3: temporal_pass<3, BINS, BS>(J, sre, sim, dc, nb, NB, tid, nthreads, bad); break;
+      case 4: temporal_pass<4, BINS, BS>(J, sre, sim, dc, nb, NB, tid, nthreads, bad); break;
+      default: temporal_pass<5, BINS, BS>(J, sre, sim, dc, nb, NB, tid, nthreads, bad); break;
     }
     __syncthreads();
   } else if (J.denoise) {
@@ -236,7 +261,8 @@ __global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
     for (int t = tid; t < nb * K; t += nthreads) {
       const int bi = t / K, k = t - bi * K;
       const float scale = J.grid ? (J.degrid * dc[2 * bi]) / J.grid[0].x : 0.0f;
-      float* col = spec + (bi * BINS + k) * 2;
+      float* cr = sre + bi * BS + k;
+      float* ci = sim + bi * BS + k;
       for (int y = 0; y < BH; ++y) {
         const int bin = y * K + k;
         float mr = 0.0f, mi = 0.0f;
@@ -244,17 +270,17 @@ __global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
           mr = scale * J.grid[bin].x;
           mi = scale * J.grid[bin].y;
         }
-        float re = col[y * K * 2], im = col[y * K * 2 + 1];
+        float re = cr[y * K], im = ci[y * K];
         apply_gain(J.g, bin, re, im, mr, mi, bad);
-        col[y * K * 2] = re;
-        col[y * K * 2 + 1] = im;
+        cr[y * K] = re;
+        ci[y * K] = im;
       }
     }
     __syncthreads();
     if (J.enh_separate) {
       for (int bi = tid; bi < nb; bi += nthreads) {
-        dc[2 * bi] = spec[(bi * BINS) * 2];
-        dc[2 * bi + 1] = spec[(bi * BINS) * 2 + 1];
+        dc[2 * bi] = sre[bi * BS];
+        dc[2 * bi + 1] = sim[bi * BS];
       }
       __syncthreads();
     }
@@ -266,11 +292,12 @@ __global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
   for (int t = tid; t < nb * K; t += nthreads) {
     const int bi = t / K, k = t - bi * K;
     float re[BH], im[BH];
-    float* col = spec + (bi * BINS + k) * 2;
+    float* cr = sre + bi * BS + k;
+    float* ci = sim + bi * BS + k;
 #pragma unroll
     for (int y = 0; y < BH; ++y) {
-      re[y] = col[y * K * 2];
-      im[y] = col[y * K * 2 + 1];
+      re[y] = cr[y * K];
+      im[y] = ci[y * K];
     }
     if (enhance) {
       const float scale = J.grid ? (J.degrid * dc[2 * bi]) / J.grid[0].x : 0.0f;
@@ -289,8 +316,8 @@ __global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
     cfft<BH, false>(re, im);
 #pragma unroll
     for (int y = 0; y < BH; ++y) {
-      col[y * K * 2] = re[y];
-      col[y * K * 2 + 1] = im[y];
+      cr[y * K] = re[y];
+      ci[y * K] = im[y];
     }
   }
   __syncthreads();
@@ -298,11 +325,12 @@ __global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
   for (int t = tid; t < nb * BH; t += nthreads) {
     const int bi = t / BH, y = t - bi * BH;
     float re[BW], im[BW];
-    const float* row = spec + (bi * BINS + y * K) * 2;
+    const float* rr = sre + bi * BS + y * K;
+    const float* ri = sim + bi * BS + y * K;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      re[k] = row[2 * k];
-      im[k] = row[2 * k + 1];
+      re[k] = rr[k];
+      im[k] = ri[k];
     }
 #pragma unroll
     for (int k = K; k < BW; ++k) {
@@ -310,12 +338,17 @@ __global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
       im[k] = 0.0f - im[BW - k];
     }
     cfft<BW, false>(re, im);
-    float* dst = J.contrib + ((block_base + bi) * BH + y) * BW;
+    float4* dst = reinterpret_cast<float4*>(J.contrib + ((block_base + bi) * BH + y) * BW);
 #pragma unroll
-    for (int x = 0; x < BW; ++x) {
-      const float v = re[x] * fct;
-      bad |= !isfinite(v);
-      dst[x] = v * J.wxs[x];
+    for (int x = 0; x < BW; x += 4) {
+      float o[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float v = re[x + e] * fct;
+        bad |= !isfinite(v);
+        o[e] = v * J.wxs[x + e];
+      }
+      dst[x / 4] = make_float4(o[0], o[1], o[2], o[3]);
     }
   }
   if (bad) atomicOr(L.flags, NF_FLAG_NONFINITE);
@@ -359,9 +392,8 @@ __global__ __launch_bounds__(256) void fft3d_synthesis_kernel(const Fft3dLaunch 
 }
 
 size_t fft3d_block_lds_bytes(int BW, int BH, int T, int NB, int Sx) {
-  const int K = BW / 2 + 1;
-  const int ww_max = (NB - 1) * Sx + BW;
-  return sizeof(float) * ((size_t)T * NB * BH * K * 2 + (size_t)BH * ww_max + 2 * (size_t)NB);
+  const int ww = odd_stride((NB - 1) * Sx + BW);
+  return sizeof(float) * ((size_t)T * NB * spec_block_stride(BW) * 2 + (size_t)BH * ww + 2 * (size_t)NB);
 }
 
 hipError_t launch_fft3d_block(int BW, int BH, const Fft3dLaunch& L, int total_groups, size_t lds, hipStream_t stream) {
