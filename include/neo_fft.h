@@ -172,6 +172,31 @@ nf_status nf_dfttest_wiener_backward(const nf_plan* plan, nf_plane source, nf_pl
                                      nf_mut_plane grad_source, const float* table, float* grad_table,
                                      unsigned* device_flags, void* stream);
 
+/* Sampled noise models. A DFTTest plan with nlocation (and ftype < 2) and an
+ * FFT3D plan with pfactor > 0 (and bt != -1, pshow off) filter with a model
+ * estimated from source frames; processing before the model is published
+ * fails with NF_EINVAL ("... sample model not initialized", reference
+ * plan.cpp:378-389). needs_model / model_ready report that state. */
+nf_status nf_plan_sampling(const nf_plan* plan, int* needs_model, int* model_ready);
+/* The plan's nlocation quadruples (frame, plane, y, x); out may be NULL. */
+nf_status nf_dfttest_noise_locations(const nf_plan* plan, int* out, int cap, int* count);
+/* DFTNoise::prepare (src/algorithms/dft_noise.cpp:25-41) with the filter's
+ * gather (src/plugin/filter.hpp:381-389): samples[l*tbsize + z] is the whole
+ * device plane (sample format of the plan) of frame location[l].frame + z,
+ * plane location[l].plane; the S x S rectangle at (y, x) must lie inside it.
+ * Publishes the calibrated power table into the plan; a published model is
+ * kept (later calls return NF_OK at once). Synchronous on `stream`. */
+nf_status nf_dfttest_prepare_noise(nf_plan* plan, const nf_plane* samples, int n_samples, void* stream);
+/* Plan::prepare_pattern / pattern_candidate (plan.cpp:241-284): searches the
+ * pattern block on `source` (the pframe plane, ROI applied by the caller),
+ * publishes the scaled model once, and reports the selected block (bx, by)
+ * into selected_xy when non-NULL. Synchronous on `stream`. */
+nf_status nf_fft3d_prepare_pattern(nf_plan* plan, nf_plane source, int* selected_xy, void* stream);
+/* Copies the published model (DFTTest: tbsize*S*(S/2+1) floats; FFT3D:
+ * bh*(bw/2+1)) to host memory; count receives its length (reference
+ * Plan::pattern_power / DFTNoise::power, plan.hpp:92-93, dft_noise.hpp:18). */
+nf_status nf_plan_read_model(const nf_plan* plan, float* host, int cap, int* count, void* stream);
+
 /* FFT3D Kalman mode (bt=0): device state of one plane (reference
  * KalmanState, src/kernels/kalman.hpp:5-14: complex L + float C + float Q,
  * 16 bytes per bin). create = Plan::initial_kalman (plan.cpp:293-300). */

@@ -103,6 +103,18 @@ struct DeviceTables {
   }
 };
 
+// Owned mutable device buffer (sampled noise models).
+struct DeviceBuffer {
+  float* ptr = nullptr;
+  void alloc(size_t floats) {
+    hip_check(hipMalloc((void**)&ptr, sizeof(float) * floats), "hipMalloc(model)");
+    hip_check(hipMemset(ptr, 0, sizeof(float) * floats), "hipMemset(model)");
+  }
+  ~DeviceBuffer() {
+    if (ptr) (void)hipFree(ptr);
+  }
+};
+
 }  // namespace
 
 struct nf_plan {
@@ -134,6 +146,18 @@ struct nf_plan {
   const float *d_wxa = nullptr, *d_wxs = nullptr, *d_wya = nullptr, *d_wys = nullptr;
   const float *d_ftw = nullptr, *d_itw = nullptr, *d_primary = nullptr;
   int NB = 4, groups_x = 0;
+  // sampled noise models: DFTTest nlocation (DFTNoise) and FFT3D pfactor /
+  // pshow; the model is published into `model` by the prepare entry points
+  int sampled = 0, preview = 0, tabled = 0;
+  bool model_ready = false;
+  DeviceBuffer model;
+  std::vector<int> locations;  // DFTTest (frame, plane, y, x) quadruples
+  float calibration = 0;
+  const float* d_sample_h = nullptr;
+  const float2* d_sample_grid = nullptr;
+  int px = 0, py = 0;
+  float pfactor = 0;
+  const float* d_weights = nullptr;
 };
 
 struct nf_kalman_state {
@@ -317,8 +341,8 @@ extern "C" nf_status nf_dfttest_plan_create(int width, int height, nf_sample_for
     if (c.smode != 1) throw Unsupported("DFTTest smode=0 is not available on this device path yet");
     if (!(c.sbsize == 8 || c.sbsize == 16 || c.sbsize == 32))
       throw Unsupported("DFTTest sbsize must be 8, 16 or 32 on this device path");
-    if (c.n_nlocation) throw Unsupported("DFTTest nlocation noise sampling is not available on this device path yet");
     const int T = c.tbsize, B = c.sbsize;
+    p->sampled = c.ftype < 2 && c.n_nlocation > 0;  // plan.cpp:194
     // plan.cpp:188-226
     const auto win = dft_window_3d(T, B, c.smode == 0 ? 0 : c.sosize, c.smode, c.swin, c.twin, c.sbeta, c.tbeta,
                                    c.tmode, c.tmode ? c.tosize : 0);
@@ -328,14 +352,34 @@ extern "C" nf_status nf_dfttest_plan_create(int width, int height, nf_sample_for
     const float scale = c.ftype < 2 ? win.wscale : 1.0f;
     GainParams g{};
     g.type = c.ftype;
-    g.a = curves.empty() ? finite(c.sigma / scale) : 0.0f;
+    g.a = curves.empty() && !p->sampled ? finite(c.sigma / scale) : 0.0f;
     g.b = finite(c.sigma2 / scale);
     g.low = finite(c.pmin / win.wscale);
     g.high = finite(c.pmax / win.wscale);
     g.exponent = c.f0beta;
     g.floor_ = 0;
     std::vector<float> primary;
-    if (!curves.empty()) primary = dft_profile(curves, T, B, c.sigma, scale);
+    if (!curves.empty() && !p->sampled) primary = dft_profile(curves, T, B, c.sigma, scale);
+    // DFTNoise (dft_noise.cpp:5-24): sample window without overlap, its
+    // template spectrum and the calibration (1/n) * (wscale_s / wscale_o) * alpha
+    std::vector<float> sample_h;
+    std::vector<std::complex<float>> sample_grid;
+    if (p->sampled) {
+      for (int i = 0; i < c.n_nlocation; ++i) p->locations.push_back(c.nlocation[i]);
+      const auto sw = dft_window_3d(T, B, 0, 0, c.swin, c.twin, c.sbeta, c.tbeta, 0, 0);
+      const float alpha = c.alpha < 0 ? (c.ftype == 0 ? 5.0f : 7.0f) : c.alpha;
+      const int nloc = c.n_nlocation / 4;
+      p->calibration = finite(finite((1.0f / float(nloc)) * finite(sw.wscale / win.wscale)) * alpha);
+      sample_h = sw.h;
+      std::vector<float> block(sw.h.size());
+      for (size_t i = 0; i < block.size(); ++i) block[i] = 255.0f * sw.h[i];
+      sample_grid = T > 1 ? rfft3_host(T, B, B, block.data()) : rfft2_host(B, B, block.data());
+      for (const auto& v : sample_grid) {
+        finite(v.real());
+        finite(v.imag());
+      }
+      require(!c.zmean || sample_grid[0].real() != 0, "DFTTest sample template DC must be nonzero");
+    }
     std::vector<std::complex<float>> grid;
     if (c.zmean) {
       std::vector<float> block(win.h.size());
@@ -347,11 +391,18 @@ extern "C" nf_status nf_dfttest_plan_create(int width, int height, nf_sample_for
     const size_t o_h = p->tables.add(win.h), o_hs = p->tables.add(hsyn);
     const size_t o_pr = primary.empty() ? 0 : p->tables.add(primary);
     const size_t o_g = grid.empty() ? 0 : p->tables.add(grid);
+    const size_t o_sh = sample_h.empty() ? 0 : p->tables.add(sample_h);
+    const size_t o_sg = sample_grid.empty() ? 0 : p->tables.add(sample_grid);
     p->tables.upload();
+    if (p->sampled) {
+      p->d_sample_h = p->tables.at<float>(o_sh);
+      p->d_sample_grid = c.zmean ? p->tables.at<float2>(o_sg) : nullptr;
+      p->model.alloc((size_t)T * B * (B / 2 + 1));
+    }
     p->d_ttw = p->tables.at<float>(o_ttw);
     p->d_h = p->tables.at<float>(o_h);
     p->d_hsyn = p->tables.at<float>(o_hs);
-    g.primary = primary.empty() ? nullptr : p->tables.at<float>(o_pr);
+    g.primary = p->sampled ? p->model.ptr : primary.empty() ? nullptr : p->tables.at<float>(o_pr);
     p->d_grid = grid.empty() ? nullptr : p->tables.at<float2>(o_g);
     p->g = g;
     p->mean_scale = c.zmean ? 1.0f : 0.0f;
@@ -399,21 +450,32 @@ extern "C" nf_status nf_fft3d_plan_create(int width, int height, nf_sample_forma
     p->bh = c.bh;
     p->bt = c.bt;
     p->temporal = std::max(1, c.bt);
-    if (c.pfactor > 0) throw Unsupported("FFT3D pattern-sampled noise (pfactor) is not available on this device path yet");
     if (!(c.bw == c.bh && (c.bw == 8 || c.bw == 16 || c.bw == 32)))
       throw Unsupported("FFT3D bw=bh in {8,16,32} required on this device path");
     const auto wx = fft3d_window(c.bw, p->gx.overlap, c.wintype);
     const auto wy = fft3d_window(c.bh, p->gy.overlap, c.wintype);
     // plan.cpp:101-169
     const float factor = fmt.floating ? 1.0f / 255 : float(1 << (fmt.bits - 8));
-    p->sigma_eff = finite(c.sigma * factor);
+    p->sampled = c.pfactor > 0;
+    p->sigma_eff = p->sampled ? 0.0f : finite(c.sigma * factor);
     p->norm = 1.0f / (float(c.bw) * float(c.bh));
-    const float sig[4] = {p->sigma_eff, finite(s2 * factor), finite(s3 * factor), finite(s4 * factor)};
+    float sig[4] = {0, 0, 0, 0};
+    if (!p->sampled) {
+      sig[0] = p->sigma_eff;
+      sig[1] = finite(s2 * factor);
+      sig[2] = finite(s3 * factor);
+      sig[3] = finite(s4 * factor);
+    }
     p->varying = sig[1] != sig[0] || sig[2] != sig[0] || sig[3] != sig[0];
-    if (c.pshow && p->varying) throw Unsupported("FFT3D pshow preview is not available on this device path yet");
-    p->denoise = c.bt != -1;
-    p->kalman = c.bt == 0;
+    p->preview = c.pshow && (p->sampled || p->varying);  // plan.cpp:113-115
+    p->denoise = c.bt != -1 && !p->preview;
+    p->kalman = c.bt == 0 && !p->preview;
+    p->tabled = p->varying || (p->sampled && p->denoise);
+    p->px = c.px;
+    p->py = c.py;
+    p->pfactor = c.pfactor;
     if (p->kalman) {
+      p->sigma_eff = finite(c.sigma * factor);
       p->r0 = finite((p->sigma_eff * p->sigma_eff) / p->norm);
       p->ratio2 = finite(c.kratio * c.kratio);
     }
@@ -422,7 +484,7 @@ extern "C" nf_status nf_fft3d_plan_create(int width, int height, nf_sample_forma
     g.floor_ = (c.beta - 1) / c.beta;
     g.exponent = 1;
     std::vector<float> primary;
-    if (p->denoise) {
+    if (p->denoise && !p->sampled) {
       if (p->varying) {
         primary = fft3d_profile(c.bw, c.bh, sig);
         for (float r : primary) finite(float(p->temporal) * r);
@@ -431,13 +493,34 @@ extern "C" nf_status nf_fft3d_plan_create(int width, int height, nf_sample_forma
         for (int T = 1; T <= p->temporal; ++T) finite(((float(T) * p->sigma_eff) * p->sigma_eff) / p->norm);
       }
     }
-    if (p->kalman) {
+    if (p->kalman && !p->sampled) {
       if (primary.empty()) finite(p->r0 * p->ratio2);
       else for (float r : primary) finite(std::max(r, 1e-15f) * p->ratio2);
     }
     const auto et = enhancement_tables(c.bw, c.bh, factor, c.sharpen, c.scutoff, c.svr, c.smin, c.smax, c.dehalo,
                                        c.hr, c.ht);
-    p->enh_active = c.sharpen != 0 || c.dehalo != 0;
+    p->enh_active = !p->preview && (c.sharpen != 0 || c.dehalo != 0);
+    // pattern sampling weights (plan.cpp:142-160)
+    std::vector<float> weights;
+    if (p->preview || (p->sampled && p->denoise)) {
+      if (c.px == 0 && c.py == 0)
+        require(p->gx.count >= 5 && p->gy.count >= 5, "FFT3D automatic pattern search requires a 5x5 grid");
+      else
+        require(c.px < p->gx.count && c.py < p->gy.count, "FFT3D pattern coordinates outside grid");
+      const float cutoff2 = finite(c.pcutoff * c.pcutoff);
+      require(cutoff2 > 0, "FFT3D pattern cutoff underflow");
+      const int K = c.bw / 2 + 1;
+      weights.resize((size_t)c.bh * K);
+      for (int y = 0; y < c.bh; ++y) {
+        const float fy = 2 * float(std::min(y, c.bh - y)) / float(c.bh);
+        const float other = fy * fy;
+        for (int x = 0; x < K; ++x) {
+          const float fx = 2 * float(x) / float(c.bw);
+          const float q = finite(other + finite(fx * fx));
+          weights[(size_t)y * K + x] = finite(q / finite(q + cutoff2));
+        }
+      }
+    }
     std::vector<std::complex<float>> grid;
     if (c.degrid != 0) {
       std::vector<float> block((size_t)c.bw * c.bh);
@@ -472,7 +555,12 @@ extern "C" nf_status nf_fft3d_plan_create(int width, int height, nf_sample_forma
     const size_t o_g = grid.empty() ? 0 : p->tables.add(grid);
     const size_t o_sh = et.sharpen.empty() ? 0 : p->tables.add(et.sharpen);
     const size_t o_ha = et.halo.empty() ? 0 : p->tables.add(et.halo);
+    const size_t o_w = weights.empty() ? 0 : p->tables.add(weights);
     p->tables.upload();
+    if (!weights.empty()) {
+      p->d_weights = p->tables.at<float>(o_w);
+      p->model.alloc(weights.size());
+    }
     p->d_wxa = p->tables.at<float>(o_wxa);
     p->d_wxs = p->tables.at<float>(o_wxs);
     p->d_wya = p->tables.at<float>(o_wya);
@@ -492,8 +580,8 @@ extern "C" nf_status nf_fft3d_plan_create(int width, int height, nf_sample_forma
     e.sharpen_tab = et.sharpen.empty() ? nullptr : p->tables.at<float>(o_sh);
     e.halo_tab = et.halo.empty() ? nullptr : p->tables.at<float>(o_ha);
     p->enh = e;
-    g.primary = p->d_primary;
-    if (!p->varying && p->denoise) {  // plan.cpp:141: enhancement fused with uniform Wiener
+    g.primary = p->sampled ? p->model.ptr : p->d_primary;
+    if (!p->tabled && p->denoise) {  // plan.cpp:141: enhancement fused with uniform Wiener
       g.sharpen = e.sharpen;
       g.dehalo = e.dehalo;
       g.ea = e.ea;
@@ -540,7 +628,15 @@ void check_plane(const nf_plan& p, int w, int h, long long stride, const void* d
           "invalid plane stride");
 }
 
+// Run-time guard of the sampled models (plan.cpp:378-389).
+void require_model(const nf_plan& p) {
+  if (p.algorithm == 0) require(!p.sampled || p.model_ready, "DFTTest sample model not initialized");
+  else if (p.sampled && p.denoise)
+    require(p.model_ready, p.kalman ? "Kalman sampled model not initialized" : "FFT3D sampled model not initialized");
+}
+
 DftJob make_dft_job(const nf_plan& p, const nf_plane& src, const nf_mut_plane& dst) {
+  require_model(p);
   DftJob j{};
   j.gx = p.gx;
   j.gy = p.gy;
@@ -582,11 +678,12 @@ Fft3dJob make_fft3d_job(const nf_plan& p, int T, int mode) {
   j.peak = p.peak;
   j.degrid = p.degrid;
   j.lower = p.lower;
-  j.noise = (p.denoise && !p.varying) ? ((float(T) * p.sigma_eff) * p.sigma_eff) / p.norm : 0.0f;
+  require_model(p);
+  j.noise = (p.denoise && !p.tabled) ? ((float(T) * p.sigma_eff) * p.sigma_eff) / p.norm : 0.0f;
   j.noise_mult = float(T);
   j.denoise = p.denoise;
-  j.varying = p.varying;
-  j.enh_separate = p.varying && p.enh_active;
+  j.varying = p.tabled;
+  j.enh_separate = p.tabled && p.enh_active;
   j.g = p.g;
   j.enh = p.enh;
   j.wxa = p.d_wxa;
@@ -596,7 +693,7 @@ Fft3dJob make_fft3d_job(const nf_plan& p, int T, int mode) {
   j.grid = p.d_grid;
   j.ftw = p.d_ftw;
   j.itw = p.d_itw;
-  j.k_pattern = p.varying ? p.d_primary : nullptr;
+  j.k_pattern = p.tabled ? p.g.primary : nullptr;
   j.k_r0 = p.r0;
   j.k_ratio2 = p.ratio2;
   j.blocks_per_group = p.NB;
@@ -841,6 +938,67 @@ void enqueue_fft3d_jobs(const std::vector<const nf_plan*>& plans, std::vector<Ff
   if (contrib) hip_check(hipFreeAsync(contrib, stream), "hipFreeAsync(contrib)");
 }
 
+// ------------------------------------------------------------ pattern search
+PatternArgs pattern_args(const nf_plan& p, const nf_plane& src, unsigned* flags) {
+  PatternArgs A{};
+  A.gx = p.gx;
+  A.gy = p.gy;
+  A.storage = p.storage;
+  A.base = p.base;
+  A.src = src.data;
+  A.stride = src.stride_bytes;
+  A.wxa = p.d_wxa;
+  A.wya = p.d_wya;
+  A.grid = p.d_grid;
+  A.degrid = p.degrid;
+  A.weights = p.d_weights;
+  const bool automatic = p.px == 0 && p.py == 0;  // plan.cpp:248-250
+  const int left = automatic ? 2 : p.px, right = automatic ? p.gx.count - 3 : p.px;
+  const int top = automatic ? 2 : p.py, bottom = automatic ? p.gy.count - 3 : p.py;
+  A.left = left;
+  A.top = top;
+  A.ncx = right - left + 1;
+  A.ncy = bottom - top + 1;
+  require(A.ncx > 0 && A.ncy > 0, "FFT3D no pattern candidate");
+  A.flags = flags;
+  return A;
+}
+
+// Scratch of one search: scores, the selected index and a preview block.
+struct PatternScratch {
+  char* mem = nullptr;
+  hipStream_t stream = nullptr;
+  void take(PatternArgs& A, int bw, int bh, hipStream_t s) {
+    stream = s;
+    const size_t n = (size_t)A.ncx * A.ncy;
+    const size_t bytes = 256 + ((n * sizeof(float) + 255) & ~size_t(255)) + sizeof(float) * (size_t)bw * bh;
+    hip_check(hipMallocAsync((void**)&mem, bytes, s), "hipMallocAsync(pattern)");
+    A.selected = reinterpret_cast<int*>(mem);
+    A.scores = reinterpret_cast<float*>(mem + 256);
+    A.inv = reinterpret_cast<float*>(mem + 256 + ((n * sizeof(float) + 255) & ~size_t(255)));
+  }
+  ~PatternScratch() {
+    if (mem) (void)hipFreeAsync(mem, stream);
+  }
+};
+
+void enqueue_pattern_select(const nf_plan& p, const PatternArgs& A, hipStream_t stream) {
+  hip_check(launch_fft3d_pattern(p.bw, A, 0, stream), "fft3d pattern score launch");
+  hip_check(launch_pattern_argmin(A.scores, A.ncx * A.ncy, A.selected, stream), "pattern argmin launch");
+}
+
+// pshow preview of one plane (plan.cpp:409-419).
+void enqueue_fft3d_preview(const nf_plan& p, const nf_plane& src, const nf_mut_plane& dst, unsigned* flags,
+                           hipStream_t stream) {
+  PatternArgs A = pattern_args(p, src, flags);
+  PatternScratch scratch;
+  scratch.take(A, p.bw, p.bh, stream);
+  enqueue_pattern_select(p, A, stream);
+  hip_check(launch_fft3d_pattern(p.bw, A, 2, stream), "fft3d preview launch");
+  hip_check(launch_fft3d_preview_store(A, dst.data, dst.stride_bytes, p.peak, p.bw, p.bh, stream),
+            "fft3d preview store launch");
+}
+
 nf_status enqueue_one(const nf_plan* plan, const nf_plane* sources, int n_sources, const int* targets, int n_targets,
                       nf_mut_plane dst, int frame, int plane, unsigned* flags, hipStream_t stream) {
   require(plan && sources && n_sources > 0, "sources must not be empty");
@@ -881,6 +1039,10 @@ nf_status enqueue_one(const nf_plan* plan, const nf_plane* sources, int n_source
     require(!plan->kalman, "Kalman plans use nf_kalman_advance / nf_kalman_render");
     require(n_sources == plan->temporal || n_sources == 1,
             "FFT3D sources size must match plan temporal_size or single-frame fallback");
+    if (plan->preview) {
+      enqueue_fft3d_preview(*plan, sources[0], dst, flags, stream);
+      return NF_OK;
+    }
     Fft3dJob j = make_fft3d_job(*plan, n_sources, 0);
     for (int s = 0; s < n_sources; ++s) {
       j.src[s] = sources[s].data;
@@ -942,6 +1104,11 @@ extern "C" nf_status nf_process_batch(const nf_plan* const* plans, int n_jobs, c
       } else {
         require(!p->kalman, "Kalman plans use nf_kalman_advance / nf_kalman_render");
         require(ns == p->temporal || ns == 1, "FFT3D sources size must match plan temporal_size or single-frame fallback");
+        if (p->preview) {
+          pv.pop_back();
+          enqueue_fft3d_preview(*p, s[0], dsts[i], device_flags, (hipStream_t)stream);
+          continue;
+        }
         Fft3dJob j = make_fft3d_job(*p, ns, 0);
         for (int k = 0; k < ns; ++k) {
           j.src[k] = s[k].data;
@@ -957,7 +1124,7 @@ extern "C" nf_status nf_process_batch(const nf_plan* const* plans, int n_jobs, c
       for (auto* p : pv) require((p->temporal > 1) == temporal, "batched DFTTest jobs must share tbsize");
       if (temporal) enqueue_dft_vol_jobs(pv, dj, tq, device_flags, (hipStream_t)stream);
       else enqueue_dft_jobs(pv, dj, device_flags, (hipStream_t)stream);
-    } else {
+    } else if (!fj.empty()) {
       enqueue_fft3d_jobs(pv, fj, device_flags, (hipStream_t)stream, true);
     }
     return NF_OK;
@@ -1061,6 +1228,135 @@ extern "C" nf_status nf_dfttest_wiener_backward(const nf_plan* plan, nf_plane so
 }
 
 // ---------------------------------------------------------------- Kalman
+// ------------------------------------------------------ sampled noise models
+extern "C" nf_status nf_plan_sampling(const nf_plan* plan, int* needs_model, int* model_ready) {
+  if (!plan || !needs_model || !model_ready) return set_error(NF_EINVAL, "null argument");
+  *needs_model = plan->algorithm == 0 ? plan->sampled : (plan->sampled && plan->denoise);
+  *model_ready = plan->model_ready ? 1 : 0;
+  return NF_OK;
+}
+
+extern "C" nf_status nf_dfttest_noise_locations(const nf_plan* plan, int* out, int cap, int* count) {
+  if (!plan || !count || plan->algorithm != 0) return set_error(NF_EINVAL, "not a DFTTest plan");
+  *count = (int)plan->locations.size() / 4;
+  if (out)
+    for (int i = 0; i < (int)plan->locations.size() && i < 4 * cap; ++i) out[i] = plan->locations[i];
+  return NF_OK;
+}
+
+extern "C" nf_status nf_dfttest_prepare_noise(nf_plan* plan, const nf_plane* samples, int n_samples, void* stream) {
+  return guarded([&]() -> nf_status {
+    require(plan && plan->algorithm == 0, "not a DFTTest plan");
+    require(plan->sampled, "inactive DFTTest sample model");
+    if (plan->model_ready) return NF_OK;  // first publication wins (dft_noise.cpp:26)
+    const int T = plan->temporal, B = plan->B;
+    const int nloc = (int)plan->locations.size() / 4;
+    require(samples && n_samples == nloc * T, "one sample plane per location and temporal slot required");
+    const size_t bps = bytes_per_sample(plan->storage);
+    std::vector<const void*> rects(n_samples);
+    std::vector<long long> strides(n_samples);
+    for (int l = 0; l < nloc; ++l) {
+      const int y = plan->locations[4 * l + 2], x = plan->locations[4 * l + 3];
+      for (int z = 0; z < T; ++z) {
+        const nf_plane& s = samples[l * T + z];
+        require(s.data != nullptr, "null plane");
+        require(s.stride_bytes != 0 && std::llabs(s.stride_bytes) >= (long long)s.width * (long long)bps,
+                "invalid plane stride");
+        require(x <= s.width - B && y <= s.height - B, "sample frame rectangle differs from plan");
+        rects[l * T + z] = static_cast<const char*>(s.data) + (long long)y * s.stride_bytes + (long long)x * bps;
+        strides[l * T + z] = s.stride_bytes;
+      }
+    }
+    const hipStream_t st = (hipStream_t)stream;
+    const size_t bins = (size_t)T * B * (B / 2 + 1);
+    const size_t ptr_bytes = ((sizeof(void*) * n_samples + 255) & ~size_t(255));
+    const size_t str_bytes = ((sizeof(long long) * n_samples + 255) & ~size_t(255));
+    char* mem = nullptr;
+    hip_check(hipMallocAsync((void**)&mem, ptr_bytes + str_bytes + sizeof(float) * bins * nloc, st),
+              "hipMallocAsync(noise)");
+    DftNoiseArgs A{};
+    A.rects = reinterpret_cast<const void* const*>(mem);
+    A.strides = reinterpret_cast<const long long*>(mem + ptr_bytes);
+    A.power = reinterpret_cast<float*>(mem + ptr_bytes + str_bytes);
+    A.T = T;
+    A.storage = plan->storage;
+    A.dscale = plan->dscale;
+    A.h = plan->d_sample_h;
+    A.grid = plan->d_sample_grid;
+    A.ttw = plan->d_ttw;
+    nf_status st_out = NF_OK;
+    try {
+      hip_check(hipMemcpyAsync(mem, rects.data(), sizeof(void*) * n_samples, hipMemcpyHostToDevice, st),
+                "hipMemcpyAsync(noise rects)");
+      hip_check(hipMemcpyAsync(mem + ptr_bytes, strides.data(), sizeof(long long) * n_samples, hipMemcpyHostToDevice,
+                               st),
+                "hipMemcpyAsync(noise strides)");
+      st_out = run_sync(st, [&](unsigned* flag) {
+        A.flags = flag;
+        hip_check(launch_dft_noise(B, A, nloc, plan->calibration, plan->model.ptr, st), "dft noise launch");
+      });
+    } catch (...) {
+      (void)hipFreeAsync(mem, st);
+      throw;
+    }
+    hip_check(hipFreeAsync(mem, st), "hipFreeAsync(noise)");
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (st_out == NF_OK) plan->model_ready = true;
+    return st_out;
+  });
+}
+
+extern "C" nf_status nf_fft3d_prepare_pattern(nf_plan* plan, nf_plane source, int* selected_xy, void* stream) {
+  return guarded([&]() -> nf_status {
+    require(plan && plan->algorithm == 1, "not an FFT3D plan");
+    require(plan->sampled && plan->denoise, "FFT3D sampling is inactive");
+    check_plane(*plan, source.width, source.height, source.stride_bytes, source.data);
+    if (plan->model_ready && !selected_xy) return NF_OK;  // first publication wins (plan.cpp:281)
+    const hipStream_t st = (hipStream_t)stream;
+    int sel = -1;
+    PatternArgs A{};
+    const nf_status r = run_sync(st, [&](unsigned* flag) {
+      A = pattern_args(*plan, source, flag);
+      PatternScratch scratch;
+      scratch.take(A, plan->bw, plan->bh, st);
+      A.factor = plan->kalman ? 1.0f : plan->pfactor;
+      A.maximum = float(plan->temporal);
+      A.ratio2 = plan->ratio2;
+      A.kalman = plan->kalman;
+      // a repeated call recomputes the selection but keeps the published model
+      A.model = plan->model_ready ? nullptr : plan->model.ptr;
+      enqueue_pattern_select(*plan, A, st);
+      if (A.model) hip_check(launch_fft3d_pattern(plan->bw, A, 1, st), "fft3d pattern model launch");
+      hip_check(hipMemcpyAsync(&sel, A.selected, sizeof(int), hipMemcpyDeviceToHost, st), "hipMemcpyAsync(sel)");
+    });
+    if (r != NF_OK) return r;
+    require(sel >= 0, "FFT3D no pattern candidate");
+    if (selected_xy) {
+      selected_xy[0] = A.left + sel % A.ncx;
+      selected_xy[1] = A.top + sel / A.ncx;
+    }
+    plan->model_ready = true;
+    return NF_OK;
+  });
+}
+
+extern "C" nf_status nf_plan_read_model(const nf_plan* plan, float* host, int cap, int* count, void* stream) {
+  return guarded([&]() -> nf_status {
+    require(plan && count, "null argument");
+    const int n = plan->algorithm == 0 ? plan->temporal * plan->B * (plan->B / 2 + 1) : plan->bh * (plan->bw / 2 + 1);
+    *count = plan->model.ptr ? n : 0;
+    require(plan->model_ready, plan->algorithm == 0 ? "DFTTest sample model not initialized"
+                                                    : "FFT3D sampled model not initialized");
+    if (host && plan->model.ptr) {
+      hip_check(hipMemcpyAsync(host, plan->model.ptr, sizeof(float) * std::min(n, cap), hipMemcpyDeviceToHost,
+                               (hipStream_t)stream),
+                "hipMemcpyAsync(model)");
+      hip_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+    }
+    return NF_OK;
+  });
+}
+
 extern "C" nf_status nf_kalman_state_create(const nf_plan* plan, nf_kalman_state** out) {
   return guarded([&]() -> nf_status {
     require(plan && out, "null argument");

@@ -160,6 +160,41 @@ struct Fft3dLaunch {
   unsigned* flags;
 };
 
+// DFTTest sampled noise model (DFTNoise::prepare).
+struct DftNoiseArgs {
+  const void* const* rects;  // [nloc*T] top-left sample of each rectangle
+  const long long* strides;  // [nloc*T] row strides in bytes
+  int T, storage;
+  float dscale;
+  const float* h;            // sample window [T][B][B]
+  const float2* grid;        // sample template spectrum [T][B][B/2+1] (zmean) or nullptr
+  const float* ttw;          // temporal twiddles for depths outside 3..5
+  float* power;              // [nloc][T*B*(B/2+1)] per-location mean-removed power
+  unsigned* flags;
+};
+
+// FFT3D pattern search / model / preview.
+struct PatternArgs {
+  Axis gx, gy;
+  int storage;
+  float base;
+  const void* src;
+  long long stride;
+  const float* wxa;
+  const float* wya;
+  const float2* grid;   // degrid template or nullptr
+  float degrid;
+  const float* weights; // sampling weights per bin
+  int left, top, ncx, ncy;
+  float* scores;        // [ncx*ncy]
+  int* selected;        // device word: winning candidate index
+  float* model;         // published model (bins)
+  float factor, maximum, ratio2;
+  int kalman;
+  float* inv;           // preview block [bh][bw]
+  unsigned* flags;
+};
+
 }  // namespace nf
 
 // Kernel arguments travel in the 4 KiB kernarg segment.

@@ -25,4 +25,12 @@ size_t fft3d_block_lds_bytes(int BW, int BH, int T, int NB, int Sx);
 hipError_t launch_fft3d_block(int BW, int BH, const Fft3dLaunch& L, int total_groups, size_t lds, hipStream_t stream);
 hipError_t launch_fft3d_synthesis(const Fft3dLaunch& L, hipStream_t stream);
 hipError_t fft3d_set_lds_limit();
+
+size_t dft_noise_lds_bytes(int B, int T);
+hipError_t launch_dft_noise(int B, const DftNoiseArgs& A, int nloc, float calibration, float* out,
+                            hipStream_t stream);
+hipError_t launch_fft3d_pattern(int BW, const PatternArgs& A, int mode, hipStream_t stream);
+hipError_t launch_pattern_argmin(const float* scores, int n, int* selected, hipStream_t stream);
+hipError_t launch_fft3d_preview_store(const PatternArgs& A, void* dst, long long dst_stride, float peak, int bw,
+                                      int bh, hipStream_t stream);
 }  // namespace nf

@@ -1,0 +1,364 @@
+// nf_sampling.hip -- sampled noise models on gfx950.
+//
+// DFTTest (nlocation): DFTNoise::prepare, reference
+// src/algorithms/dft_noise.cpp:5-41. One workgroup per sample location
+// decodes the S x S x T rectangle (filter.hpp:381-389), applies the sample
+// window, runs the 3D transform in the same operation order as the strip
+// kernels (spatial rows, columns, then the temporal butterflies or direct
+// DFT), removes the template mean and writes the per-bin power of that
+// location. A second kernel sums the locations per bin in location order
+// (the reference's ordered scalar addition, kernels/scalar/model.cpp:20-28)
+// and applies the calibration.
+//
+// FFT3D (pfactor / pshow): Plan::select_pattern / pattern_candidate,
+// plan.cpp:227-284. One workgroup per candidate block computes the
+// mean-removed power and its weighted score in bin order; a single-workgroup
+// reduction picks the first minimum in scan order; the winner's power is
+// recomputed and scaled into the published model, or, for the preview, the
+// unwindowed winner goes through the forward and inverse transforms and is
+// written into an otherwise zero plane (plan.cpp:409-419).
+#include "nf_device.hpp"
+#include "nf_fft.hpp"
+#include "nf_launch.hpp"
+
+namespace nf {
+
+// ------------------------------------------------------------ DFTTest model
+template <int B>
+__global__ __launch_bounds__(256) void dft_noise_block_kernel(const DftNoiseArgs A) {
+  extern __shared__ float smem[];
+  constexpr int K = B / 2 + 1;
+  constexpr int BS = spec_block_stride(B);
+  const int T = A.T;
+  const int l = blockIdx.x;
+  const int tid = threadIdx.x, nthreads = blockDim.x;
+  float* sre = smem;           // [T][BS]
+  float* sim = sre + T * BS;   // [T][BS]
+  bool bad = false;
+  // rows: decode (x - 0) * scale, window (v * 1) * h, forward row transform
+  for (int t = tid; t < T * B; t += nthreads) {
+    const int z = t / B, y = t - z * B;
+    const void* rect = A.rects[l * T + z];
+    const long long stride = A.strides[l * T + z];
+    float re[B], im[B];
+#pragma unroll
+    for (int x = 0; x < B; ++x) {
+      const float v = (load_sample(rect, stride, A.storage, y, x) - 0.0f) * A.dscale;
+      bad |= !isfinite(v);
+      re[x] = (v * 1.0f) * A.h[(z * B + y) * B + x];
+      im[x] = 0.0f;
+    }
+    cfft<B, true>(re, im);
+    float* dr = sre + z * BS + y * K;
+    float* di = sim + z * BS + y * K;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      dr[k] = re[k];
+      di[k] = (k == B / 2) ? 0.0f : im[k];
+    }
+  }
+  __syncthreads();
+  for (int t = tid; t < T * K; t += nthreads) {
+    const int z = t / K, k = t - z * K;
+    float re[B], im[B];
+    float* cr = sre + z * BS + k;
+    float* ci = sim + z * BS + k;
+#pragma unroll
+    for (int y = 0; y < B; ++y) {
+      re[y] = cr[y * K];
+      im[y] = ci[y * K];
+    }
+    cfft<B, true>(re, im);
+#pragma unroll
+    for (int y = 0; y < B; ++y) {
+      cr[y * K] = re[y];
+      ci[y * K] = im[y];
+    }
+  }
+  __syncthreads();
+  if (T > 1) {
+    for (int bin = tid; bin < B * K; bin += nthreads) {
+      Cpx v[15];
+      for (int z = 0; z < T; ++z) v[z] = Cpx{sre[z * BS + bin], sim[z * BS + bin]};
+      if (T >= 3 && T <= 5) {
+        temporal_butterfly(T, v, true);
+      } else {
+        Cpx w[15];
+        for (int m = 0; m < T; ++m) w[m] = dft_term(T, v, m, A.ttw, true);
+        for (int m = 0; m < T; ++m) v[m] = w[m];
+      }
+      for (int z = 0; z < T; ++z) {
+        sre[z * BS + bin] = v[z].r;
+        sim[z * BS + bin] = v[z].i;
+      }
+    }
+    __syncthreads();
+  }
+  const float ratio = A.grid ? sre[0] / A.grid[0].x : 0.0f;  // finite(X0 / G0)
+  bad |= !isfinite(ratio);
+  float* q = A.power + (long long)l * T * B * K;
+  for (int k = tid; k < T * B * K; k += nthreads) {
+    const int z = k / (B * K), bin = k - z * (B * K);
+    float re = sre[z * BS + bin], im = sim[z * BS + bin];
+    if (A.grid) {
+      const float2 gv = A.grid[k];
+      re = re - ratio * gv.x;
+      im = im - ratio * gv.y;
+    }
+    const float p = (re * re) + (im * im);
+    bad |= !isfinite(p);
+    q[k] = p;
+  }
+  if (bad) atomicOr(A.flags, NF_FLAG_NONFINITE);
+}
+
+__global__ __launch_bounds__(256) void dft_noise_reduce_kernel(const float* power, int nloc, int bins,
+                                                               float calibration, float* out, unsigned* flags) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= bins) return;
+  bool bad = false;
+  float sum = 0.0f;
+  for (int l = 0; l < nloc; ++l) {
+    sum = sum + power[(long long)l * bins + k];
+    bad |= !isfinite(sum);
+  }
+  const float v = calibration * sum;
+  bad |= !isfinite(1.0f * v);
+  out[k] = v;
+  if (bad) atomicOr(flags, NF_FLAG_NONFINITE);
+}
+
+size_t dft_noise_lds_bytes(int B, int T) { return sizeof(float) * 2 * (size_t)T * spec_block_stride(B); }
+
+hipError_t launch_dft_noise(int B, const DftNoiseArgs& A, int nloc, float calibration, float* out,
+                            hipStream_t stream) {
+  const size_t lds = dft_noise_lds_bytes(B, A.T);
+  switch (B) {
+    case 8:
+      hipLaunchKernelGGL(dft_noise_block_kernel<8>, dim3(nloc), dim3(256), lds, stream, A);
+      break;
+    case 16:
+      hipLaunchKernelGGL(dft_noise_block_kernel<16>, dim3(nloc), dim3(256), lds, stream, A);
+      break;
+    case 32:
+      hipLaunchKernelGGL(dft_noise_block_kernel<32>, dim3(nloc), dim3(256), lds, stream, A);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int bins = A.T * B * (B / 2 + 1);
+  hipLaunchKernelGGL(dft_noise_reduce_kernel, dim3((bins + 255) / 256), dim3(256), 0, stream, A.power, nloc, bins,
+                     calibration, out, A.flags);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ FFT3D pattern
+// mode 0: score every candidate; 1: model of the selected candidate;
+// 2: preview inverse of the selected candidate (unwindowed).
+template <int BW>
+__global__ __launch_bounds__(64) void fft3d_pattern_kernel(const PatternArgs A, int mode) {
+  constexpr int BH = BW;
+  constexpr int K = BW / 2 + 1;
+  constexpr int BINS = BH * K;
+  constexpr int BS = spec_block_stride(BW);
+  __shared__ float sre[BS], sim[BS], pw[BINS], win[BH * (BW + 1)];
+  const int tid = threadIdx.x, nthreads = blockDim.x;
+  const int cand = mode == 0 ? (int)blockIdx.x : *A.selected;
+  const int bx = A.left + cand % A.ncx, by = A.top + cand / A.ncx;
+  const int ox = bx * A.gx.step, oy = by * A.gy.step;
+  const bool window = mode != 2;
+  bool bad = false;
+  // gather_pattern (plan.cpp:227-240): reflect, (x - base), window (v*wy)*wx
+  for (int idx = tid; idx < BH * BW; idx += nthreads) {
+    const int y = idx / BW, x = idx - y * BW;
+    const int sy = reflect(oy + y, A.gy), sx = reflect(ox + x, A.gx);
+    float v = load_sample(A.src, A.stride, A.storage, sy, sx) - A.base;
+    bad |= !isfinite(v);
+    if (window) v = (v * A.wya[y]) * A.wxa[x];
+    win[y * (BW + 1) + x] = v;
+  }
+  __syncthreads();
+  for (int y = tid; y < BH; y += nthreads) {
+    float re[BW], im[BW];
+#pragma unroll
+    for (int x = 0; x < BW; ++x) {
+      re[x] = win[y * (BW + 1) + x];
+      im[x] = 0.0f;
+    }
+    cfft<BW, true>(re, im);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      sre[y * K + k] = re[k];
+      sim[y * K + k] = (k == BW / 2) ? 0.0f : im[k];
+    }
+  }
+  __syncthreads();
+  for (int k = tid; k < K; k += nthreads) {
+    float re[BH], im[BH];
+#pragma unroll
+    for (int y = 0; y < BH; ++y) {
+      re[y] = sre[y * K + k];
+      im[y] = sim[y * K + k];
+    }
+    cfft<BH, true>(re, im);
+    if (mode == 2) {
+      // preview: the unfiltered spectrum goes straight back (fft.inverse)
+      cfft<BH, false>(re, im);
+    }
+#pragma unroll
+    for (int y = 0; y < BH; ++y) {
+      sre[y * K + k] = re[y];
+      sim[y * K + k] = im[y];
+    }
+  }
+  __syncthreads();
+  if (mode == 2) {
+    const float fct = 1.0f / ((float)BW * (float)BH);
+    for (int y = tid; y < BH; y += nthreads) {
+      float re[BW], im[BW];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        re[k] = sre[y * K + k];
+        im[k] = sim[y * K + k];
+      }
+#pragma unroll
+      for (int k = K; k < BW; ++k) {
+        re[k] = re[BW - k];
+        im[k] = 0.0f - im[BW - k];
+      }
+      cfft<BW, false>(re, im);
+#pragma unroll
+      for (int x = 0; x < BW; ++x) {
+        const float v = re[x] * fct;
+        bad |= !isfinite(v);
+        A.inv[y * BW + x] = v;
+      }
+    }
+    if (bad) atomicOr(A.flags, NF_FLAG_NONFINITE);
+    return;
+  }
+  // mean-removed power (model.cpp:20-28 with accumulate = false)
+  const float g = A.grid ? (A.degrid * sre[0]) / A.grid[0].x : 0.0f;
+  bad |= !isfinite(A.degrid * sre[0]) || !isfinite(g);
+  for (int k = tid; k < BINS; k += nthreads) {
+    float re = sre[k], im = sim[k];
+    if (A.grid) {
+      const float2 gv = A.grid[k];
+      re = re - g * gv.x;
+      im = im - g * gv.y;
+    }
+    const float p = (re * re) + (im * im);
+    bad |= !isfinite(p);
+    pw[k] = p;
+  }
+  __syncthreads();
+  if (mode == 0) {
+    if (tid == 0) {
+      float score = 0.0f;
+      for (int k = 0; k < BINS; ++k) {
+        score = score + pw[k] * A.weights[k];
+        bad |= !isfinite(score);
+      }
+      A.scores[cand] = score;
+    }
+  } else {
+    // pattern_candidate scale: v = (factor * p) * weight, finite(T * v)
+    for (int k = tid; k < BINS; k += nthreads) {
+      float v = A.factor * pw[k];
+      v = v * A.weights[k];
+      bad |= !isfinite(A.maximum * v);
+      if (A.kalman) bad |= !isfinite((v > 1e-15f ? v : 1e-15f) * A.ratio2);
+      A.model[k] = v;
+    }
+  }
+  if (bad) atomicOr(A.flags, NF_FLAG_NONFINITE);
+}
+
+// First minimum over candidates in scan order (strict '<' keeps the earliest).
+__global__ __launch_bounds__(256) void pattern_argmin_kernel(const float* scores, int n, int* selected) {
+  __shared__ float bs[256];
+  __shared__ int bi[256];
+  const int tid = threadIdx.x;
+  float best = 0.0f;
+  int idx = -1;
+  for (int i = tid; i < n; i += blockDim.x) {
+    const float s = scores[i];
+    if (idx < 0 || s < best) {
+      best = s;
+      idx = i;
+    }
+  }
+  bs[tid] = best;
+  bi[tid] = idx;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (tid < off) {
+      const float s2 = bs[tid + off];
+      const int i2 = bi[tid + off];
+      if (i2 >= 0 && (bi[tid] < 0 || s2 < bs[tid] || (s2 == bs[tid] && i2 < bi[tid]))) {
+        bs[tid] = s2;
+        bi[tid] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) *selected = bi[0];
+}
+
+// Preview output: inverse of the selected block at its place, zero elsewhere,
+// FFT3D conversion (scalar/spatial.cpp:63-88).
+__global__ __launch_bounds__(256) void fft3d_preview_store_kernel(const PatternArgs A, void* dst, long long dst_stride,
+                                                                  float peak, int bw, int bh) {
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = A.gx.length, h = A.gy.length;
+  if (gid >= (long long)w * h) return;
+  const int oy = (int)(gid / w), ox = (int)(gid - (long long)oy * w);
+  const int cand = *A.selected;
+  const int bx = A.left + cand % A.ncx, by = A.top + cand / A.ncx;
+  const int y = oy + A.gy.offset - by * A.gy.step, x = ox + A.gx.offset - bx * A.gx.step;
+  const float a = (y >= 0 && y < bh && x >= 0 && x < bw) ? A.inv[y * bw + x] : 0.0f;
+  char* drow = static_cast<char*>(dst) + (long long)oy * dst_stride;
+  if (A.storage == kF32) {
+    reinterpret_cast<float*>(drow)[ox] = a < 0.0f ? 0.0f : (1.0f < a ? 1.0f : a);
+  } else {
+    const float v = (a + 0.5f) + A.base;
+    const float cl = v < 0.0f ? 0.0f : (peak < v ? peak : v);
+    if (A.storage == kU8) reinterpret_cast<uint8_t*>(drow)[ox] = (uint8_t)cl;
+    else reinterpret_cast<uint16_t*>(drow)[ox] = (uint16_t)cl;
+  }
+}
+
+hipError_t launch_fft3d_pattern(int BW, const PatternArgs& A, int mode, hipStream_t stream) {
+  const int grid = mode == 0 ? A.ncx * A.ncy : 1;
+  switch (BW) {
+    case 8:
+      hipLaunchKernelGGL(fft3d_pattern_kernel<8>, dim3(grid), dim3(64), 0, stream, A, mode);
+      break;
+    case 16:
+      hipLaunchKernelGGL(fft3d_pattern_kernel<16>, dim3(grid), dim3(64), 0, stream, A, mode);
+      break;
+    case 32:
+      hipLaunchKernelGGL(fft3d_pattern_kernel<32>, dim3(grid), dim3(64), 0, stream, A, mode);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_pattern_argmin(const float* scores, int n, int* selected, hipStream_t stream) {
+  hipLaunchKernelGGL(pattern_argmin_kernel, dim3(1), dim3(256), 0, stream, scores, n, selected);
+  return hipGetLastError();
+}
+
+hipError_t launch_fft3d_preview_store(const PatternArgs& A, void* dst, long long dst_stride, float peak, int bw,
+                                      int bh, hipStream_t stream) {
+  const long long n = (long long)A.gx.length * A.gy.length;
+  hipLaunchKernelGGL(fft3d_preview_store_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, A, dst,
+                     dst_stride, peak, bw, bh);
+  return hipGetLastError();
+}
+
+}  // namespace nf

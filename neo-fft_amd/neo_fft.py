@@ -116,6 +116,12 @@ _SIGNATURES = {
                                                  ctypes.c_void_p]),
     "nf_dfttest_wiener_backward": (ctypes.c_int, [ctypes.c_void_p, Plane, Plane, MutPlane, ctypes.c_void_p,
                                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "nf_plan_sampling": (ctypes.c_int, [ctypes.c_void_p, c_int_p, c_int_p]),
+    "nf_dfttest_noise_locations": (ctypes.c_int, [ctypes.c_void_p, c_int_p, ctypes.c_int, c_int_p]),
+    "nf_dfttest_prepare_noise": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Plane), ctypes.c_int,
+                                                ctypes.c_void_p]),
+    "nf_fft3d_prepare_pattern": (ctypes.c_int, [ctypes.c_void_p, Plane, c_int_p, ctypes.c_void_p]),
+    "nf_plan_read_model": (ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, c_int_p, ctypes.c_void_p]),
     "nf_kalman_state_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     "nf_kalman_state_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "nf_kalman_state_destroy": (None, [ctypes.c_void_p]),
@@ -307,6 +313,36 @@ def dfttest_volume(plan, sources, dsts, flags, first_frame=0, plane=0, stream=No
 
 def _ptr(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def plan_sampling(plan):
+    """(needs_model, model_ready) of a plan with a sampled noise model."""
+    n, r = ctypes.c_int(), ctypes.c_int()
+    _check(lib().nf_plan_sampling(plan.handle, ctypes.byref(n), ctypes.byref(r)))
+    return bool(n.value), bool(r.value)
+
+
+def dfttest_prepare_noise(plan, samples, stream=None):
+    """samples: one device plane per (location, temporal slot), location-major."""
+    arr = (Plane * len(samples))(*[_plane(t) for t in samples])
+    _check(lib().nf_dfttest_prepare_noise(plan.handle, arr, len(samples), _stream_handle(stream)))
+
+
+def fft3d_prepare_pattern(plan, source, stream=None):
+    """Pattern search on one device plane; returns the selected block (bx, by)."""
+    sel = (ctypes.c_int * 2)()
+    _check(lib().nf_fft3d_prepare_pattern(plan.handle, _plane(source), sel, _stream_handle(stream)))
+    return sel[0], sel[1]
+
+
+def read_model(plan, stream=None):
+    import numpy as np
+    n = ctypes.c_int()
+    _check(lib().nf_plan_read_model(plan.handle, None, 0, ctypes.byref(n), _stream_handle(stream)))
+    out = np.zeros(n.value, np.float32)
+    _check(lib().nf_plan_read_model(plan.handle, out.ctypes.data_as(c_float_p), n.value, ctypes.byref(n),
+                                    _stream_handle(stream)))
+    return out
 
 
 def wiener_forward(plan, source, dst, flags, table=None, stream=None):

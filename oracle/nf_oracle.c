@@ -1112,11 +1112,15 @@ int nfo_dfttest_plane(const nfo_dft_cfg* c, int w, int h, int bits, int floating
   g_nonfinite = 0;
   const float scale = c->ftype < 2 ? wscale : 1.0f;
   const int curves = c->n_slocation || c->n_ssx || c->n_ssy || c->n_sst;
-  const float pa = curves ? 0 : fin(c->sigma / scale);
+  const int sampled = c->ftype < 2 && c->noise_table != NULL;  /* plan.cpp:194-207 */
+  const float pa = (curves || sampled) ? 0 : fin(c->sigma / scale);
   const float pb = fin(c->sigma2 / scale);
   const float plow = fin(c->pmin / wscale), phigh = fin(c->pmax / wscale);
   if (g_nonfinite) { rc = fail("non-finite sample or intermediate"); goto out; }
-  if (curves) {
+  if (sampled) {
+    primary = (float*)malloc(sizeof(float) * bins);
+    memcpy(primary, c->noise_table, sizeof(float) * bins);
+  } else if (curves) {
     primary = (float*)malloc(sizeof(float) * bins);
     if (nfo_dft_profile(c, T, B, c->sigma, scale, primary)) { rc = -1; goto out; }
   }
@@ -1198,6 +1202,72 @@ out:
   return rc;
 }
 
+/* DFTNoise (src/algorithms/dft_noise.cpp:5-41) with the filter's sample
+ * gather (src/plugin/filter.hpp:381-389) and the model kernels
+ * (src/kernels/scalar/model.cpp:6-43). The reference transforms each sample
+ * volume with its generic single-volume FFT (PocketFFT); this restatement uses
+ * the codelet-order transform, so the table's last bits are parity unpinned
+ * (tolerance-pinned against a binary64 DFT in the tests). */
+int nfo_dft_noise_model(const nfo_dft_cfg* c, float alpha, int bits, int floating, const void* const* rects,
+                        const long long* strides, int nloc, float* out) {
+  if (valid_format(bits, floating)) return -1;
+  REQUIRE(c->ftype < 2 && nloc >= 1 && nloc <= 500, "inactive DFTTest sample model");
+  const int B = c->sbsize, T = c->tbsize, K = B / 2 + 1;
+  const size_t S2 = (size_t)B * B, V = (size_t)T * S2, bins = (size_t)T * B * K;
+  float* hs = (float*)malloc(sizeof(float) * V);
+  float* ho = (float*)malloc(sizeof(float) * V);
+  float* blk = (float*)malloc(sizeof(float) * V);
+  float* grid = (float*)malloc(sizeof(float) * 2 * bins);
+  float* spec = (float*)malloc(sizeof(float) * 2 * bins);
+  float* sum = (float*)calloc(bins, sizeof(float));
+  int rc = 0;
+  float ws_sample, ws_out;
+  if (nfo_dft_window_3d(T, B, 0, 0, c->swin, c->twin, c->sbeta, c->tbeta, 0, 0, hs, &ws_sample) ||
+      nfo_dft_window_3d(T, B, c->smode == 0 ? 0 : c->sosize, c->smode, c->swin, c->twin, c->sbeta, c->tbeta, c->tmode,
+                        c->tmode ? c->tosize : 0, ho, &ws_out)) { rc = -1; goto out; }
+  g_nonfinite = 0;
+  const float a = alpha < 0 ? (c->ftype == 0 ? 5.0f : 7.0f) : alpha;
+  const float calibration = fin(fin((1.0f / (float)nloc) * fin(ws_sample / ws_out)) * a);
+  for (size_t i = 0; i < V; ++i) blk[i] = 255.0f * hs[i];
+  if (T > 1) nfo_rfft3(T, B, B, blk, grid);
+  else nfo_rfft2(B, B, blk, grid);
+  for (size_t i = 0; i < 2 * bins; ++i) fin(grid[i]);
+  if (g_nonfinite) { rc = fail("non-finite sample or intermediate"); goto out; }
+  if (c->zmean && grid[0] == 0) { rc = fail("DFTTest sample template DC must be nonzero"); goto out; }
+  {
+    const float dscale = floating ? 255.0f : 1.0f / (float)(1 << (bits - 8));
+    for (int l = 0; l < nloc; ++l) {
+      g_nonfinite = 0;
+      for (int z = 0; z < T; ++z) {
+        const void* r = rects[(size_t)l * T + z];
+        for (int y = 0; y < B; ++y)
+          for (int x = 0; x < B; ++x)
+            blk[((size_t)z * B + y) * B + x] = fin((load_sample(r, strides[(size_t)l * T + z], bits, floating, y, x) - 0.0f) * dscale);
+      }
+      for (size_t i = 0; i < V; ++i) blk[i] = fin((blk[i] * 1.0f) * hs[i]);
+      if (T > 1) nfo_rfft3(T, B, B, blk, spec);
+      else nfo_rfft2(B, B, blk, spec);
+      const float ratio = c->zmean ? fin(spec[0] / grid[0]) : 0.0f;
+      for (size_t k = 0; k < bins; ++k) {
+        const float re = spec[2 * k] - (c->zmean ? ratio * grid[2 * k] : 0);
+        const float im = spec[2 * k + 1] - (c->zmean ? ratio * grid[2 * k + 1] : 0);
+        const float q = fin((re * re) + (im * im));
+        sum[k] = fin(sum[k] + q);
+      }
+      if (g_nonfinite) { rc = fail("non-finite sample or intermediate"); goto out; }
+    }
+    for (size_t k = 0; k < bins; ++k) {
+      const float v = calibration * sum[k];
+      fin(1.0f * v);
+      out[k] = v;
+    }
+    if (g_nonfinite) { rc = fail("non-finite sample or intermediate"); goto out; }
+  }
+out:
+  free(hs); free(ho); free(blk); free(grid); free(spec); free(sum);
+  return rc;
+}
+
 /* ------------------------------------------------------------------- FFT3D
  * src/algorithms/plan.cpp:45-63 (validate), :91-169 (plan), :331-537 (run) */
 typedef struct {
@@ -1208,11 +1278,14 @@ typedef struct {
   float a, floor_, sigma_eff, norm, beta, degrid, r0, ratio2;
   int varying, denoise, kalman, enh, T;
   float base;
+  int sampled, preview;
+  float* weights;       /* pattern sampling weights (plan.cpp:142-160) */
+  const float* table;   /* per-bin noise: profile (varying) or pattern (sampled) */
 } fft3d_plan;
 
 static void fft3d_free(fft3d_plan* p) {
   free(p->wxa); free(p->wxs); free(p->wya); free(p->wys);
-  free(p->grid); free(p->primary); free(p->sh); free(p->halo);
+  free(p->grid); free(p->primary); free(p->sh); free(p->halo); free(p->weights);
   memset(p, 0, sizeof *p);
 }
 
@@ -1231,25 +1304,33 @@ static int fft3d_build(const nfo_fft3d_cfg* c, int w, int h, int bits, int float
   if (nfo_fft3d_window(c->bw, p->gx.overlap, c->wintype, p->wxa, p->wxs) ||
       nfo_fft3d_window(c->bh, p->gy.overlap, c->wintype, p->wya, p->wys)) { fft3d_free(p); return -1; }
   p->T = c->bt > 1 ? c->bt : 1;
-  p->denoise = c->bt != -1;
-  p->kalman = c->bt == 0;
   p->base = (!floating && chroma) ? (float)(1 << (bits - 1)) : 0.0f;
   g_nonfinite = 0;
   const float factor = floating ? 1.0f / 255 : (float)(1 << (bits - 8));
-  p->sigma_eff = fin(c->sigma * factor);
+  REQUIRE(isfinite(c->pfactor) && c->pfactor >= 0, "FFT3D pfactor must be nonnegative");
+  p->sampled = c->pfactor > 0;
+  p->sigma_eff = p->sampled ? 0.0f : fin(c->sigma * factor);
   p->norm = 1.0f / ((float)c->bw * (float)c->bh);
   p->beta = c->beta;
   p->degrid = c->degrid;
-  const float sig[4] = {p->sigma_eff, fin(c->sigma2 * factor), fin(c->sigma3 * factor), fin(c->sigma4 * factor)};
+  float sig[4] = {0, 0, 0, 0};
+  if (!p->sampled) {
+    sig[0] = p->sigma_eff; sig[1] = fin(c->sigma2 * factor); sig[2] = fin(c->sigma3 * factor);
+    sig[3] = fin(c->sigma4 * factor);
+  }
   p->varying = sig[1] != sig[0] || sig[2] != sig[0] || sig[3] != sig[0];
+  p->preview = c->pshow && (p->sampled || p->varying);  /* plan.cpp:113-115 */
+  p->denoise = c->bt != -1 && !p->preview;
+  p->kalman = c->bt == 0 && !p->preview;
   if (p->kalman) {
+    p->sigma_eff = fin(c->sigma * factor);
     p->r0 = fin((p->sigma_eff * p->sigma_eff) / p->norm);
     p->ratio2 = fin(c->kratio * c->kratio);
   }
   p->floor_ = (c->beta - 1) / c->beta;
   const int K = c->bw / 2 + 1;
   const size_t bins = (size_t)c->bh * K;
-  if (p->denoise) {
+  if (p->denoise && !p->sampled) {
     if (p->varying) {
       p->primary = (float*)malloc(sizeof(float) * bins);
       if (nfo_fft3d_profile(c->bw, c->bh, sig, p->primary)) { fft3d_free(p); return -1; }
@@ -1261,7 +1342,29 @@ static int fft3d_build(const nfo_fft3d_cfg* c, int w, int h, int bits, int float
     }
   }
   if (g_nonfinite) { fft3d_free(p); return fail("non-finite sample or intermediate"); }
-  p->enh = c->sharpen != 0 || c->dehalo != 0;
+  p->table = p->varying ? p->primary : NULL;
+  if (p->sampled && p->denoise) p->table = c->pattern;
+  if (p->preview || (p->sampled && p->denoise)) {
+    if (c->px == 0 && c->py == 0) {
+      if (!(p->gx.count >= 5 && p->gy.count >= 5)) { fft3d_free(p); return fail("FFT3D automatic pattern search requires a 5x5 grid"); }
+    } else if (!(c->px < p->gx.count && c->py < p->gy.count)) {
+      fft3d_free(p); return fail("FFT3D pattern coordinates outside grid");
+    }
+    const float cutoff2 = fin(c->pcutoff * c->pcutoff);
+    if (!(cutoff2 > 0)) { fft3d_free(p); return fail("FFT3D pattern cutoff underflow"); }
+    p->weights = (float*)malloc(sizeof(float) * bins);
+    for (int y = 0; y < c->bh; ++y) {
+      const int m = y < c->bh - y ? y : c->bh - y;
+      const float fy = 2 * (float)m / (float)c->bh;
+      const float other = fy * fy;
+      for (int x = 0; x < K; ++x) {
+        const float fx = 2 * (float)x / (float)c->bw;
+        const float q = fin(other + fin(fx * fx));
+        p->weights[(size_t)y * K + x] = fin(q / fin(q + cutoff2));  /* scalar/table.cpp weights */
+      }
+    }
+  }
+  p->enh = !p->preview && (c->sharpen != 0 || c->dehalo != 0);
   if (p->enh) {
     p->sh = (float*)malloc(sizeof(float) * bins);
     p->halo = (float*)malloc(sizeof(float) * bins);
@@ -1313,6 +1416,112 @@ static void fft3d_gather(const fft3d_plan* p, const float* pad, int oy, int ox, 
       blk[y * bw + x] = (pad[(size_t)(oy + y) * p->gx.cover + ox + x] * p->wya[y]) * p->wxa[x];
 }
 
+/* Plan::select_pattern (plan.cpp:241-270): every candidate block of the grid
+ * interior (or the one at px, py) is windowed, transformed, mean-removed and
+ * scored by sum(power * weight) in bin order; the first minimum wins. */
+static int fft3d_select(const nfo_fft3d_cfg* c, const fft3d_plan* p, const float* pad, int* sel_bx, int* sel_by,
+                        float* best_power) {
+  const int bw = c->bw, bh = c->bh, bins = bh * (bw / 2 + 1);
+  const int automatic = c->px == 0 && c->py == 0;
+  const int left = automatic ? 2 : c->px, right = automatic ? p->gx.count - 3 : c->px;
+  const int top = automatic ? 2 : c->py, bottom = automatic ? p->gy.count - 3 : c->py;
+  float* blk = (float*)malloc(sizeof(float) * bw * bh);
+  float* spec = (float*)malloc(sizeof(float) * 2 * bins);
+  float* power = (float*)malloc(sizeof(float) * bins);
+  float best = 0;
+  int found = 0, rc = 0;
+  g_nonfinite = 0;
+  for (int by = top; by <= bottom; ++by)
+    for (int bx = left; bx <= right; ++bx) {
+      fft3d_gather(p, pad, by * p->gy.step, bx * p->gx.step, bw, bh, blk);
+      nfo_rfft2(bh, bw, blk, spec);
+      const float g = p->grid ? fin(fin(p->degrid * spec[0]) / p->grid[0]) : 0;
+      for (int k = 0; k < bins; ++k) {
+        const float re = spec[2 * k] - (p->grid ? g * p->grid[2 * k] : 0);
+        const float im = spec[2 * k + 1] - (p->grid ? g * p->grid[2 * k + 1] : 0);
+        power[k] = fin((re * re) + (im * im));
+      }
+      float score = 0;
+      for (int k = 0; k < bins; ++k) score = fin(score + power[k] * p->weights[k]);
+      if (!found || score < best) {
+        found = 1; best = score; *sel_bx = bx; *sel_by = by;
+        if (best_power) memcpy(best_power, power, sizeof(float) * bins);
+      }
+    }
+  if (g_nonfinite) rc = fail("non-finite sample or intermediate");
+  else if (!found) rc = fail("FFT3D no pattern candidate");
+  free(blk); free(spec); free(power);
+  return rc;
+}
+
+int nfo_fft3d_pattern(const nfo_fft3d_cfg* c, int w, int h, int bits, int floating, int chroma, const void* src,
+                      long long stride, int* sel_bx, int* sel_by, float* power, float* model) {
+  fft3d_plan p;
+  if (fft3d_build(c, w, h, bits, floating, chroma, &p)) return -1;
+  if (!(p.sampled && p.denoise) && !p.preview) { fft3d_free(&p); return fail("FFT3D sampling is inactive"); }
+  const int bins = c->bh * (c->bw / 2 + 1);
+  const size_t plane_sz = (size_t)p.gx.cover * p.gy.cover;
+  float* pad = (float*)malloc(sizeof(float) * plane_sz);
+  float* pw = (float*)malloc(sizeof(float) * bins);
+  int rc = 0;
+  g_nonfinite = 0;
+  pad_plane(src, stride, bits, floating, &p.gx, &p.gy, p.base, 1.0f, pad);
+  if (g_nonfinite) rc = fail("non-finite sample or intermediate");
+  if (!rc) rc = fft3d_select(c, &p, pad, sel_bx, sel_by, pw);
+  if (!rc && power) memcpy(power, pw, sizeof(float) * bins);
+  if (!rc && model) {
+    /* Plan::pattern_candidate (plan.cpp:271-279) with scalar/model.cpp scale */
+    g_nonfinite = 0;
+    const float factor = p.kalman ? 1.0f : c->pfactor, maximum = (float)p.T;
+    for (int k = 0; k < bins; ++k) {
+      float v = factor * pw[k];
+      v = v * p.weights[k];
+      fin(maximum * v);
+      model[k] = v;
+    }
+    if (p.kalman)
+      for (int k = 0; k < bins; ++k) fin((model[k] > 1e-15f ? model[k] : 1e-15f) * p.ratio2);
+    if (g_nonfinite) rc = fail("non-finite sample or intermediate");
+  }
+  free(pad); free(pw);
+  fft3d_free(&p);
+  return rc;
+}
+
+/* pshow preview (plan.cpp:409-419): the selected block of the first source,
+ * unwindowed, through the forward and inverse transforms, placed into an
+ * otherwise zero accumulator. */
+static int fft3d_preview(const nfo_fft3d_cfg* c, const fft3d_plan* p, int bits, int floating, const void* src,
+                         long long stride, void* dst, long long dst_stride) {
+  const int bw = c->bw, bh = c->bh, bins = bh * (bw / 2 + 1);
+  const size_t plane_sz = (size_t)p->gx.cover * p->gy.cover;
+  float* pad = (float*)malloc(sizeof(float) * plane_sz);
+  float* acc = (float*)calloc(plane_sz, sizeof(float));
+  float* blk = (float*)malloc(sizeof(float) * bw * bh);
+  float* spec = (float*)malloc(sizeof(float) * 2 * bins);
+  float* inv = (float*)malloc(sizeof(float) * bw * bh);
+  int rc = 0, bx = 0, by = 0;
+  g_nonfinite = 0;
+  pad_plane(src, stride, bits, floating, &p->gx, &p->gy, p->base, 1.0f, pad);
+  if (g_nonfinite) rc = fail("non-finite sample or intermediate");
+  if (!rc) rc = fft3d_select(c, p, pad, &bx, &by, NULL);
+  if (!rc) {
+    const int ox = bx * p->gx.step, oy = by * p->gy.step;
+    for (int y = 0; y < bh; ++y)
+      for (int x = 0; x < bw; ++x) blk[y * bw + x] = pad[(size_t)(oy + y) * p->gx.cover + ox + x];
+    nfo_rfft2(bh, bw, blk, spec);
+    nfo_irfft2(bh, bw, spec, inv, 1.0f / ((float)bw * (float)bh));
+    g_nonfinite = 0;
+    for (int i = 0; i < bw * bh; ++i) fin(inv[i]);
+    if (g_nonfinite) rc = fail("non-finite sample or intermediate");
+    for (int y = 0; y < bh && !rc; ++y)
+      for (int x = 0; x < bw; ++x) acc[(size_t)(oy + y) * p->gx.cover + ox + x] = inv[y * bw + x];
+  }
+  if (!rc) store_plane(acc, p->gx.cover, &p->gx, &p->gy, bits, floating, 1, p->base, dst, dst_stride);
+  free(pad); free(acc); free(blk); free(spec); free(inv);
+  return rc;
+}
+
 static int fft3d_run(const nfo_fft3d_cfg* c, const fft3d_plan* p, int bits, int floating,
                      const void* const* srcs, const long long* strides, int T, const float* kalman_last,
                      void* dst, long long dst_stride) {
@@ -1333,7 +1542,7 @@ static int fft3d_run(const nfo_fft3d_cfg* c, const fft3d_plan* p, int bits, int 
     if (g_nonfinite) { rc = fail("non-finite sample or intermediate"); goto out; }
   }
   {
-    const float noise_uniform = p->denoise && !p->varying ? (((float)T * p->sigma_eff) * p->sigma_eff) / p->norm : 0;
+    const float noise_uniform = p->denoise && !p->table ? (((float)T * p->sigma_eff) * p->sigma_eff) / p->norm : 0;
     const float lower = (p->beta - 1.0f) / p->beta;
     for (int by = 0; by < p->gy.count; ++by) {
       const int oy = by * p->gy.step;
@@ -1351,11 +1560,11 @@ static int fft3d_run(const nfo_fft3d_cfg* c, const fft3d_plan* p, int bits, int 
             nfo_rfft2(bh, bw, blk, s);
             const float scale = p->grid ? (p->degrid * s[0]) / p->grid[0] : 0;
             if (p->denoise) {
-              const int e = !p->varying && p->enh;
-              if (nfo_spectral(s, p->grid, bins, scale, -1, p->a, 0, 0, 0, 1, p->floor_, p->primary,
+              const int e = !p->table && p->enh;
+              if (nfo_spectral(s, p->grid, bins, scale, -1, p->a, 0, 0, 0, 1, p->floor_, p->table,
                                e ? c->sharpen : 0, e ? c->dehalo : 0, p->abc[0], p->abc[1], p->abc[2], p->sh,
                                p->halo)) { rc = -1; goto out; }
-              if (p->varying && fft3d_enhance(c, p, s, bins)) { rc = -1; goto out; }
+              if (p->table && fft3d_enhance(c, p, s, bins)) { rc = -1; goto out; }
             } else if (fft3d_enhance(c, p, s, bins)) { rc = -1; goto out; }
           }
           if (fft3d_emit(c, p, s, inv, row, ox)) { rc = -1; goto out; }
@@ -1367,7 +1576,7 @@ static int fft3d_run(const nfo_fft3d_cfg* c, const fft3d_plan* p, int bits, int 
             ptrs[j] = spectra + 2 * (size_t)bins * j;
           }
           if (nfo_fft3d_temporal(ptrs, T, c_slot, bins, p->degrid, p->grid, noise_uniform,
-                                 p->varying ? p->primary : NULL, (float)T, lower, out_spec)) { rc = -1; goto out; }
+                                 p->table, (float)T, lower, out_spec)) { rc = -1; goto out; }
           if (fft3d_enhance(c, p, out_spec, bins)) { rc = -1; goto out; }
           if (fft3d_emit(c, p, out_spec, inv, row, ox)) { rc = -1; goto out; }
         }
@@ -1394,6 +1603,8 @@ int nfo_fft3d_plane(const nfo_fft3d_cfg* c, int w, int h, int bits, int floating
   int rc;
   if (p.kalman) rc = fail("Kalman planes use nfo_kalman_*");
   else if (!(nsrc == p.T || nsrc == 1)) rc = fail("FFT3D sources size must match plan temporal_size or single-frame fallback");
+  else if (p.preview) rc = fft3d_preview(c, &p, bits, floating, srcs[0], strides[0], dst, dst_stride);
+  else if (p.sampled && p.denoise && !c->pattern) rc = fail("FFT3D sampled model not initialized");
   else rc = fft3d_run(c, &p, bits, floating, srcs, strides, nsrc, NULL, dst, dst_stride);
   fft3d_free(&p);
   return rc;
@@ -1422,6 +1633,7 @@ int nfo_kalman_advance(const nfo_fft3d_cfg* c, int w, int h, int bits, int float
   if (fft3d_build(c, w, h, bits, floating, chroma, &p)) return -1;
   int rc = 0;
   if (!p.kalman) { fft3d_free(&p); return fail("not a Kalman plan"); }
+  if (p.sampled && !c->pattern) { fft3d_free(&p); return fail("Kalman sampled model not initialized"); }
   const int bw = c->bw, bh = c->bh, bins = bh * (bw / 2 + 1);
   const size_t plane_sz = (size_t)p.gx.cover * p.gy.cover;
   float* pad = (float*)malloc(sizeof(float) * plane_sz);
@@ -1435,7 +1647,7 @@ int nfo_kalman_advance(const nfo_fft3d_cfg* c, int w, int h, int bits, int float
       fft3d_gather(&p, pad, by * p.gy.step, bx * p.gx.step, bw, bh, blk);
       nfo_rfft2(bh, bw, blk, spec);
       const size_t off = ((size_t)by * p.gx.count + bx) * bins;
-      rc = nfo_kalman_step(spec, last + 2 * off, cov + off, proc + off, p.varying ? p.primary : NULL, p.r0, p.ratio2, bins);
+      rc = nfo_kalman_step(spec, last + 2 * off, cov + off, proc + off, p.table, p.r0, p.ratio2, bins);
     }
   free(pad); free(blk); free(spec);
   fft3d_free(&p);

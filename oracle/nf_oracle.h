@@ -37,6 +37,8 @@ typedef struct {
   /* optional frequency curves: (position, value) pairs, n = number of floats */
   const float *slocation, *ssx, *ssy, *sst;
   int n_slocation, n_ssx, n_ssy, n_sst, ssystem;
+  /* sampled noise model (nlocation), T*B*(B/2+1) floats, or NULL */
+  const float* noise_table;
 } nfo_dft_cfg;
 
 typedef struct {
@@ -44,6 +46,11 @@ typedef struct {
   float sigma, beta, degrid, kratio;
   float sigma2, sigma3, sigma4;
   float sharpen, scutoff, svr, smin, smax, dehalo, hr, ht;
+  /* pattern sampling: pfactor > 0 selects it; pattern = the published
+   * per-bin model (bh*(bw/2+1) floats) from nfo_fft3d_pattern */
+  float pfactor, pcutoff;
+  int px, py, pshow;
+  const float* pattern;
 } nfo_fft3d_cfg;
 
 const char* nfo_last_error(void);
@@ -100,6 +107,19 @@ int nfo_dfttest_plane(const nfo_dft_cfg* c, int w, int h, int bits, int floating
 int nfo_fft3d_plane(const nfo_fft3d_cfg* c, int w, int h, int bits, int floating, int chroma,
                     const void* const* srcs, const long long* strides, int nsrc, void* dst,
                     long long dst_stride);
+/* DFTNoise::prepare (dft_noise.cpp:5-41): rects[l*T+z] points at the top-left
+ * sample of location l's S x S rectangle in frame (frame_l + z); strides in
+ * bytes. alpha < 0 selects the default (5 for ftype 0, else 7). Writes the
+ * calibrated power table, T*S*(S/2+1) floats. */
+int nfo_dft_noise_model(const nfo_dft_cfg* c, float alpha, int bits, int floating, const void* const* rects,
+                        const long long* strides, int nloc, float* out);
+/* FFT3D pattern search and model (plan.cpp:142-160, 227-284): selected block
+ * coordinates, the unscaled power of the selected block and the published
+ * model (power * factor * weight, factor = 1 for Kalman plans else pfactor).
+ * power/model may be NULL. */
+int nfo_fft3d_pattern(const nfo_fft3d_cfg* c, int w, int h, int bits, int floating, int chroma, const void* src,
+                      long long stride, int* sel_bx, int* sel_by, float* power, float* model);
+
 /* Kalman (bt=0): state = last[2*bins] (complex), cov[bins], proc[bins] */
 long long nfo_kalman_bins(const nfo_fft3d_cfg* c, int w, int h);
 int nfo_kalman_init(const nfo_fft3d_cfg* c, int w, int h, int bits, int floating, float* last,

@@ -28,21 +28,26 @@ class DftCfg(ctypes.Structure):
                 + [(n, ctypes.c_float) for n in ("sigma", "sigma2", "pmin", "pmax", "sbeta", "tbeta", "f0beta")]
                 + [(n, ctypes.c_int) for n in ("zmean", "dither", "dither_seed")]
                 + [(n, c_float_p) for n in ("slocation", "ssx", "ssy", "sst")]
-                + [(n, ctypes.c_int) for n in ("n_slocation", "n_ssx", "n_ssy", "n_sst", "ssystem")])
+                + [(n, ctypes.c_int) for n in ("n_slocation", "n_ssx", "n_ssy", "n_sst", "ssystem")]
+                + [("noise_table", c_float_p)])
 
 
 class Fft3dCfg(ctypes.Structure):
     _fields_ = ([(n, ctypes.c_int) for n in ("bw", "bh", "ow", "oh", "bt", "wintype")]
                 + [(n, ctypes.c_float) for n in ("sigma", "beta", "degrid", "kratio", "sigma2", "sigma3", "sigma4",
-                                                 "sharpen", "scutoff", "svr", "smin", "smax", "dehalo", "hr", "ht")])
+                                                 "sharpen", "scutoff", "svr", "smin", "smax", "dehalo", "hr", "ht")]
+                + [("pfactor", ctypes.c_float), ("pcutoff", ctypes.c_float)]
+                + [(n, ctypes.c_int) for n in ("px", "py", "pshow")]
+                + [("pattern", c_float_p)])
 
 
 DFT_DEFAULTS = dict(ftype=0, sigma=8.0, sigma2=8.0, pmin=0.0, pmax=500.0, sbsize=16, smode=1, sosize=12, tbsize=3,
                     tmode=0, tosize=0, swin=0, twin=7, sbeta=2.5, tbeta=2.5, zmean=True, f0beta=1.0, dither=0,
-                    dither_seed=0, slocation=None, ssx=None, ssy=None, sst=None, ssystem=0)
+                    dither_seed=0, slocation=None, ssx=None, ssy=None, sst=None, ssystem=0, noise_table=None)
 FFT3D_DEFAULTS = dict(sigma=2.0, beta=1.0, bw=32, bh=32, bt=3, ow=-1, oh=-1, kratio=2.0, sharpen=0.0, scutoff=0.3,
                       svr=1.0, smin=4.0, smax=20.0, wintype=0, degrid=1.0, dehalo=0.0, hr=2.0, ht=50.0,
-                      sigma2=None, sigma3=None, sigma4=None)
+                      sigma2=None, sigma3=None, sigma4=None, pfactor=0.0, pcutoff=0.1, px=0, py=0, pshow=False,
+                      pattern=None)
 
 
 def lib():
@@ -93,6 +98,10 @@ def dft_cfg(**kw):
             keep.append(arr)
             setattr(c, n, _fptr(arr))
         setattr(c, "n_" + n, cnt)
+    if p["noise_table"] is not None:
+        nt = np.ascontiguousarray(p["noise_table"], np.float32)
+        keep.append(nt)
+        c.noise_table = _fptr(nt)
     c._keep = keep
     return c
 
@@ -107,8 +116,13 @@ def fft3d_cfg(**kw):
     for n in ("bw", "bh", "ow", "oh", "bt", "wintype"):
         setattr(c, n, int(p[n]))
     for n in ("sigma", "beta", "degrid", "kratio", "sigma2", "sigma3", "sigma4", "sharpen", "scutoff", "svr", "smin",
-              "smax", "dehalo", "hr", "ht"):
+              "smax", "dehalo", "hr", "ht", "pfactor", "pcutoff"):
         setattr(c, n, float(p[n]))
+    c.px, c.py, c.pshow = int(p["px"]), int(p["py"]), int(bool(p["pshow"]))
+    if p["pattern"] is not None:
+        pt = np.ascontiguousarray(p["pattern"], np.float32)
+        c._keep = [pt]
+        c.pattern = _fptr(pt)
     return c
 
 
@@ -349,3 +363,39 @@ class KalmanState:
                                        int(chroma), self.last.ctypes.data_as(c_float_p),
                                        dst.ctypes.data_as(ctypes.c_void_p), ctypes.c_longlong(dst.strides[0])))
         return dst
+
+
+def dft_noise_model(frames, locations, bits=8, floating=False, alpha=-1.0, **params):
+    """frames[n][p]: 2D sample planes; locations: (frame, plane, y, x) tuples.
+    Returns the calibrated noise power table (T*S*(S/2+1) floats)."""
+    c = dft_cfg(**params)
+    T, S = c.tbsize, c.sbsize
+    dt = _fmt_dtype(bits, floating)
+    keep, ptrs, strides = [], [], []
+    for (f, p, y, x) in locations:
+        for z in range(T):
+            a = np.ascontiguousarray(frames[f + z][p], dt)
+            keep.append(a)
+            ptrs.append(a.ctypes.data + y * a.strides[0] + x * a.strides[1])
+            strides.append(a.strides[0])
+    out = np.zeros(T * S * (S // 2 + 1), np.float32)
+    pa = (ctypes.c_void_p * len(ptrs))(*ptrs)
+    sa = (ctypes.c_longlong * len(strides))(*strides)
+    _check(lib().nfo_dft_noise_model(ctypes.byref(c), ctypes.c_float(alpha), bits, int(floating), pa, sa,
+                                     len(locations), _fptr(out)))
+    return out
+
+
+def fft3d_pattern(src, bits=8, floating=False, chroma=False, **params):
+    """Pattern search on one plane: ((bx, by), power, model)."""
+    c = fft3d_cfg(**params)
+    a = np.ascontiguousarray(src, _fmt_dtype(bits, floating))
+    h, w = a.shape
+    n = c.bh * (c.bw // 2 + 1)
+    power = np.zeros(n, np.float32)
+    model = np.zeros(n, np.float32)
+    bx, by = ctypes.c_int(), ctypes.c_int()
+    _check(lib().nfo_fft3d_pattern(ctypes.byref(c), w, h, bits, int(floating), int(chroma), ctypes.c_void_p(a.ctypes.data),
+                                   ctypes.c_longlong(a.strides[0]), ctypes.byref(bx), ctypes.byref(by), _fptr(power),
+                                   _fptr(model)))
+    return (bx.value, by.value), power, model

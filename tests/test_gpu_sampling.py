@@ -1,0 +1,203 @@
+"""Sampled noise models on the GPU against the oracle: DFTTest nlocation
+(DFTNoise), FFT3D pfactor pattern search/model (spatial, temporal, Kalman)
+and the pshow preview, at plan level and through the filter layer. The
+kernels mirror the oracle's operation order, so tables, selected blocks and
+planes are compared for exact equality."""
+import numpy as np
+import pytest
+
+from gpu_util import compare, empty_like_np, to_device, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+def _plane(rng, h, w, bits, floating):
+    if floating:
+        return rng.uniform(0.0, 1.0, size=(h, w)).astype(np.float32)
+    peak = (1 << bits) - 1
+    return rng.integers(0, peak + 1, size=(h, w)).astype(np.uint8 if bits == 8 else np.uint16)
+
+
+def _dtype(bits, floating):
+    return np.float32 if floating else (np.uint8 if bits == 8 else np.uint16)
+
+
+@pytest.mark.parametrize("B,T", [(8, 1), (16, 1), (16, 3), (32, 1), (16, 5), (8, 2)])
+@pytest.mark.parametrize("zmean", [True, False])
+def test_dfttest_noise_model_and_plane(nf, oracle, B, T, zmean):
+    rng = np.random.default_rng(B * 10 + T + zmean)
+    h, w, bits = 70, 90, 8
+    frames = [[_plane(rng, h, w, bits, False)] for _ in range(T + 3)]
+    locations = [(0, 0, 3, 5), (2, 0, h - B, w - B), (1, 0, 20, 30)]
+    tmode = 0 if T % 2 else 1
+    params = dict(tbsize=T, tmode=tmode, sbsize=B, sosize=B * 3 // 4, ftype=0, sigma=8.0, zmean=zmean)
+    plan = nf.dfttest_plan(w, h, bits=bits, nlocation=[v for loc in locations for v in loc], **params)
+    assert nf.plan_sampling(plan) == (True, False)
+    dev = [[to_device(p) for p in fr] for fr in frames]
+    with pytest.raises(nf.InvalidArgument, match="sample model not initialized"):
+        plan.process([dev[0][0]] * T, empty_like_np(frames[0][0]), targets=[0] if tmode else None)
+    nf.dfttest_prepare_noise(plan, [dev[f + z][p] for (f, p, _, _) in locations for z in range(T)])
+    assert nf.plan_sampling(plan) == (True, True)
+    got_table = nf.read_model(plan)
+    ref_table = oracle.dft_noise_model(frames, locations, bits=bits, **params)
+    assert np.array_equal(got_table, ref_table), np.abs(got_table - ref_table).max()
+    slots = [frames[z][0] for z in range(T)]
+    ref = oracle.dfttest_plane(slots, bits=bits, noise_table=ref_table, targets=[0] if tmode else None, **params)
+    dst = empty_like_np(slots[0])
+    plan.process([to_device(s) for s in slots], dst, targets=[0] if tmode else None)
+    got = to_host(dst, np.uint8)
+    assert np.array_equal(got, ref), compare(got, ref)
+
+
+@pytest.mark.parametrize("bits,floating", [(10, False), (32, True)])
+def test_dfttest_noise_formats(nf, oracle, bits, floating):
+    rng = np.random.default_rng(bits)
+    h, w = 50, 60
+    frames = [[_plane(rng, h, w, bits, floating)] for _ in range(3)]
+    locations = [(0, 0, 1, 2), (0, 0, 30, 40)]
+    params = dict(tbsize=3, sbsize=16, sosize=12, ftype=1, sigma=8.0)
+    plan = nf.dfttest_plan(w, h, bits=bits, floating=floating, nlocation=[v for loc in locations for v in loc],
+                           **params)
+    dev = [[to_device(p) for p in fr] for fr in frames]
+    nf.dfttest_prepare_noise(plan, [dev[f + z][p] for (f, p, _, _) in locations for z in range(3)])
+    ref_table = oracle.dft_noise_model(frames, locations, bits=bits, floating=floating, **params)
+    assert np.array_equal(nf.read_model(plan), ref_table)
+    slots = [frames[z][0] for z in range(3)]
+    ref = oracle.dfttest_plane(slots, bits=bits, floating=floating, noise_table=ref_table, **params)
+    dst = empty_like_np(slots[0])
+    plan.process([to_device(s) for s in slots], dst)
+    got = to_host(dst, _dtype(bits, floating))
+    assert np.array_equal(got, ref), compare(got, ref)
+
+
+@pytest.mark.parametrize("bw", [8, 16, 32])
+@pytest.mark.parametrize("bt", [1, 3])
+@pytest.mark.parametrize("pxy", [(0, 0), (3, 2)])
+def test_fft3d_pattern_model_and_plane(nf, oracle, bw, bt, pxy):
+    rng = np.random.default_rng(bw + bt + pxy[0])
+    h, w = 120, 150
+    frames = [_plane(rng, h, w, 8, False) for _ in range(bt)]
+    params = dict(bw=bw, bh=bw, bt=bt, pfactor=1.5, px=pxy[0], py=pxy[1], sharpen=0.3, sigma=3.0)
+    plan = nf.fft3d_plan(w, h, bits=8, **params)
+    assert nf.plan_sampling(plan) == (True, False)
+    with pytest.raises(nf.InvalidArgument, match="sampled model not initialized"):
+        plan.process([to_device(f) for f in frames], empty_like_np(frames[0]))
+    sel = nf.fft3d_prepare_pattern(plan, to_device(frames[0]))
+    (rx, ry), _, ref_model = oracle.fft3d_pattern(frames[0], bits=8, **params)
+    assert sel == (rx, ry)
+    got_model = nf.read_model(plan)
+    assert np.array_equal(got_model, ref_model)
+    ref = oracle.fft3d_plane(frames, bits=8, pattern=ref_model, **params)
+    dst = empty_like_np(frames[0])
+    plan.process([to_device(f) for f in frames], dst)
+    got = to_host(dst, np.uint8)
+    assert np.array_equal(got, ref), compare(got, ref)
+
+
+@pytest.mark.parametrize("bits,floating,chroma", [(16, False, True), (32, True, False)])
+def test_fft3d_pattern_formats(nf, oracle, bits, floating, chroma):
+    rng = np.random.default_rng(7)
+    h, w = 96, 128
+    frames = [_plane(rng, h, w, bits, floating) for _ in range(3)]
+    params = dict(bw=16, bh=16, bt=3, pfactor=0.8)
+    plan = nf.fft3d_plan(w, h, bits=bits, floating=floating, chroma=chroma, **params)
+    sel = nf.fft3d_prepare_pattern(plan, to_device(frames[1]))
+    (rx, ry), _, ref_model = oracle.fft3d_pattern(frames[1], bits=bits, floating=floating, chroma=chroma, **params)
+    assert sel == (rx, ry)
+    assert np.array_equal(nf.read_model(plan), ref_model)
+    ref = oracle.fft3d_plane(frames, bits=bits, floating=floating, chroma=chroma, pattern=ref_model, **params)
+    dst = empty_like_np(frames[0])
+    plan.process([to_device(f) for f in frames], dst)
+    got = to_host(dst, _dtype(bits, floating))
+    assert np.array_equal(got, ref), compare(got, ref)
+
+
+def test_fft3d_sampled_kalman(nf, oracle):
+    rng = np.random.default_rng(23)
+    h, w = 80, 96
+    base = _plane(rng, h, w, 8, False).astype(np.int32)
+    frames = [np.clip(base + rng.integers(-6, 7, size=(h, w)), 0, 255).astype(np.uint8) for _ in range(5)]
+    params = dict(bw=16, bh=16, bt=0, pfactor=1.0, sigma=3.0, kratio=2.0)
+    plan = nf.fft3d_plan(w, h, bits=8, **params)
+    nf.fft3d_prepare_pattern(plan, to_device(frames[0]))
+    _, _, ref_model = oracle.fft3d_pattern(frames[0], bits=8, **params)
+    assert np.array_equal(nf.read_model(plan), ref_model)
+    st = nf.KalmanState(plan)
+    ref = oracle.KalmanState(dict(params, pattern=ref_model), w, h, bits=8)
+    for f in frames:
+        st.advance(to_device(f))
+        ref.advance(f)
+        dst = empty_like_np(f)
+        st.render(to_device(f), dst)
+        got = to_host(dst, np.uint8)
+        assert np.array_equal(got, ref.render()), compare(got, ref.render())
+
+
+@pytest.mark.parametrize("kw", [dict(pfactor=1.0), dict(sigma2=5.0), dict(pfactor=1.0, px=2, py=3)])
+@pytest.mark.parametrize("bits,floating", [(8, False), (32, True)])
+def test_fft3d_pshow_preview(nf, oracle, kw, bits, floating):
+    rng = np.random.default_rng(31)
+    h, w = 90, 110
+    frames = [_plane(rng, h, w, bits, floating) for _ in range(3)]
+    params = dict(bw=16, bh=16, bt=3, pshow=True, **kw)
+    plan = nf.fft3d_plan(w, h, bits=bits, floating=floating, **params)
+    ref = oracle.fft3d_plane(frames, bits=bits, floating=floating, **params)
+    dst = empty_like_np(frames[0])
+    plan.process([to_device(f) for f in frames], dst)
+    got = to_host(dst, _dtype(bits, floating))
+    assert np.array_equal(got, ref), compare(got, ref)
+
+
+def test_pattern_search_grid_requirements(nf):
+    with pytest.raises(nf.InvalidArgument, match="5x5 grid"):
+        nf.fft3d_plan(24, 24, bits=8, bw=16, bh=16, pfactor=1.0)
+    with pytest.raises(nf.InvalidArgument, match="outside grid"):
+        nf.fft3d_plan(200, 200, bits=8, bw=16, bh=16, pfactor=1.0, px=40, py=1)
+
+
+def _yuv(n, w, h, seed):
+    rng = np.random.default_rng(seed)
+    base = [rng.integers(0, 256, size=s) for s in ((h, w), (h // 2, w // 2), (h // 2, w // 2))]
+    return [[np.clip(b + rng.integers(-12, 13, size=b.shape), 0, 255).astype(np.uint8) for b in base]
+            for _ in range(n)]
+
+
+def test_filter_dfttest_nlocation(nf, oracle):
+    N, w, h = 7, 96, 64
+    frames = _yuv(N, w, h, 3)
+    locations = [(1, 0, 10, 20), (4, 1, 5, 8)]
+    params = dict(tbsize=3, sbsize=16, sosize=12, sigma=8.0)
+    clip = nf.Clip([[to_device(p) for p in fr] for fr in frames], color_family="yuv", bits=8, subsampling=(1, 1))
+    out = nf.DFTTest(clip, nlocation=[v for loc in locations for v in loc], **params)
+    sample_frames = sorted({f + z for (f, _, _, _) in locations for z in range(3)})
+    table = oracle.dft_noise_model(frames, locations, bits=8, **params)
+    for n in (0, 3, N - 1):
+        slots = [min(max(n + j - 1, 0), N - 1) for j in range(3)]
+        assert out.requests(n) == sorted(set(slots) | set(sample_frames))
+        got = out.get_frame(n)
+        for p in range(3):
+            ref = oracle.dfttest_plane([frames[s][p] for s in slots], bits=8, noise_table=table, **params)
+            assert np.array_equal(to_host(got[p], np.uint8), ref), (n, p)
+
+
+def test_filter_fft3d_pfactor_with_roi(nf, oracle):
+    N, w, h = 5, 160, 128
+    frames = _yuv(N, w, h, 9)
+    params = dict(bw=16, bh=16, bt=3, pfactor=1.0, pframe=3, l=8, t=4, r=0, b=12)
+    clip = nf.Clip([[to_device(p) for p in fr] for fr in frames], color_family="yuv", bits=8, subsampling=(1, 1))
+    out = nf.FFT3D(clip, **params)
+    oparams = dict(bw=16, bh=16, bt=3, pfactor=1.0)
+    rois = [(8, 4, w - 8, h - 4 - 12), (4, 2, w // 2 - 4, h // 2 - 2 - 6), (4, 2, w // 2 - 4, h // 2 - 2 - 6)]
+    models = []
+    for p, (l, t, rw, rh) in enumerate(rois):
+        _, _, m = oracle.fft3d_pattern(frames[3][p][t:t + rh, l:l + rw], bits=8, chroma=p > 0, **oparams)
+        models.append(m)
+    for n in (0, 2, N - 1):
+        assert 3 in out.requests(n)
+        got = out.get_frame(n)
+        slots = [n] if n in (0, N - 1) else [n - 1, n, n + 1]
+        for p, (l, t, rw, rh) in enumerate(rois):
+            ref = frames[n][p].copy()
+            ref[t:t + rh, l:l + rw] = oracle.fft3d_plane([frames[s][p][t:t + rh, l:l + rw] for s in slots], bits=8,
+                                                         chroma=p > 0, pattern=models[p], **oparams)
+            assert np.array_equal(to_host(got[p], np.uint8), ref), (n, p)
