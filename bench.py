@@ -35,9 +35,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--resident", type=int, default=8, help="distinct resident source frames")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-frames", type=int, default=2)
+    ap.add_argument("--cpu-frames", type=int, default=300)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-dfttest", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other workload figures")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run 5 device copies of this many MiB (counter calibration) and exit")
     return ap.parse_args()
@@ -198,12 +199,118 @@ def main():
             "achieved_gbs": round(BYTES_PER_FRAME / (dk_ms / max(1, dk_n) / 1e3) / 1e9, 2),
         }
 
+    if not args.no_configs:
+        result["configs"] = other_configs(nf, torch, clip, stream, flags, args)
+
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(clip, args)
         if result["cpu_baseline"].get("value"):
             result["gpu_over_cpu"] = round(fps / result["cpu_baseline"]["value"], 1)
     print(json.dumps(result))
     return 0
+
+
+def _timed(torch, stream, fn, steps):
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0.record(stream)
+    for i in range(steps):
+        fn(i)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    return t0.elapsed_time(t1)
+
+
+def other_configs(nf, torch, clip, stream, flags, args):
+    """The remaining workloads, each timed on device-resident synthetic input
+    after a short warm-up: frames/s (or batches/s) and the kernel share."""
+    import numpy as np
+    from synthetic import gray8_textured
+    out = {}
+    dev = torch.device("cuda:0")
+
+    # 640x360 GRAY8 DFTTest (tbsize=1, sbsize=16, sosize=12, Hann, sigma=8)
+    g = [torch.from_numpy(f).to(dev) for f in gray8_textured(640, 360, 24, seed=1)]
+    plan = nf.dfttest_plan(640, 360, bits=8, **DFT_PARAMS)
+    gd = torch.empty_like(g[0])
+
+    def c1(i):
+        nf.process_batch([(plan, [g[i % 24]], gd)], flags, stream=stream)
+    for i in range(3):
+        c1(i)
+    ms = _timed(torch, stream, c1, 96)
+    out["dfttest_gray8_640x360"] = {"frames_per_s": round(96 / (ms / 1e3), 1), "ms_per_frame": round(ms / 96, 4),
+                                    "bytes_per_frame": 460800,
+                                    "achieved_gbs": round(460800 / (ms / 96 / 1e3) / 1e9, 2)}
+
+    # 1080p YUV420P10 views of the resident clip (samples >> 6)
+    p10 = [[(f.to(torch.int32) & 0xFFFF).div(64, rounding_mode="floor").to(torch.int16) for f in fr]
+           for fr in _resident_frames(torch, clip, dev)]
+    dims = [(W, H), (W // 2, H // 2), (W // 2, H // 2)]
+    outs = [torch.empty_like(p10[0][p]) for p in range(3)]
+
+    # FFT3D Kalman (bt=0, bw=bh=32, ow=oh default, sigma=2, kratio=2), sequential
+    kplans = [nf.fft3d_plan(w, h, bits=10, chroma=p > 0, bt=0, sigma=2.0) for p, (w, h) in enumerate(dims)]
+    states = [nf.KalmanState(kp) for kp in kplans]
+
+    def c3a(i):
+        fr = p10[i % len(p10)]
+        for p in range(3):
+            states[p].enqueue_advance(fr[p], flags, stream=stream)
+        for p in range(3):
+            states[p].enqueue_render(fr[p], outs[p], flags, stream=stream)
+    for i in range(3):
+        c3a(i)
+    steps = max(20, args.steps // 4)
+    ms = _timed(torch, stream, c3a, steps)
+    state_bytes = sum(st.nbytes for st in states)
+    kbytes = 2 * (W * H + 2 * (W // 2) * (H // 2)) * 2 + 2 * state_bytes
+    out["fft3d_kalman_1080p_p10"] = {"frames_per_s": round(steps / (ms / 1e3), 1), "ms_per_frame": round(ms / steps, 4),
+                                     "bytes_per_frame": kbytes,
+                                     "achieved_gbs": round(kbytes / (ms / steps / 1e3) / 1e9, 1),
+                                     "hbm_frac": round(kbytes / (ms / steps / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    # DFTTest tbsize=5 tmode=1 tosize=0, one pass per lattice block writes 5 frames
+    vparams = dict(tbsize=5, tmode=1, tosize=0, sbsize=16, sosize=12, ftype=0, sigma=8.0)
+    vplans = [nf.dfttest_plan(w, h, bits=10, chroma=p > 0, **vparams) for p, (w, h) in enumerate(dims)]
+    vouts = [[torch.empty_like(p10[0][p]) for _ in range(5)] for p in range(3)]
+
+    def c3b(i):
+        for p in range(3):
+            nf.dfttest_volume(vplans[p], [p10[(i + z) % len(p10)][p] for z in range(5)], vouts[p], flags,
+                              stream=stream)
+    c3b(0)
+    vsteps = max(4, args.steps // 20)
+    ms = _timed(torch, stream, c3b, vsteps)
+    out["dfttest_t5_tmode1_1080p_p10"] = {"frames_per_s": round(5 * vsteps / (ms / 1e3), 1),
+                                          "ms_per_frame": round(ms / (5 * vsteps), 4)}
+
+    # 16 float32 256x256 patches: Wiener forward + adjoint per patch
+    wplan = nf.dfttest_plan(256, 256, bits=32, floating=True, tbsize=1, sbsize=16, sosize=12, ftype=0, sigma=8.0)
+    gen = torch.Generator(device=dev).manual_seed(5)
+    xs = torch.rand(16, 256, 256, device=dev, generator=gen)
+    ys = torch.empty_like(xs)
+    gx = torch.empty_like(xs)
+    table = torch.full((16 * 9,), 8.0 * 12.0, device=dev)
+    gt = torch.empty_like(table)
+
+    def c5(i):
+        for k in range(16):
+            nf.wiener_forward(wplan, xs[k], ys[k], flags, table=table, stream=stream)
+            nf.wiener_backward(wplan, xs[k], ys[k], flags, grad_source=gx[k], table=table, grad_table=gt,
+                               stream=stream)
+    c5(0)
+    ms = _timed(torch, stream, c5, 10)
+    out["wiener_adjoint_16x256x256"] = {"batches_per_s": round(10 / (ms / 1e3), 1), "ms_per_batch": round(ms / 10, 4)}
+    if int(flags.item()) != 0:
+        out["error"] = "non-finite flag raised"
+    return out
+
+
+def _resident_frames(torch, clip, dev):
+    import numpy as np
+    return [[torch.from_numpy(np.ascontiguousarray(p).view(np.int16)).to(dev) for p in fr] for fr in clip]
 
 
 def cpu_baseline(clip, args):

@@ -210,6 +210,13 @@ nf_status nf_kalman_advance(const nf_plan* plan, nf_plane source, nf_kalman_stat
                             void* stream);
 nf_status nf_kalman_render(const nf_plan* plan, nf_plane source, nf_mut_plane dst,
                            const nf_kalman_state* state, void* stream);
+/* Asynchronous forms of the two calls above: error bits are OR-ed into the
+ * caller's device word (as nf_plan_enqueue); for replay loops that advance
+ * several frames and planes before one synchronisation. */
+nf_status nf_kalman_enqueue_advance(const nf_plan* plan, nf_plane source, nf_kalman_state* state,
+                                    unsigned* device_flags, void* stream);
+nf_status nf_kalman_enqueue_render(const nf_plan* plan, nf_plane source, nf_mut_plane dst,
+                                   const nf_kalman_state* state, unsigned* device_flags, void* stream);
 /* Host access to the state (tests): last as interleaved complex, cov, proc. */
 nf_status nf_kalman_state_download(const nf_kalman_state* state, float* last, float* cov,
                                    float* proc);

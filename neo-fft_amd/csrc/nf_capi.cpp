@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -133,6 +134,19 @@ struct nf_plan {
   const float2* d_grid = nullptr;
   int tile_w = 64, tile_h = 64, nbx_max = 0, ww_max = 0;
   size_t lds = 0;
+  // Launch-shape candidates (DFTTest tbsize=1: output tiles; FFT3D: blocks
+  // per workgroup). Every candidate gives bit-identical results; the first
+  // call on a plan times them and keeps the fastest (NF_AUTOTUNE=0 keeps
+  // candidate 0; NF_DFT_TILE_W/H and NF_FFT3D_NB pin a single candidate).
+  struct DftTile {
+    int tile_w, tile_h, nbx_max, ww_max;
+    size_t lds;
+  };
+  std::vector<DftTile> dft_tiles;
+  std::vector<int> fft3d_nbs;
+  mutable std::atomic<int> choice{0};
+  mutable std::atomic<int> tuned{0};
+  mutable std::mutex tune_mu;
   int tmode = 0, tosize = 0;
   const float* d_ttw = nullptr;
   // temporal kernel tiling per mode: 0 per-frame ring, 1 per-frame tile (several
@@ -270,19 +284,38 @@ Curves curves_of(const nf_dfttest_config& c) {
   return k;
 }
 
+nf_plan::DftTile dft_tile(int B, int S, int tw, int th) {
+  nf_plan::DftTile t{};
+  t.tile_w = tw;
+  t.tile_h = th;
+  t.nbx_max = (tw + B - 2) / S + 1;
+  t.ww_max = odd_stride((t.nbx_max - 1) * S + B);
+  DftJob j{};
+  j.nbx_max = t.nbx_max;
+  j.ww_max = t.ww_max;
+  j.tile_w = tw;
+  t.lds = dfttest_t1_lds_bytes(B, j);
+  return t;
+}
+
 void choose_dft_tiling(nf_plan& p) {
   const char* tw = std::getenv("NF_DFT_TILE_W");
   const char* th = std::getenv("NF_DFT_TILE_H");
-  p.tile_w = tw ? std::max(8, std::atoi(tw)) : 64;
-  p.tile_h = th ? std::max(8, std::atoi(th)) : 128;
-  p.nbx_max = (p.tile_w + p.B - 2) / p.S + 1;
-  p.ww_max = odd_stride((p.nbx_max - 1) * p.S + p.B);
-  DftJob j{};
-  j.nbx_max = p.nbx_max;
-  j.ww_max = p.ww_max;
-  j.tile_w = p.tile_w;
-  p.lds = dfttest_t1_lds_bytes(p.B, j);
-  require(p.lds <= 160 * 1024, "DFTTest tile does not fit the LDS");
+  if (tw || th) {
+    p.dft_tiles.push_back(dft_tile(p.B, p.S, tw ? std::max(8, std::atoi(tw)) : 64, th ? std::max(8, std::atoi(th)) : 128));
+  } else {
+    static const int cand[][2] = {{64, 128}, {64, 64}, {32, 64}, {32, 32}, {128, 64}, {128, 128}};
+    for (const auto& c : cand) {
+      const auto t = dft_tile(p.B, p.S, c[0], c[1]);
+      if (t.lds <= 160 * 1024) p.dft_tiles.push_back(t);
+    }
+  }
+  require(!p.dft_tiles.empty() && p.dft_tiles[0].lds <= 160 * 1024, "DFTTest tile does not fit the LDS");
+  p.tile_w = p.dft_tiles[0].tile_w;
+  p.tile_h = p.dft_tiles[0].tile_h;
+  p.nbx_max = p.dft_tiles[0].nbx_max;
+  p.ww_max = p.dft_tiles[0].ww_max;
+  p.lds = p.dft_tiles[0].lds;
   if (p.temporal > 1) {
     // Three launch shapes: 0 = one covering temporal block (ring
     // accumulator), 1 = several covering blocks (tile accumulator), 2 = all
@@ -595,8 +628,16 @@ extern "C" nf_status nf_fft3d_plan_create(int width, int height, nf_sample_forma
     p->lower = (c.beta - 1.0f) / c.beta;
     p->base = (!fmt.floating && fmt.chroma) ? float(1 << (fmt.bits - 1)) : 0.0f;
     p->peak = fmt.floating ? 1.0f : float((1 << fmt.bits) - 1);
-    p->NB = c.bw == 32 ? 2 : c.bw == 16 ? 4 : 8;
-    if (const char* nb = std::getenv("NF_FFT3D_NB")) p->NB = std::max(1, std::atoi(nb));
+    if (const char* nb = std::getenv("NF_FFT3D_NB")) {
+      p->fft3d_nbs.push_back(std::max(1, std::atoi(nb)));
+    } else {
+      const int first = c.bw == 32 ? 2 : c.bw == 16 ? 4 : 8;
+      p->fft3d_nbs.push_back(first);
+      for (int nb : {1, 2, 3, 4, 8})
+        if (nb != first && fft3d_block_lds_bytes(c.bw, c.bh, p->temporal, nb, p->gx.step) <= 160 * 1024)
+          p->fft3d_nbs.push_back(nb);
+    }
+    p->NB = p->fft3d_nbs[0];
     p->groups_x = (p->gx.count + p->NB - 1) / p->NB;
     p->lds = fft3d_block_lds_bytes(c.bw, c.bh, p->temporal, p->NB, p->gx.step);
     require(p->lds <= 160 * 1024, "FFT3D block group does not fit the LDS");
@@ -793,13 +834,97 @@ void timed(int cls, hipStream_t stream, Launch&& launch) {
   if (t_timer.pending.size() > 4096) t_timer.drain();
 }
 
+DftJob apply_tile(DftJob j, const nf_plan& p, int c) {
+  const auto& t = p.dft_tiles[c];
+  j.tile_w = t.tile_w;
+  j.tile_h = t.tile_h;
+  j.tile_x0 = p.gx.offset / t.tile_w;
+  j.tile_y0 = p.gy.offset / t.tile_h;
+  j.tiles_x = (p.gx.offset + p.gx.length + t.tile_w - 1) / t.tile_w - j.tile_x0;
+  j.tiles_y = (p.gy.offset + p.gy.length + t.tile_h - 1) / t.tile_h - j.tile_y0;
+  j.nbx_max = t.nbx_max;
+  j.ww_max = t.ww_max;
+  return j;
+}
+
+Fft3dJob apply_nb(Fft3dJob j, const nf_plan& p, int c) {
+  const int nb = p.fft3d_nbs[c];
+  j.blocks_per_group = nb;
+  j.groups_x = (p.gx.count + nb - 1) / nb;
+  return j;
+}
+
+bool autotune_enabled(hipStream_t stream) {
+  const char* e = std::getenv("NF_AUTOTUNE");
+  if (e && std::atoi(e) == 0) return false;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(stream, &st) != hipSuccess || st != hipStreamCaptureStatusNone) return false;
+  return true;
+}
+
+// Times every candidate with `run(c)` (one launch of the group's jobs, the
+// candidate applied to each) and publishes the fastest to every plan of the
+// group that is not tuned yet. Outputs of all candidates are identical.
+template <class Run>
+void autotune(const std::vector<const nf_plan*>& group, int ncand, hipStream_t stream, Run&& run) {
+  bool pending = false;
+  for (auto* p : group) pending = pending || !p->tuned.load();
+  if (ncand <= 1 || !pending || !autotune_enabled(stream)) return;
+  std::lock_guard<std::mutex> lock(group[0]->tune_mu);
+  if (group[0]->tuned.load() && group.size() == 1) return;
+  hipEvent_t a, b;
+  hip_check(hipEventCreate(&a), "hipEventCreate");
+  hip_check(hipEventCreate(&b), "hipEventCreate");
+  int best = 0;
+  float best_ms = 0;
+  for (int c = 0; c < ncand; ++c) {
+    run(c);  // warm-up
+    hip_check(hipEventRecord(a, stream), "hipEventRecord");
+    for (int r = 0; r < 3; ++r) run(c);
+    hip_check(hipEventRecord(b, stream), "hipEventRecord");
+    hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+    if (c == 0 || ms < best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  for (auto* p : group)
+    if (!p->tuned.load()) {
+      p->choice.store(best);
+      p->tuned.store(1);
+    }
+}
+
+void launch_dft_group(const std::vector<const nf_plan*>& plans, const std::vector<DftJob>& jobs, size_t first,
+                      size_t n, unsigned* flags, hipStream_t stream);
+
 // DFTTest: jobs share B; up to kMaxInlineJobs per launch.
-void enqueue_dft_jobs(const std::vector<const nf_plan*>& plans, const std::vector<DftJob>& jobs, unsigned* flags,
+void enqueue_dft_jobs(const std::vector<const nf_plan*>& plans, std::vector<DftJob> jobs, unsigned* flags,
                       hipStream_t stream) {
   ensure_lds_limits();
   for (size_t first = 0; first < jobs.size(); first += kMaxInlineJobs) {
-    DftLaunch L{};
     const size_t n = std::min<size_t>(kMaxInlineJobs, jobs.size() - first);
+    const std::vector<const nf_plan*> group(plans.begin() + first, plans.begin() + first + n);
+    size_t ncand = group[0]->dft_tiles.size();
+    for (auto* p : group) ncand = std::min(ncand, p->dft_tiles.size());
+    autotune(group, (int)ncand, stream, [&](int c) {
+      std::vector<DftJob> trial(jobs.begin() + first, jobs.begin() + first + n);
+      for (size_t i = 0; i < n; ++i) trial[i] = apply_tile(trial[i], *group[i], c);
+      launch_dft_group(group, trial, 0, n, flags, stream);
+    });
+    for (size_t i = first; i < first + n; ++i) jobs[i] = apply_tile(jobs[i], *plans[i], plans[i]->choice.load());
+    launch_dft_group(plans, jobs, first, n, flags, stream);
+  }
+}
+
+void launch_dft_group(const std::vector<const nf_plan*>& plans, const std::vector<DftJob>& jobs, size_t first,
+                      size_t n, unsigned* flags, hipStream_t stream) {
+  {
+    DftLaunch L{};
     int tiles = 0;
     size_t lds = 0;
     const int B = plans[first]->B;
@@ -808,7 +933,7 @@ void enqueue_dft_jobs(const std::vector<const nf_plan*>& plans, const std::vecto
       L.jobs[i] = jobs[first + i];
       L.tile_start[i] = tiles;
       tiles += L.jobs[i].tiles_x * L.jobs[i].tiles_y;
-      lds = std::max(lds, plans[first + i]->lds);
+      lds = std::max(lds, dfttest_t1_lds_bytes(B, L.jobs[i]));
     }
     L.tile_start[n] = tiles;
     L.njobs = (int)n;
@@ -910,6 +1035,36 @@ void enqueue_fft3d_jobs(const std::vector<const nf_plan*>& plans, std::vector<Ff
   if (total) hip_check(hipMallocAsync((void**)&contrib, total * sizeof(float), stream), "hipMallocAsync(contrib)");
   for (size_t i = 0; i < jobs.size(); ++i) jobs[i].contrib = contrib ? contrib + off[i] : nullptr;
   for (size_t first = 0; first < jobs.size(); first += kMaxInlineJobs) {
+    // tuned on stateless block passes only (Kalman state updates must run once)
+    const size_t n = std::min<size_t>(kMaxInlineJobs, jobs.size() - first);
+    bool stateless = true;
+    std::vector<const nf_plan*> group(plans.begin() + first, plans.begin() + first + n);
+    size_t ncand = group[0]->fft3d_nbs.size();
+    for (size_t i = first; i < first + n; ++i) {
+      stateless = stateless && jobs[i].mode == 0 && jobs[i].T == plans[i]->temporal;
+      ncand = std::min(ncand, plans[i]->fft3d_nbs.size());
+    }
+    if (stateless)
+      autotune(group, (int)ncand, stream, [&](int c) {
+        Fft3dLaunch L{};
+        int groups = 0;
+        size_t lds = 0;
+        for (size_t i = 0; i < n; ++i) {
+          const nf_plan& p = *group[i];
+          L.jobs[i] = apply_nb(jobs[first + i], p, c);
+          L.group_start[i] = groups;
+          groups += L.jobs[i].groups_x * p.gy.count;
+          lds = std::max(lds, fft3d_block_lds_bytes(p.bw, p.bh, jobs[first + i].T, L.jobs[i].blocks_per_group,
+                                                    p.gx.step));
+        }
+        L.group_start[n] = groups;
+        L.njobs = (int)n;
+        L.flags = flags;
+        hip_check(launch_fft3d_block(group[0]->bw, group[0]->bh, L, groups, lds, stream), "fft3d block kernel launch");
+      });
+    for (size_t i = first; i < first + n; ++i) jobs[i] = apply_nb(jobs[i], *plans[i], plans[i]->choice.load());
+  }
+  for (size_t first = 0; first < jobs.size(); first += kMaxInlineJobs) {
     Fft3dLaunch L{};
     const size_t n = std::min<size_t>(kMaxInlineJobs, jobs.size() - first);
     int groups = 0;
@@ -922,10 +1077,10 @@ void enqueue_fft3d_jobs(const std::vector<const nf_plan*>& plans, std::vector<Ff
       L.jobs[i] = jobs[first + i];
       L.group_start[i] = groups;
       L.pixel_start[i] = pixels;
-      groups += p.groups_x * p.gy.count;
+      groups += L.jobs[i].groups_x * p.gy.count;
       pixels += (long long)p.width * p.height;
-      lds = std::max(lds, fft3d_block_lds_bytes(bw, bh, jobs[first + i].mode == 0 ? jobs[first + i].T : 1, p.NB,
-                                                p.gx.step));
+      lds = std::max(lds, fft3d_block_lds_bytes(bw, bh, jobs[first + i].mode == 0 ? jobs[first + i].T : 1,
+                                                L.jobs[i].blocks_per_group, p.gx.step));
     }
     L.group_start[n] = groups;
     L.pixel_start[n] = pixels;
@@ -1408,41 +1563,70 @@ extern "C" nf_status nf_kalman_state_download(const nf_kalman_state* s, float* l
   });
 }
 
+namespace {
+void enqueue_kalman_advance(const nf_plan* plan, nf_plane source, nf_kalman_state* state, unsigned* flags,
+                            hipStream_t stream) {
+  require(plan && plan->algorithm == 1 && plan->kalman, "not a Kalman plan");
+  require(state && state->plan && state->bins == (size_t)plan->gx.count * plan->gy.count * plan->bh * (plan->bw / 2 + 1),
+          "Kalman state shape differs");
+  require(flags != nullptr, "device flag word required");
+  check_plane(*plan, source.width, source.height, source.stride_bytes, source.data);
+  Fft3dJob j = make_fft3d_job(*plan, 1, 1);
+  j.src[0] = source.data;
+  j.src_stride[0] = source.stride_bytes;
+  j.k_last = state->last;
+  j.k_cov = state->cov;
+  j.k_proc = state->proc;
+  enqueue_fft3d_jobs({plan}, {j}, flags, stream, false);
+}
+
+void enqueue_kalman_render(const nf_plan* plan, nf_plane source, nf_mut_plane dst, const nf_kalman_state* state,
+                           unsigned* flags, hipStream_t stream) {
+  require(plan && plan->algorithm == 1 && plan->kalman, "not a Kalman plan");
+  require(state && state->bins == (size_t)plan->gx.count * plan->gy.count * plan->bh * (plan->bw / 2 + 1),
+          "Kalman checkpoint shape differs");
+  require(flags != nullptr, "device flag word required");
+  check_plane(*plan, source.width, source.height, source.stride_bytes, source.data);
+  check_plane(*plan, dst.width, dst.height, dst.stride_bytes, dst.data);
+  Fft3dJob j = make_fft3d_job(*plan, 1, 2);
+  j.k_last = state->last;
+  j.k_cov = state->cov;
+  j.k_proc = state->proc;
+  j.dst = dst.data;
+  j.dst_stride = dst.stride_bytes;
+  enqueue_fft3d_jobs({plan}, {j}, flags, stream, true);
+}
+}  // namespace
+
 extern "C" nf_status nf_kalman_advance(const nf_plan* plan, nf_plane source, nf_kalman_state* state, void* stream) {
   return guarded([&]() -> nf_status {
-    require(plan && plan->algorithm == 1 && plan->kalman, "not a Kalman plan");
-    require(state && state->plan && state->bins == (size_t)plan->gx.count * plan->gy.count * plan->bh * (plan->bw / 2 + 1),
-            "Kalman state shape differs");
-    check_plane(*plan, source.width, source.height, source.stride_bytes, source.data);
-    return run_sync((hipStream_t)stream, [&](unsigned* flag) {
-      Fft3dJob j = make_fft3d_job(*plan, 1, 1);
-      j.src[0] = source.data;
-      j.src_stride[0] = source.stride_bytes;
-      j.k_last = state->last;
-      j.k_cov = state->cov;
-      j.k_proc = state->proc;
-      enqueue_fft3d_jobs({plan}, {j}, flag, (hipStream_t)stream, false);
-    });
+    return run_sync((hipStream_t)stream,
+                    [&](unsigned* flag) { enqueue_kalman_advance(plan, source, state, flag, (hipStream_t)stream); });
   });
 }
 
 extern "C" nf_status nf_kalman_render(const nf_plan* plan, nf_plane source, nf_mut_plane dst,
                                       const nf_kalman_state* state, void* stream) {
   return guarded([&]() -> nf_status {
-    require(plan && plan->algorithm == 1 && plan->kalman, "not a Kalman plan");
-    require(state && state->bins == (size_t)plan->gx.count * plan->gy.count * plan->bh * (plan->bw / 2 + 1),
-            "Kalman checkpoint shape differs");
-    check_plane(*plan, source.width, source.height, source.stride_bytes, source.data);
-    check_plane(*plan, dst.width, dst.height, dst.stride_bytes, dst.data);
     return run_sync((hipStream_t)stream, [&](unsigned* flag) {
-      Fft3dJob j = make_fft3d_job(*plan, 1, 2);
-      j.k_last = state->last;
-      j.k_cov = state->cov;
-      j.k_proc = state->proc;
-      j.dst = dst.data;
-      j.dst_stride = dst.stride_bytes;
-      enqueue_fft3d_jobs({plan}, {j}, flag, (hipStream_t)stream, true);
+      enqueue_kalman_render(plan, source, dst, state, flag, (hipStream_t)stream);
     });
+  });
+}
+
+extern "C" nf_status nf_kalman_enqueue_advance(const nf_plan* plan, nf_plane source, nf_kalman_state* state,
+                                               unsigned* device_flags, void* stream) {
+  return guarded([&]() -> nf_status {
+    enqueue_kalman_advance(plan, source, state, device_flags, (hipStream_t)stream);
+    return NF_OK;
+  });
+}
+
+extern "C" nf_status nf_kalman_enqueue_render(const nf_plan* plan, nf_plane source, nf_mut_plane dst,
+                                              const nf_kalman_state* state, unsigned* device_flags, void* stream) {
+  return guarded([&]() -> nf_status {
+    enqueue_kalman_render(plan, source, dst, state, device_flags, (hipStream_t)stream);
+    return NF_OK;
   });
 }
 

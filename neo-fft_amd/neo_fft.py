@@ -129,6 +129,10 @@ _SIGNATURES = {
     "nf_kalman_advance": (ctypes.c_int, [ctypes.c_void_p, Plane, ctypes.c_void_p, ctypes.c_void_p]),
     "nf_kalman_render": (ctypes.c_int, [ctypes.c_void_p, Plane, MutPlane, ctypes.c_void_p, ctypes.c_void_p]),
     "nf_kalman_state_download": (ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p, c_float_p]),
+    "nf_kalman_enqueue_advance": (ctypes.c_int, [ctypes.c_void_p, Plane, ctypes.c_void_p, ctypes.c_void_p,
+                                                 ctypes.c_void_p]),
+    "nf_kalman_enqueue_render": (ctypes.c_int, [ctypes.c_void_p, Plane, MutPlane, ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_void_p]),
     "nf_kernel_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
     "nf_filter_create_dfttest": (ctypes.c_int, [ctypes.POINTER(VideoInfo), ctypes.POINTER(DFTTestConfig), c_int_p,
                                                 ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
@@ -425,6 +429,14 @@ class KalmanState:
     def render(self, source, dst, stream=None):
         _check(lib().nf_kalman_render(self.plan.handle, _plane(source), _plane(dst, MutPlane), self._h,
                                       _stream_handle(stream)))
+
+    def enqueue_advance(self, source, flags, stream=None):
+        _check(lib().nf_kalman_enqueue_advance(self.plan.handle, _plane(source), self._h,
+                                               ctypes.c_void_p(flags.data_ptr()), _stream_handle(stream)))
+
+    def enqueue_render(self, source, dst, flags, stream=None):
+        _check(lib().nf_kalman_enqueue_render(self.plan.handle, _plane(source), _plane(dst, MutPlane), self._h,
+                                              ctypes.c_void_p(flags.data_ptr()), _stream_handle(stream)))
 
     def download(self):
         import numpy as np

@@ -300,3 +300,29 @@ def test_dfttest_dither_temporal(nf, oracle):
     ref = oracle.dfttest_plane(frames, bits=8, targets=[2], frame=7, plane=0, **params)
     got = _run(plan, frames, targets=[2], frame=7, plane=0)
     assert np.array_equal(got, ref), compare(got, ref)
+
+
+@pytest.mark.parametrize("tile", [(32, 32), (64, 64), (128, 64), (64, 128), (48, 80)])
+def test_dfttest_launch_shape_invariance(nf, oracle, monkeypatch, tile):
+    """Every output-tile shape gives the same bits (canonical overlap-add)."""
+    monkeypatch.setenv("NF_DFT_TILE_W", str(tile[0]))
+    monkeypatch.setenv("NF_DFT_TILE_H", str(tile[1]))
+    rng = np.random.default_rng(77)
+    img = _rand_plane(rng, 150, 230, 16, False)
+    params = dict(tbsize=1, sbsize=16, sosize=12, ftype=0, sigma=6.0)
+    plan = nf.dfttest_plan(230, 150, bits=16, **params)
+    got = _run(plan, [img])
+    ref = oracle.dfttest_plane([img], bits=16, **params)
+    assert np.array_equal(got, ref), compare(got, ref)
+
+
+@pytest.mark.parametrize("nb", [1, 2, 3, 4, 8])
+def test_fft3d_launch_shape_invariance(nf, oracle, monkeypatch, nb):
+    monkeypatch.setenv("NF_FFT3D_NB", str(nb))
+    rng = np.random.default_rng(78)
+    frames = [_rand_plane(rng, 140, 210, 8, False) for _ in range(3)]
+    params = dict(bw=32, bh=32, ow=16, oh=16, bt=3, sigma=4.0)
+    plan = nf.fft3d_plan(210, 140, bits=8, **params)
+    got = _run(plan, frames)
+    ref = oracle.fft3d_plane(frames, bits=8, **params)
+    assert np.array_equal(got, ref), compare(got, ref)
