@@ -172,6 +172,17 @@ nf_status nf_dfttest_wiener_backward(const nf_plan* plan, nf_plane source, nf_pl
                                      nf_mut_plane grad_source, const float* table, float* grad_table,
                                      unsigned* device_flags, void* stream);
 
+/* Block-transform form of a plan. NF_TRANSFORM_RADIX2 (default) evaluates
+ * the reference's radix-2 codelets in the same binary32 operation order
+ * (bit-identical results). NF_TRANSFORM_MFMA_F16X3 evaluates each spatial
+ * block DFT as dense DFT-matrix products on the matrix cores with operands
+ * split into FP16 high/low parts and FP32 accumulation: not bit-identical,
+ * error against binary64 reported by the tests (DFTTest, sbsize 16, tbsize
+ * 1/3/5 with tmode 0). No counterpart in the reference. */
+#define NF_TRANSFORM_RADIX2 0
+#define NF_TRANSFORM_MFMA_F16X3 1
+nf_status nf_plan_set_transform(nf_plan* plan, int transform);
+
 /* Sampled noise models. A DFTTest plan with nlocation (and ftype < 2) and an
  * FFT3D plan with pfactor > 0 (and bt != -1, pshow off) filter with a model
  * estimated from source frames; processing before the model is published

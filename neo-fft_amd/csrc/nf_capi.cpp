@@ -65,6 +65,7 @@ void ensure_lds_limits() {
     if (g_lds_status == hipSuccess) g_lds_status = dither_set_lds_limit();
     if (g_lds_status == hipSuccess) g_lds_status = dfttest_adjoint_set_lds_limit();
     if (g_lds_status == hipSuccess) g_lds_status = fft3d_set_lds_limit();
+    if (g_lds_status == hipSuccess) g_lds_status = dfttest_mfma_set_lds_limit();
   });
   hip_check(g_lds_status, "hipFuncSetAttribute");
 }
@@ -172,6 +173,9 @@ struct nf_plan {
   int px = 0, py = 0;
   float pfactor = 0;
   const float* d_weights = nullptr;
+  // block transform form (nf_plan_set_transform) and its twiddle table
+  int transform = NF_TRANSFORM_RADIX2;
+  const float* d_tw = nullptr;
 };
 
 struct nf_kalman_state {
@@ -421,6 +425,13 @@ extern "C" nf_status nf_dfttest_plan_create(int width, int height, nf_sample_for
       require(grid[0].real() != 0, "DFTTest zmean requires nonzero template DC");
     }
     const size_t o_ttw = p->tables.add(temporal_twiddles(T));
+    std::vector<float> tw(2 * (size_t)B);
+    for (int k = 0; k < B; ++k) {
+      const double ang = 2.0 * 3.1415926535897932384626433832795 * double(k) / double(B);
+      tw[2 * k] = float(std::cos(ang));
+      tw[2 * k + 1] = float(std::sin(ang));
+    }
+    const size_t o_tw = p->tables.add(tw);
     const size_t o_h = p->tables.add(win.h), o_hs = p->tables.add(hsyn);
     const size_t o_pr = primary.empty() ? 0 : p->tables.add(primary);
     const size_t o_g = grid.empty() ? 0 : p->tables.add(grid);
@@ -433,6 +444,7 @@ extern "C" nf_status nf_dfttest_plan_create(int width, int height, nf_sample_for
       p->model.alloc((size_t)T * B * (B / 2 + 1));
     }
     p->d_ttw = p->tables.at<float>(o_ttw);
+    p->d_tw = p->tables.at<float>(o_tw);
     p->d_h = p->tables.at<float>(o_h);
     p->d_hsyn = p->tables.at<float>(o_hs);
     g.primary = p->sampled ? p->model.ptr : primary.empty() ? nullptr : p->tables.at<float>(o_pr);
@@ -1154,6 +1166,34 @@ void enqueue_fft3d_preview(const nf_plan& p, const nf_plane& src, const nf_mut_p
             "fft3d preview store launch");
 }
 
+// DFTTest through the matrix-core form: jobs share B and tbsize.
+void enqueue_dft_mfma_jobs(const std::vector<const nf_plan*>& plans, const std::vector<DftJob>& jobs,
+                           const std::vector<DftTemporal>& tq, unsigned* flags, hipStream_t stream) {
+  ensure_lds_limits();
+  for (size_t first = 0; first < jobs.size(); first += kMaxVolJobs) {
+    DftVolLaunch L{};
+    const size_t n = std::min<size_t>(kMaxVolJobs, jobs.size() - first);
+    int tiles = 0;
+    size_t lds = 0;
+    const int B = plans[first]->B, T = plans[first]->temporal;
+    for (size_t i = 0; i < n; ++i) {
+      const nf_plan& p = *plans[first + i];
+      require(p.B == B && p.temporal == T, "batched matrix-core DFTTest jobs must share sbsize and tbsize");
+      L.jobs[i] = apply_tile(jobs[first + i], p, 0);
+      L.tq[i] = tq[first + i];
+      L.tile_start[i] = tiles;
+      tiles += L.jobs[i].tiles_x * L.jobs[i].tiles_y;
+      lds = std::max(lds, dfttest_mfma_lds_bytes(B, T, L.jobs[i].nbx_max, L.jobs[i].ww_max, L.jobs[i].tile_w));
+    }
+    L.tile_start[n] = tiles;
+    L.njobs = (int)n;
+    L.flags = flags;
+    timed(0, stream, [&] {
+      hip_check(launch_dfttest_mfma(B, T, L, tiles, lds, plans[first]->d_tw, stream), "dfttest mfma kernel launch");
+    });
+  }
+}
+
 nf_status enqueue_one(const nf_plan* plan, const nf_plane* sources, int n_sources, const int* targets, int n_targets,
                       nf_mut_plane dst, int frame, int plane, unsigned* flags, hipStream_t stream) {
   require(plan && sources && n_sources > 0, "sources must not be empty");
@@ -1161,7 +1201,11 @@ nf_status enqueue_one(const nf_plan* plan, const nf_plane* sources, int n_source
   check_plane(*plan, dst.width, dst.height, dst.stride_bytes, dst.data);
   for (int i = 0; i < n_sources; ++i)
     check_plane(*plan, sources[i].width, sources[i].height, sources[i].stride_bytes, sources[i].data);
-  if (plan->algorithm == 0) {
+  if (plan->algorithm == 0 && plan->transform == NF_TRANSFORM_MFMA_F16X3) {
+    require(n_targets == 0, "invalid temporal block inputs");
+    const DftJob j = make_dft_job(*plan, sources[0], dst);
+    enqueue_dft_mfma_jobs({plan}, {j}, {make_temporal(*plan, sources, n_sources, nullptr, 0, 0)}, flags, stream);
+  } else if (plan->algorithm == 0) {
     DftJob j = make_dft_job(*plan, sources[0], dst);
     // 8-bit dither (plan.cpp:619-623): the float crop goes through a
     // stream-ordered buffer into the wavefront scan
@@ -1255,7 +1299,7 @@ extern "C" nf_status nf_process_batch(const nf_plan* const* plans, int n_jobs, c
         require(!(p->dither > 0 && p->storage == kU8), "dithered DFTTest planes use nf_plan_enqueue");
         require(ns == p->temporal, "DFTTest sources size must match plan temporal_size");
         dj.push_back(make_dft_job(*p, s[0], dsts[i]));
-        if (p->temporal > 1) tq.push_back(make_temporal(*p, s, ns, nullptr, 0, 0));
+        if (p->temporal > 1 || p->transform == NF_TRANSFORM_MFMA_F16X3) tq.push_back(make_temporal(*p, s, ns, nullptr, 0, 0));
       } else {
         require(!p->kalman, "Kalman plans use nf_kalman_advance / nf_kalman_render");
         require(ns == p->temporal || ns == 1, "FFT3D sources size must match plan temporal_size or single-frame fallback");
@@ -1276,8 +1320,13 @@ extern "C" nf_status nf_process_batch(const nf_plan* const* plans, int n_jobs, c
     }
     if (alg == 0) {
       const bool temporal = pv[0]->temporal > 1;
-      for (auto* p : pv) require((p->temporal > 1) == temporal, "batched DFTTest jobs must share tbsize");
-      if (temporal) enqueue_dft_vol_jobs(pv, dj, tq, device_flags, (hipStream_t)stream);
+      const bool mfma = pv[0]->transform == NF_TRANSFORM_MFMA_F16X3;
+      for (auto* p : pv) {
+        require((p->temporal > 1) == temporal, "batched DFTTest jobs must share tbsize");
+        require((p->transform == NF_TRANSFORM_MFMA_F16X3) == mfma, "batched DFTTest jobs must share the transform form");
+      }
+      if (mfma) enqueue_dft_mfma_jobs(pv, dj, tq, device_flags, (hipStream_t)stream);
+      else if (temporal) enqueue_dft_vol_jobs(pv, dj, tq, device_flags, (hipStream_t)stream);
       else enqueue_dft_jobs(pv, dj, device_flags, (hipStream_t)stream);
     } else if (!fj.empty()) {
       enqueue_fft3d_jobs(pv, fj, device_flags, (hipStream_t)stream, true);
@@ -1383,6 +1432,21 @@ extern "C" nf_status nf_dfttest_wiener_backward(const nf_plan* plan, nf_plane so
 }
 
 // ---------------------------------------------------------------- Kalman
+extern "C" nf_status nf_plan_set_transform(nf_plan* plan, int transform) {
+  return guarded([&]() -> nf_status {
+    require(plan, "null argument");
+    require(transform == NF_TRANSFORM_RADIX2 || transform == NF_TRANSFORM_MFMA_F16X3, "unknown transform form");
+    if (transform == NF_TRANSFORM_MFMA_F16X3) {
+      if (plan->algorithm != 0) throw Unsupported("the matrix-core form covers DFTTest plans");
+      if (!dfttest_mfma_supported(plan->B, plan->temporal) || plan->tmode != 0)
+        throw Unsupported("the matrix-core form covers sbsize 16 with tbsize 1, 3 or 5 (tmode 0)");
+      if (plan->dither > 0 && plan->storage == kU8) throw Unsupported("the matrix-core form does not dither");
+    }
+    plan->transform = transform;
+    return NF_OK;
+  });
+}
+
 // ------------------------------------------------------ sampled noise models
 extern "C" nf_status nf_plan_sampling(const nf_plan* plan, int* needs_model, int* model_ready) {
   if (!plan || !needs_model || !model_ready) return set_error(NF_EINVAL, "null argument");

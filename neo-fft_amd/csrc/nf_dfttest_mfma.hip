@@ -1,0 +1,364 @@
+// nf_dfttest_mfma.hip -- DFTTest with the block DFTs as dense DFT-matrix
+// products on the CDNA4 matrix cores (v_mfma_f32_16x16x16_f16), operands
+// split into FP16 high and low parts with FP32 accumulation.
+//
+// This is the precision form beside the FP32 radix-2 path: the same filter
+// (reference plan.cpp:538-640; gains scalar/spectral.cpp:5-53; temporal
+// butterflies fft_backend.cpp:397-420), with every 16-point spatial
+// transform evaluated as a 16x16x16 matrix product. A value v is carried as
+// hi = fp16(v), lo = fp16(v - hi) (both rounded toward zero); a product A*B
+// is hi*hi + lo*hi + hi*lo
+// (the lo*lo term, 2^-22 relative, is dropped), accumulated in FP32. The
+// DFT-matrix constants are held at 256x scale (|c| >= 0.38 for the non-zero
+// entries, so their low parts stay normal FP16 numbers) and every product is
+// brought back with an exact 1/256 in FP32. Results
+// are not bit-identical to the radix-2 path; their error against a binary64
+// evaluation is measured by the tests.
+//
+// One wave transforms one 16x16 block as four chained products whose
+// results stay in the accumulator layout (lane l holds column l%16, rows
+// 4*(l/16)..+3), which is exactly the operand layout the next product needs:
+//   R   = x  . Cf      rows y, columns kx   (row transform, kx < 9 kept)
+//   X   = F  . R      rows ky, columns kx   (column transform)
+//   Z^T = X^T . Fi/16 rows kx, columns yy   (inverse column transform)
+//   y^T = M  . Z^T    rows xx, columns yy   (inverse row transform, Hermitian
+//                                            weights 1,2,..,2,1 and 1/16)
+// The temporal transform (tbsize > 1), the zmean template, the gain and the
+// synthesis window run on the VALU between the products. Overlap-add uses
+// the strip kernel's LDS ring in the canonical order.
+#include "nf_device.hpp"
+#include "nf_fft.hpp"
+#include "nf_gain.hpp"
+#include "nf_launch.hpp"
+
+namespace nf {
+
+namespace {
+
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+struct Hl {
+  h4 hi, lo;
+};
+
+// hi and lo come from one packed conversion each, so the hi that enters the
+// product is the very value the residual was taken from (two separately
+// emitted conversions of the same float could round a tie differently).
+__device__ __forceinline__ Hl split(f4 v) {
+  Hl s;
+  const auto h01 = __builtin_amdgcn_cvt_pkrtz(v[0], v[1]);
+  const auto h23 = __builtin_amdgcn_cvt_pkrtz(v[2], v[3]);
+  const float r0 = v[0] - (float)h01[0], r1 = v[1] - (float)h01[1];
+  const float r2 = v[2] - (float)h23[0], r3 = v[3] - (float)h23[1];
+  const auto l01 = __builtin_amdgcn_cvt_pkrtz(r0, r1);
+  const auto l23 = __builtin_amdgcn_cvt_pkrtz(r2, r3);
+  s.hi[0] = h01[0];
+  s.hi[1] = h01[1];
+  s.hi[2] = h23[0];
+  s.hi[3] = h23[1];
+  s.lo[0] = l01[0];
+  s.lo[1] = l01[1];
+  s.lo[2] = l23[0];
+  s.lo[3] = l23[1];
+  return s;
+}
+
+__device__ __forceinline__ f4 mma(const Hl& a, const Hl& b, f4 c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x16f16(a.hi, b.hi, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x16f16(a.lo, b.hi, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x16f16(a.hi, b.lo, c, 0, 0, 0);
+  return c;
+}
+
+constexpr float kConstScale = 256.0f, kUnscale = 1.0f / 256.0f;
+
+__device__ __forceinline__ f4 unscale(f4 v) {
+#pragma unroll
+  for (int e = 0; e < 4; ++e) v[e] = v[e] * kUnscale;
+  return v;
+}
+
+__device__ __forceinline__ Hl split_scaled(const float* v, float scale) {
+  f4 t;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) t[e] = v[e] * scale;
+  return split(t);
+}
+
+}  // namespace
+
+// tw: cos/sin of 2*pi*k/16, k = 0..15, as (cos, sin) pairs (binary64 -> float).
+template <int TT>
+__global__ __launch_bounds__(256) void dfttest_mfma16_kernel(const DftVolLaunch L, const float* tw) {
+  constexpr int B = 16, K = B / 2 + 1, OS = B + 1, BINS2 = B * K;
+  extern __shared__ float smem[];
+  int jid = 0;
+  while (jid + 1 < L.njobs && (int)blockIdx.x >= L.tile_start[jid + 1]) ++jid;
+  const DftJob& J = L.jobs[jid];
+  const DftTemporal& Q = L.tq[jid];
+  const int tile = (int)blockIdx.x - L.tile_start[jid];
+  const int tx = tile % J.tiles_x, ty = tile / J.tiles_x;
+  const int S = J.S;
+  const int tid = threadIdx.x, nthreads = blockDim.x;
+  const int lane = tid & 63, wave = tid >> 6, nwaves = nthreads >> 6;
+  const int r = lane & 15, q = lane >> 4;
+
+  const int vx0 = J.gx.offset, vx1 = J.gx.offset + J.gx.length;
+  const int vy0 = J.gy.offset, vy1 = J.gy.offset + J.gy.length;
+  const int X0 = max(vx0, (J.tile_x0 + tx) * J.tile_w), X1 = min(vx1, (J.tile_x0 + tx + 1) * J.tile_w);
+  const int Y0 = max(vy0, (J.tile_y0 + ty) * J.tile_h), Y1 = min(vy1, (J.tile_y0 + ty + 1) * J.tile_h);
+  const int bx_lo = ceil_div_int(X0 - B + 1, S), bx_hi = (X1 - 1) / S;
+  const int by_lo = ceil_div_int(Y0 - B + 1, S), by_hi = (Y1 - 1) / S;
+  const int nbx = bx_hi - bx_lo + 1;
+  const int WW = (nbx - 1) * S + B;
+  const int ww = J.ww_max;
+  const int cx0 = bx_lo * S;
+  const int TW = X1 - X0;
+  const int target = TT / 2;
+
+  float* win = smem;                              // [TT][B][ww]
+  float* out = win + TT * B * ww;                 // [nbx_max][B][OS]
+  float* ring = out + J.nbx_max * B * OS;         // [B][tile_w]
+  float* hw = ring + B * J.tile_w;                // [TT][B][OS]
+  float* hs = hw + TT * B * OS;                   // [B][OS]
+
+  bool bad = false;
+  for (int i = tid; i < TT * B * B; i += nthreads) hw[(i / B) * OS + i % B] = J.h[i];
+  for (int i = tid; i < B * B; i += nthreads) hs[(i / B) * OS + i % B] = J.hsyn[target * B * B + i];
+  for (int i = tid; i < B * J.tile_w; i += nthreads) ring[i] = 0.0f;
+
+  // Per-lane DFT-matrix fragments; entry (r, m = 4q+e) is cos/sin(2 pi r m / 16).
+  float cs[4], sn[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int m = 4 * q + e, idx = (r * m) & (B - 1);
+    cs[e] = tw[2 * idx];
+    sn[e] = tw[2 * idx + 1];
+  }
+  float t[4];
+  // row transform (B operand, k = x = m, j = kx = r): cos, -sin, kx < K
+#pragma unroll
+  for (int e = 0; e < 4; ++e) t[e] = r < K ? cs[e] : 0.0f;
+  const Hl Cf = split_scaled(t, kConstScale);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) t[e] = r < K ? -sn[e] : 0.0f;
+  const Hl Sf = split_scaled(t, kConstScale);
+  // column transform (A operand, i = ky = r, k = y = m): cos, sin, -sin
+  const Hl C2 = split_scaled(cs, kConstScale);
+  const Hl S2 = split_scaled(sn, kConstScale);
+  const Hl S2n = split_scaled(sn, -kConstScale);
+  // inverse column transform (B operand, k = ky = m, j = yy = r), 1/16
+  const Hl C3 = split_scaled(cs, kConstScale * 0.0625f);
+  const Hl S3 = split_scaled(sn, kConstScale * 0.0625f);
+  const Hl S3n = split_scaled(sn, -kConstScale * 0.0625f);
+  // inverse row transform (A operand, i = xx = r, k = kx = m): Hermitian
+  // weights, 1/16, kx < K
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int m = 4 * q + e;
+    t[e] = m < K ? ((m == 0 || m == B / 2) ? 1.0f : 2.0f) * cs[e] * 0.0625f : 0.0f;
+  }
+  const Hl Mc = split_scaled(t, kConstScale);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int m = 4 * q + e;
+    t[e] = m < K ? -((m == 0 || m == B / 2) ? 1.0f : 2.0f) * sn[e] * 0.0625f : 0.0f;
+  }
+  const Hl Ms = split_scaled(t, kConstScale);
+  const float tscale = TT > 1 ? 1.0f / (float)TT : 1.0f;
+  const bool use_grid = J.grid && J.mean_scale != 0.0f && J.grid[0].x != 0.0f;
+  __syncthreads();
+
+  for (int by = by_lo; by <= by_hi; ++by) {
+    const int oy = by * S;
+    // A: decoded, reflected source rows of every slot
+    for (int idx = tid; idx < TT * B * WW; idx += nthreads) {
+      const int z = idx / (B * WW);
+      const int rem = idx - z * (B * WW);
+      const int rr = rem / WW, c = rem - rr * WW;
+      const int f = Q.slot_frame[z];
+      const int sy = reflect(oy + rr, J.gy), sx = reflect(cx0 + c, J.gx);
+      const float v = load_sample(Q.frames[f], Q.frame_stride[f], J.storage, sy, sx) * J.dscale;
+      bad |= !isfinite(v);
+      win[(z * B + rr) * ww + c] = v;
+    }
+    __syncthreads();
+    // M: one wave per block: forward products, gain, inverse products
+    for (int bi = wave; bi < nbx; bi += nwaves) {
+      f4 Xr[TT], Xi[TT];
+#pragma unroll
+      for (int z = 0; z < TT; ++z) {
+        f4 xv;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int x = 4 * q + e;
+          xv[e] = win[(z * B + r) * ww + bi * S + x] * hw[(z * B + r) * OS + x];
+        }
+        const Hl xs = split(xv);
+        const f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+        const Hl Rr = split(unscale(mma(xs, Cf, zero)));
+        const Hl Ri = split(unscale(mma(xs, Sf, zero)));
+        Xr[z] = unscale(mma(S2, Ri, mma(C2, Rr, zero)));
+        Xi[z] = unscale(mma(S2n, Rr, mma(C2, Ri, zero)));
+      }
+      // temporal forward transform per bin (same lane, same element in every slot)
+      if (TT > 1) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          Cpx v[TT > 1 ? TT : 2];
+#pragma unroll
+          for (int z = 0; z < TT; ++z) v[z] = Cpx{Xr[z][e], Xi[z][e]};
+          temporal_butterfly(TT, v, true);
+#pragma unroll
+          for (int z = 0; z < TT; ++z) {
+            Xr[z][e] = v[z].r;
+            Xi[z][e] = v[z].i;
+          }
+        }
+      }
+      // zmean scale from the (3D) DC: lane 0, element 0
+      const float dcr = __shfl(Xr[0][0], 0, 64);
+      const float gs = use_grid ? (J.mean_scale * dcr) / J.grid[0].x : 0.0f;
+      // gain over the (3D) spectrum
+#pragma unroll
+      for (int z = 0; z < TT; ++z) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int ky = 4 * q + e;
+          if (r < K) {
+            const int k3 = z * BINS2 + ky * K + r;
+            float re = Xr[z][e], im = Xi[z][e], mr = 0.0f, mi = 0.0f;
+            if (J.grid) {
+              const float2 gv = J.grid[k3];
+              mr = gs * gv.x;
+              mi = gs * gv.y;
+            }
+            apply_gain(J.g, k3, re, im, mr, mi, bad);
+            Xr[z][e] = re;
+            Xi[z][e] = im;
+          } else {
+            Xr[z][e] = 0.0f;
+            Xi[z][e] = 0.0f;
+          }
+        }
+      }
+      // inverse temporal transform of the target slice
+      f4 Yr = Xr[0], Yi = Xi[0];
+      if (TT > 1) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          Cpx v[TT > 1 ? TT : 2];
+#pragma unroll
+          for (int z = 0; z < TT; ++z) v[z] = Cpx{Xr[z][e], Xi[z][e]};
+          Cpx res;
+          if (TT == 3 || TT == 5) {
+            res = temporal_inverse_center(TT, v);
+          } else {
+            temporal_butterfly(TT, v, false);
+            res = v[target];
+          }
+          Yr[e] = res.r * tscale;
+          Yi[e] = res.i * tscale;
+        }
+      }
+      const f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+      const Hl Ar = split(Yr), Ai = split(Yi);
+      const Hl Zr = split(unscale(mma(Ai, S3n, mma(Ar, C3, zero))));
+      const Hl Zi = split(unscale(mma(Ai, C3, mma(Ar, S3, zero))));
+      const f4 yT = unscale(mma(Ms, Zi, mma(Mc, Zr, zero)));
+      // yT: column yy = r, rows xx = 4q+e
+      float* ob = out + (bi * B + r) * OS;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int x = 4 * q + e;
+        bad |= !isfinite(yT[e]);
+        ob[x] = yT[e] * hs[r * OS + x];
+      }
+    }
+    __syncthreads();
+    // F: overlap-add into the ring, block columns in ascending order
+    {
+      const int r0 = max(oy, Y0), r1 = min(oy + B, Y1);
+      const int nr = r1 - r0;
+      for (int idx = tid; idx < nr * TW; idx += nthreads) {
+        const int rr = idx / TW;
+        const int x = X0 + (idx - rr * TW);
+        const int y = r0 + rr;
+        float* cell = ring + (y % B) * J.tile_w + (x - X0);
+        float sum = *cell;
+        const int b_first = max(bx_lo, ceil_div_int(x - B + 1, S));
+        const int b_last = min(bx_hi, x / S);
+        for (int bx = b_first; bx <= b_last; ++bx)
+          sum = sum + out[((bx - bx_lo) * B + (y - oy)) * OS + (x - bx * S)];
+        *cell = sum;
+      }
+    }
+    __syncthreads();
+    // G: rows [oy, oy+S) are final
+    {
+      const int r0 = max(oy, Y0), r1 = min(oy + S, Y1);
+      const int nr = r1 - r0;
+      for (int idx = tid; idx < nr * TW; idx += nthreads) {
+        const int rr = idx / TW;
+        const int x = X0 + (idx - rr * TW);
+        const int y = r0 + rr;
+        float* cell = ring + (y % B) * J.tile_w + (x - X0);
+        const float a = *cell;
+        *cell = 0.0f;
+        bad |= !isfinite(a);
+        const int oyy = y - vy0, oxx = x - vx0;
+        char* drow = static_cast<char*>(J.dst) + (long long)oyy * J.dst_stride;
+        if (J.storage == kF32) {
+          reinterpret_cast<float*>(drow)[oxx] = a * J.oscale;
+        } else {
+          const float v = (a * J.oscale) + 0.5f;
+          const float cl = v < 0.0f ? 0.0f : (J.peak < v ? J.peak : v);
+          if (J.storage == kU8) reinterpret_cast<uint8_t*>(drow)[oxx] = (uint8_t)cl;
+          else reinterpret_cast<uint16_t*>(drow)[oxx] = (uint16_t)cl;
+        }
+      }
+    }
+  }
+  if (bad) atomicOr(L.flags, NF_FLAG_NONFINITE);
+}
+
+size_t dfttest_mfma_lds_bytes(int B, int T, int nbx_max, int ww, int tile_w) {
+  const size_t OS = (size_t)B + 1;
+  return sizeof(float) * ((size_t)T * B * ww + (size_t)nbx_max * B * OS + (size_t)B * tile_w + (size_t)T * B * OS +
+                          (size_t)B * OS);
+}
+
+bool dfttest_mfma_supported(int B, int T) { return B == 16 && (T == 1 || T == 3 || T == 5); }
+
+hipError_t launch_dfttest_mfma(int B, int T, const DftVolLaunch& L, int total_tiles, size_t lds, const float* tw,
+                               hipStream_t stream) {
+  if (B != 16) return hipErrorInvalidValue;
+  dim3 grid(total_tiles), block(256);
+  switch (T) {
+    case 1:
+      hipLaunchKernelGGL(dfttest_mfma16_kernel<1>, grid, block, lds, stream, L, tw);
+      break;
+    case 3:
+      hipLaunchKernelGGL(dfttest_mfma16_kernel<3>, grid, block, lds, stream, L, tw);
+      break;
+    case 5:
+      hipLaunchKernelGGL(dfttest_mfma16_kernel<5>, grid, block, lds, stream, L, tw);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t dfttest_mfma_set_lds_limit() {
+  const void* fns[] = {(const void*)dfttest_mfma16_kernel<1>, (const void*)dfttest_mfma16_kernel<3>,
+                       (const void*)dfttest_mfma16_kernel<5>};
+  for (const void* f : fns) {
+    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+}  // namespace nf

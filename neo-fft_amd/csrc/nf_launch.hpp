@@ -26,6 +26,12 @@ hipError_t launch_fft3d_block(int BW, int BH, const Fft3dLaunch& L, int total_gr
 hipError_t launch_fft3d_synthesis(const Fft3dLaunch& L, hipStream_t stream);
 hipError_t fft3d_set_lds_limit();
 
+size_t dfttest_mfma_lds_bytes(int B, int T, int nbx_max, int ww, int tile_w);
+bool dfttest_mfma_supported(int B, int T);
+hipError_t launch_dfttest_mfma(int B, int T, const DftVolLaunch& L, int total_tiles, size_t lds, const float* tw,
+                               hipStream_t stream);
+hipError_t dfttest_mfma_set_lds_limit();
+
 size_t dft_noise_lds_bytes(int B, int T);
 hipError_t launch_dft_noise(int B, const DftNoiseArgs& A, int nloc, float calibration, float* out,
                             hipStream_t stream);

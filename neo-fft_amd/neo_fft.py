@@ -116,6 +116,7 @@ _SIGNATURES = {
                                                  ctypes.c_void_p]),
     "nf_dfttest_wiener_backward": (ctypes.c_int, [ctypes.c_void_p, Plane, Plane, MutPlane, ctypes.c_void_p,
                                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "nf_plan_set_transform": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "nf_plan_sampling": (ctypes.c_int, [ctypes.c_void_p, c_int_p, c_int_p]),
     "nf_dfttest_noise_locations": (ctypes.c_int, [ctypes.c_void_p, c_int_p, ctypes.c_int, c_int_p]),
     "nf_dfttest_prepare_noise": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Plane), ctypes.c_int,
@@ -317,6 +318,15 @@ def dfttest_volume(plan, sources, dsts, flags, first_frame=0, plane=0, stream=No
 
 def _ptr(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+TRANSFORM_RADIX2 = 0
+TRANSFORM_MFMA_F16X3 = 1
+
+
+def set_transform(plan, transform):
+    """Select the block-transform form of a plan (radix-2 FP32 or matrix-core FP16 split)."""
+    _check(lib().nf_plan_set_transform(plan.handle, int(transform)))
 
 
 def plan_sampling(plan):

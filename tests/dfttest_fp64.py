@@ -67,3 +67,38 @@ def curve_table(positions, values, block, divisor, dims=2):
     fy = np.minimum(i, block - i) / (block // 2)
     fx = fy[: block // 2 + 1]
     return (interp(fy)[:, None] * interp(fx)[None, :] / divisor).reshape(-1)
+
+
+def dfttest_plane(slots, h3, wscale, sigma, block=16, overlap=12, zmean=True, dscale=255.0):
+    """binary64 DFTTest (ftype 0, f0beta 1, tmode 0, smode 1) of one float
+    plane: slots = T planes (centre = output frame), h3 = the plan's analysis
+    window [T][B][B] (reference windows.cpp values), a = sigma / wscale.
+    Returns the float output plane (accumulator / 255 * dscale / 255)."""
+    T = len(slots)
+    h, w = slots[0].shape
+    ax, ay = _axis(w, block, overlap), _axis(h, block, overlap)
+    hw = np.asarray(h3, np.float64).reshape(T, block, block)
+    V = T * block * block
+    iy = _reflect_index(np.arange(ay["cover"]), ay, h)
+    ix = _reflect_index(np.arange(ax["cover"]), ax, w)
+    q = np.stack([np.asarray(s, np.float64)[np.ix_(iy, ix)] * dscale for s in slots])
+    S = ax["step"]
+    acc = np.zeros((ay["cover"], ax["cover"]))
+    a = float(np.float32(sigma) / np.float32(wscale))
+    G = np.fft.rfftn(255 * hw, axes=(0, 1, 2))
+    c = T // 2
+    for by in range(ay["count"]):
+        blocks = np.stack([np.lib.stride_tricks.sliding_window_view(q[z, by * S:by * S + block], (block, block))[0, ::S]
+                           for z in range(T)], axis=1)  # [nbx][T][B][B]
+        X = np.fft.rfftn(blocks * hw, axes=(1, 2, 3))
+        if zmean:
+            m = (X[:, :1, :1, :1].real / G[0, 0, 0].real) * G
+        else:
+            m = np.zeros_like(X)
+        R = X - m
+        p = np.abs(R) ** 2
+        Wn = np.maximum((p - a) / (p + 1e-15), 0)
+        y = np.fft.irfftn(Wn * R + m, s=(T, block, block), axes=(1, 2, 3))[:, c] * (hw[c] * V)
+        for bx in range(ax["count"]):
+            acc[by * S:by * S + block, bx * S:bx * S + block] += y[bx]
+    return acc[ay["offset"]:ay["offset"] + h, ax["offset"]:ax["offset"] + w] / 255

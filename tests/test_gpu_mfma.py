@@ -1,0 +1,77 @@
+"""Matrix-core DFTTest form (FP16 split hi/lo DFT-matrix products, FP32
+accumulation) against a binary64 evaluation of the same filter: float planes
+must stay within 1e-4 of the plane's range (max) and report their RMS error;
+the radix-2 FP32 form is measured the same way for comparison. Integer planes
+must agree with the bit-exact radix-2 form to one step everywhere (8-bit: on
+all but 0.1% of samples exactly)."""
+import numpy as np
+import pytest
+
+import dfttest_fp64 as F64
+from gpu_util import compare, empty_like_np, to_device, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+def _smooth_noisy(rng, h, w):
+    yy, xx = np.mgrid[0:h, 0:w]
+    img = 0.5 + 0.25 * np.sin(xx * 0.11 + yy * 0.05) + 0.1 * np.cos(yy * 0.3)
+    return np.clip(img + rng.normal(0, 0.03, (h, w)), 0, 1).astype(np.float32)
+
+
+def _run(nf, plan, slots, dtype):
+    dst = empty_like_np(slots[0])
+    plan.process([to_device(s) for s in slots], dst)
+    return to_host(dst, dtype)
+
+
+@pytest.mark.parametrize("T", [1, 3, 5])
+def test_mfma_float_error_against_binary64(nf, oracle, T):
+    rng = np.random.default_rng(40 + T)
+    h, w = 120, 176
+    slots = [_smooth_noisy(rng, h, w) for _ in range(T)]
+    params = dict(tbsize=T, sbsize=16, sosize=12, ftype=0, sigma=8.0)
+    hwin, ws = oracle.dft_window_3d(T, 16, 12, 1, 0, 7)
+    ref = F64.dfttest_plane([s.astype(np.float64) for s in slots], hwin, ws, 8.0)
+    errs = {}
+    for form in (nf.TRANSFORM_RADIX2, nf.TRANSFORM_MFMA_F16X3):
+        plan = nf.dfttest_plan(w, h, bits=32, floating=True, **params)
+        nf.set_transform(plan, form)
+        got = _run(nf, plan, slots, np.float32).astype(np.float64)
+        d = np.abs(got - ref)
+        errs[form] = (d.max(), np.sqrt((d ** 2).mean()))
+    print(f"T={T} max/rms error vs binary64: radix2 {errs[0]}, mfma {errs[1]}")
+    assert errs[nf.TRANSFORM_RADIX2][0] <= 2e-6
+    assert errs[nf.TRANSFORM_MFMA_F16X3][0] <= 1e-4
+    assert errs[nf.TRANSFORM_MFMA_F16X3][1] <= 1e-5
+
+
+@pytest.mark.parametrize("T", [1, 3])
+@pytest.mark.parametrize("bits", [8, 16])
+def test_mfma_integer_planes_match_radix2(nf, oracle, T, bits):
+    rng = np.random.default_rng(bits + T)
+    h, w = 100, 150
+    peak = (1 << bits) - 1
+    base = rng.integers(0, peak + 1, (h, w))
+    dt = np.uint8 if bits == 8 else np.uint16
+    slots = [np.clip(base + rng.integers(-peak // 30, peak // 30 + 1, (h, w)), 0, peak).astype(dt) for _ in range(T)]
+    params = dict(tbsize=T, sbsize=16, sosize=12, ftype=0, sigma=6.0)
+    ref = oracle.dfttest_plane(slots, bits=bits, **params)
+    plan = nf.dfttest_plan(w, h, bits=bits, **params)
+    nf.set_transform(plan, nf.TRANSFORM_MFMA_F16X3)
+    got = _run(nf, plan, slots, dt)
+    mx, frac = compare(got, ref)
+    # the form's ~1e-6 relative error flips roundings of 16-bit outputs that
+    # lie near a half step; every sample stays within one step
+    assert mx <= 1, (mx, frac)
+    if bits == 8:
+        assert frac <= 1e-3, (mx, frac)
+
+
+def test_mfma_form_coverage(nf):
+    plan = nf.dfttest_plan(64, 64, bits=8, tbsize=1, sbsize=8, sosize=6)
+    with pytest.raises(nf.Unsupported):
+        nf.set_transform(plan, nf.TRANSFORM_MFMA_F16X3)
+    plan = nf.fft3d_plan(64, 64, bits=8)
+    with pytest.raises(nf.Unsupported):
+        nf.set_transform(plan, nf.TRANSFORM_MFMA_F16X3)
