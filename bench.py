@@ -198,6 +198,23 @@ def main():
             "avg_launch_ms": round(dk_ms / max(1, dk_n), 5),
             "achieved_gbs": round(BYTES_PER_FRAME / (dk_ms / max(1, dk_n) / 1e3) / 1e9, 2),
         }
+        # the same workload through the matrix-core form (FP16 split hi/lo)
+        for p in dplans:
+            nf.set_transform(p, nf.TRANSFORM_MFMA_F16X3)
+        for i in range(3):
+            dstep(i)
+        nf.profile_enable(True)
+        nf.profile_read(reset=True)
+        mms = _timed(torch, stream, dstep, dsteps)
+        mprof = nf.profile_read(reset=True)
+        nf.profile_enable(False)
+        mk_ms, mk_n = mprof["dfttest_strip"]
+        result["dfttest_sbsize16"]["mfma_f16x3"] = {
+            "frames_per_s": round(dsteps / (mms / 1e3), 2),
+            "avg_launch_ms": round(mk_ms / max(1, mk_n), 5),
+            "speedup_vs_radix2": round(dms / mms, 3),
+            "error_vs_binary64": mfma_error(nf, torch),
+        }
 
     if not args.no_configs:
         result["configs"] = other_configs(nf, torch, clip, stream, flags, args)
@@ -208,6 +225,34 @@ def main():
             result["gpu_over_cpu"] = round(fps / result["cpu_baseline"]["value"], 1)
     print(json.dumps(result))
     return 0
+
+
+def mfma_error(nf, torch):
+    """Max and RMS error of both forms against a binary64 evaluation on a
+    176x120 float32 plane (smooth texture + noise), tbsize 1 and 3."""
+    import numpy as np
+    import dfttest_fp64 as F64
+    from oracle import oracle as O
+    from gpu_util import empty_like_np, to_device, to_host
+    rng = np.random.default_rng(40)
+    h, w = 120, 176
+    yy, xx = np.mgrid[0:h, 0:w]
+    out = {}
+    for T in (1, 3):
+        slots = []
+        for _ in range(T):
+            img = 0.5 + 0.25 * np.sin(xx * 0.11 + yy * 0.05) + 0.1 * np.cos(yy * 0.3)
+            slots.append(np.clip(img + rng.normal(0, 0.03, (h, w)), 0, 1).astype(np.float32))
+        hwin, ws = O.dft_window_3d(T, 16, 12, 1, 0, 7)
+        ref = F64.dfttest_plane([s_.astype(np.float64) for s_ in slots], hwin, ws, 8.0)
+        for name, form in (("radix2", nf.TRANSFORM_RADIX2), ("mfma", nf.TRANSFORM_MFMA_F16X3)):
+            plan = nf.dfttest_plan(w, h, bits=32, floating=True, tbsize=T, sbsize=16, sosize=12, ftype=0, sigma=8.0)
+            nf.set_transform(plan, form)
+            dst = empty_like_np(slots[0])
+            plan.process([to_device(s_) for s_ in slots], dst)
+            d = np.abs(to_host(dst, np.float32).astype(np.float64) - ref)
+            out[f"{name}_t{T}"] = {"max": float(f"{d.max():.3g}"), "rms": float(f"{np.sqrt((d ** 2).mean()):.3g}")}
+    return out
 
 
 def _timed(torch, stream, fn, steps):

@@ -147,6 +147,8 @@ struct nf_plan {
   std::vector<int> fft3d_nbs;
   mutable std::atomic<int> choice{0};
   mutable std::atomic<int> tuned{0};
+  mutable std::atomic<int> mfma_choice{0};  // tile of the matrix-core form
+  mutable std::atomic<int> mfma_tuned{0};
   mutable std::mutex tune_mu;
   int tmode = 0, tosize = 0;
   const float* d_ttw = nullptr;
@@ -877,13 +879,16 @@ bool autotune_enabled(hipStream_t stream) {
 // Times every candidate with `run(c)` (one launch of the group's jobs, the
 // candidate applied to each) and publishes the fastest to every plan of the
 // group that is not tuned yet. Outputs of all candidates are identical.
+using TuneSlot = std::atomic<int> nf_plan::*;
+
 template <class Run>
-void autotune(const std::vector<const nf_plan*>& group, int ncand, hipStream_t stream, Run&& run) {
+void autotune(const std::vector<const nf_plan*>& group, int ncand, hipStream_t stream, Run&& run,
+              TuneSlot choice = &nf_plan::choice, TuneSlot tuned = &nf_plan::tuned) {
   bool pending = false;
-  for (auto* p : group) pending = pending || !p->tuned.load();
+  for (auto* p : group) pending = pending || !(p->*tuned).load();
   if (ncand <= 1 || !pending || !autotune_enabled(stream)) return;
   std::lock_guard<std::mutex> lock(group[0]->tune_mu);
-  if (group[0]->tuned.load() && group.size() == 1) return;
+  if ((group[0]->*tuned).load() && group.size() == 1) return;
   hipEvent_t a, b;
   hip_check(hipEventCreate(&a), "hipEventCreate");
   hip_check(hipEventCreate(&b), "hipEventCreate");
@@ -905,9 +910,10 @@ void autotune(const std::vector<const nf_plan*>& group, int ncand, hipStream_t s
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   for (auto* p : group)
-    if (!p->tuned.load()) {
-      p->choice.store(best);
-      p->tuned.store(1);
+    if (!(p->*tuned).load()) {
+      // the tuning slots are mutable members of an otherwise immutable plan
+      (const_cast<nf_plan*>(p)->*choice).store(best);
+      (const_cast<nf_plan*>(p)->*tuned).store(1);
     }
 }
 
@@ -1171,26 +1177,46 @@ void enqueue_dft_mfma_jobs(const std::vector<const nf_plan*>& plans, const std::
                            const std::vector<DftTemporal>& tq, unsigned* flags, hipStream_t stream) {
   ensure_lds_limits();
   for (size_t first = 0; first < jobs.size(); first += kMaxVolJobs) {
-    DftVolLaunch L{};
     const size_t n = std::min<size_t>(kMaxVolJobs, jobs.size() - first);
-    int tiles = 0;
-    size_t lds = 0;
     const int B = plans[first]->B, T = plans[first]->temporal;
-    for (size_t i = 0; i < n; ++i) {
-      const nf_plan& p = *plans[first + i];
-      require(p.B == B && p.temporal == T, "batched matrix-core DFTTest jobs must share sbsize and tbsize");
-      L.jobs[i] = apply_tile(jobs[first + i], p, 0);
-      L.tq[i] = tq[first + i];
-      L.tile_start[i] = tiles;
-      tiles += L.jobs[i].tiles_x * L.jobs[i].tiles_y;
-      lds = std::max(lds, dfttest_mfma_lds_bytes(B, T, L.jobs[i].nbx_max, L.jobs[i].ww_max, L.jobs[i].tile_w));
+    const std::vector<const nf_plan*> group(plans.begin() + first, plans.begin() + first + n);
+    size_t ncand = group[0]->dft_tiles.size();
+    for (auto* p : group) {
+      require(p->B == B && p->temporal == T, "batched matrix-core DFTTest jobs must share sbsize and tbsize");
+      ncand = std::min(ncand, p->dft_tiles.size());
     }
-    L.tile_start[n] = tiles;
-    L.njobs = (int)n;
-    L.flags = flags;
-    timed(0, stream, [&] {
-      hip_check(launch_dfttest_mfma(B, T, L, tiles, lds, plans[first]->d_tw, stream), "dfttest mfma kernel launch");
-    });
+    auto launch = [&](auto tile_of) {
+      DftVolLaunch L{};
+      int tiles = 0;
+      size_t lds = 0;
+      for (size_t i = 0; i < n; ++i) {
+        L.jobs[i] = apply_tile(jobs[first + i], *group[i], tile_of(i));
+        L.tq[i] = tq[first + i];
+        L.tile_start[i] = tiles;
+        tiles += L.jobs[i].tiles_x * L.jobs[i].tiles_y;
+        lds = std::max(lds, dfttest_mfma_lds_bytes(B, T, L.jobs[i].nbx_max, L.jobs[i].ww_max, L.jobs[i].tile_w));
+      }
+      require(lds <= 160 * 1024, "DFTTest matrix-core tile does not fit the LDS");
+      L.tile_start[n] = tiles;
+      L.njobs = (int)n;
+      L.flags = flags;
+      hip_check(launch_dfttest_mfma(B, T, L, tiles, lds, group[0]->d_tw, stream), "dfttest mfma kernel launch");
+    };
+    // candidates whose matrix-core LDS fits
+    int fit = 0;
+    while (fit < (int)ncand) {
+      bool ok = true;
+      for (auto* p : group) {
+        const auto& t = p->dft_tiles[fit];
+        ok = ok && dfttest_mfma_lds_bytes(B, T, t.nbx_max, t.ww_max, t.tile_w) <= 160 * 1024;
+      }
+      if (!ok) break;
+      ++fit;
+    }
+    require(fit > 0, "DFTTest matrix-core tile does not fit the LDS");
+    autotune(group, fit, stream, [&](int c) { launch([c](size_t) { return c; }); }, &nf_plan::mfma_choice,
+             &nf_plan::mfma_tuned);
+    timed(0, stream, [&] { launch([&](size_t i) { return std::min(group[i]->mfma_choice.load(), fit - 1); }); });
   }
 }
 

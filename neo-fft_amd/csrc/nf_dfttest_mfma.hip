@@ -184,96 +184,124 @@ __global__ __launch_bounds__(256) void dfttest_mfma16_kernel(const DftVolLaunch 
       win[(z * B + rr) * ww + c] = v;
     }
     __syncthreads();
-    // M: one wave per block: forward products, gain, inverse products
-    for (int bi = wave; bi < nbx; bi += nwaves) {
-      f4 Xr[TT], Xi[TT];
-#pragma unroll
-      for (int z = 0; z < TT; ++z) {
-        f4 xv;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int x = 4 * q + e;
-          xv[e] = win[(z * B + r) * ww + bi * S + x] * hw[(z * B + r) * OS + x];
-        }
-        const Hl xs = split(xv);
-        const f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
-        const Hl Rr = split(unscale(mma(xs, Cf, zero)));
-        const Hl Ri = split(unscale(mma(xs, Sf, zero)));
-        Xr[z] = unscale(mma(S2, Ri, mma(C2, Rr, zero)));
-        Xi[z] = unscale(mma(S2n, Rr, mma(C2, Ri, zero)));
-      }
-      // temporal forward transform per bin (same lane, same element in every slot)
-      if (TT > 1) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          Cpx v[TT > 1 ? TT : 2];
-#pragma unroll
-          for (int z = 0; z < TT; ++z) v[z] = Cpx{Xr[z][e], Xi[z][e]};
-          temporal_butterfly(TT, v, true);
-#pragma unroll
-          for (int z = 0; z < TT; ++z) {
-            Xr[z][e] = v[z].r;
-            Xi[z][e] = v[z].i;
-          }
-        }
-      }
-      // zmean scale from the (3D) DC: lane 0, element 0
-      const float dcr = __shfl(Xr[0][0], 0, 64);
-      const float gs = use_grid ? (J.mean_scale * dcr) / J.grid[0].x : 0.0f;
-      // gain over the (3D) spectrum
-#pragma unroll
-      for (int z = 0; z < TT; ++z) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int ky = 4 * q + e;
-          if (r < K) {
-            const int k3 = z * BINS2 + ky * K + r;
-            float re = Xr[z][e], im = Xi[z][e], mr = 0.0f, mi = 0.0f;
-            if (J.grid) {
-              const float2 gv = J.grid[k3];
-              mr = gs * gv.x;
-              mi = gs * gv.y;
-            }
-            apply_gain(J.g, k3, re, im, mr, mi, bad);
-            Xr[z][e] = re;
-            Xi[z][e] = im;
-          } else {
-            Xr[z][e] = 0.0f;
-            Xi[z][e] = 0.0f;
-          }
-        }
-      }
-      // inverse temporal transform of the target slice
-      f4 Yr = Xr[0], Yi = Xi[0];
-      if (TT > 1) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          Cpx v[TT > 1 ? TT : 2];
-#pragma unroll
-          for (int z = 0; z < TT; ++z) v[z] = Cpx{Xr[z][e], Xi[z][e]};
-          Cpx res;
-          if (TT == 3 || TT == 5) {
-            res = temporal_inverse_center(TT, v);
-          } else {
-            temporal_butterfly(TT, v, false);
-            res = v[target];
-          }
-          Yr[e] = res.r * tscale;
-          Yi[e] = res.i * tscale;
-        }
-      }
+    // M: each wave transforms two blocks at a time (independent product
+    // chains interleave): forward products, gain, inverse products
+    constexpr int NP = TT >= 5 ? 1 : 2;
+    for (int b0 = wave * NP; b0 < nbx; b0 += nwaves * NP) {
+      f4 Xr[NP][TT], Xi[NP][TT];
       const f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
-      const Hl Ar = split(Yr), Ai = split(Yi);
-      const Hl Zr = split(unscale(mma(Ai, S3n, mma(Ar, C3, zero))));
-      const Hl Zi = split(unscale(mma(Ai, C3, mma(Ar, S3, zero))));
-      const f4 yT = unscale(mma(Ms, Zi, mma(Mc, Zr, zero)));
-      // yT: column yy = r, rows xx = 4q+e
-      float* ob = out + (bi * B + r) * OS;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int x = 4 * q + e;
-        bad |= !isfinite(yT[e]);
-        ob[x] = yT[e] * hs[r * OS + x];
+      for (int z = 0; z < TT; ++z) {
+        Hl xs[NP];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+          const int bi = min(b0 + p, nbx - 1);
+          f4 xv;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int x = 4 * q + e;
+            xv[e] = win[(z * B + r) * ww + bi * S + x] * hw[(z * B + r) * OS + x];
+          }
+          xs[p] = split(xv);
+        }
+        Hl Rr[NP], Ri[NP];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+          Rr[p] = split(unscale(mma(xs[p], Cf, zero)));
+          Ri[p] = split(unscale(mma(xs[p], Sf, zero)));
+        }
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+          Xr[p][z] = unscale(mma(S2, Ri[p], mma(C2, Rr[p], zero)));
+          Xi[p][z] = unscale(mma(S2n, Rr[p], mma(C2, Ri[p], zero)));
+        }
+      }
+      Hl Ar[NP], Ai[NP];
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        // temporal forward transform per bin (same lane, same element in every slot)
+        if (TT > 1) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            Cpx v[TT > 1 ? TT : 2];
+#pragma unroll
+            for (int z = 0; z < TT; ++z) v[z] = Cpx{Xr[p][z][e], Xi[p][z][e]};
+            temporal_butterfly(TT, v, true);
+#pragma unroll
+            for (int z = 0; z < TT; ++z) {
+              Xr[p][z][e] = v[z].r;
+              Xi[p][z][e] = v[z].i;
+            }
+          }
+        }
+        // zmean scale from the (3D) DC: lane 0, element 0
+        const float dcr = __shfl(Xr[p][0][0], 0, 64);
+        const float gs = use_grid ? (J.mean_scale * dcr) / J.grid[0].x : 0.0f;
+        // gain over the (3D) spectrum
+#pragma unroll
+        for (int z = 0; z < TT; ++z) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int ky = 4 * q + e;
+            if (r < K) {
+              const int k3 = z * BINS2 + ky * K + r;
+              float re = Xr[p][z][e], im = Xi[p][z][e], mr = 0.0f, mi = 0.0f;
+              if (J.grid) {
+                const float2 gv = J.grid[k3];
+                mr = gs * gv.x;
+                mi = gs * gv.y;
+              }
+              apply_gain(J.g, k3, re, im, mr, mi, bad);
+              Xr[p][z][e] = re;
+              Xi[p][z][e] = im;
+            } else {
+              Xr[p][z][e] = 0.0f;
+              Xi[p][z][e] = 0.0f;
+            }
+          }
+        }
+        // inverse temporal transform of the target slice
+        f4 Yr = Xr[p][0], Yi = Xi[p][0];
+        if (TT > 1) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            Cpx v[TT > 1 ? TT : 2];
+#pragma unroll
+            for (int z = 0; z < TT; ++z) v[z] = Cpx{Xr[p][z][e], Xi[p][z][e]};
+            Cpx res;
+            if (TT == 3 || TT == 5) {
+              res = temporal_inverse_center(TT, v);
+            } else {
+              temporal_butterfly(TT, v, false);
+              res = v[target];
+            }
+            Yr[e] = res.r * tscale;
+            Yi[e] = res.i * tscale;
+          }
+        }
+        Ar[p] = split(Yr);
+        Ai[p] = split(Yi);
+      }
+      Hl Zr[NP], Zi[NP];
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        Zr[p] = split(unscale(mma(Ai[p], S3n, mma(Ar[p], C3, zero))));
+        Zi[p] = split(unscale(mma(Ai[p], C3, mma(Ar[p], S3, zero))));
+      }
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        const f4 yT = unscale(mma(Ms, Zi[p], mma(Mc, Zr[p], zero)));
+        const int bi = b0 + p;
+        if (bi < nbx) {
+          // yT: column yy = r, rows xx = 4q+e
+          float* ob = out + (bi * B + r) * OS;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int x = 4 * q + e;
+            bad |= !isfinite(yT[e]);
+            ob[x] = yT[e] * hs[r * OS + x];
+          }
+        }
       }
     }
     __syncthreads();
