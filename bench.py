@@ -102,10 +102,16 @@ def main():
     flags = torch.zeros(1, dtype=torch.int32, device=dev)
     stream = torch.cuda.current_stream()
 
+    for pl in plans:
+        nf.set_spectrum_cache(pl, 4)
+
     def step(n):
-        c = 1 + n % (R - 2)
-        jobs = [(plans[p], [frames[c - 1][p], frames[c][p], frames[c + 1][p]], dsts[p]) for p in range(3)]
-        nf.process_batch(jobs, flags, stream=stream)
+        # output frame n + 1 of a stream whose frame f is resident buffer f % R;
+        # consecutive outputs share two source frames (spectrum reuse)
+        o = n + 1
+        jobs = [(plans[p], [frames[(o - 1) % R][p], frames[o % R][p], frames[(o + 1) % R][p]], dsts[p])
+                for p in range(3)]
+        nf.process_batch(jobs, flags, stream=stream, frames=[o, o, o])
 
     for i in range(args.warmup):
         step(i)

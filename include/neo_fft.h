@@ -172,6 +172,17 @@ nf_status nf_dfttest_wiener_backward(const nf_plan* plan, nf_plane source, nf_pl
                                      nf_mut_plane grad_source, const float* table, float* grad_table,
                                      unsigned* device_flags, void* stream);
 
+/* Raw-spectrum reuse for stateless temporal FFT3D plans (bt >= 2), the
+ * device analogue of the reference's SpectraCache (src/runtime/
+ * spectra_cache.hpp): with `frames` > 0 the plan keeps the forward spectra of
+ * up to `frames` (>= bt) source frames, and nf_plan_enqueue / nf_process_batch
+ * calls with a full temporal window compute only the spectra of source frames
+ * they have not seen (source frame of slot j = frame - bt/2 + j, matched with
+ * the plane pointer and stride). Results are bit-identical to computing every
+ * slot. Source frames must not change while cached; one stream at a time per
+ * plan. frames = 0 releases the cache. */
+nf_status nf_plan_set_spectrum_cache(nf_plan* plan, int frames);
+
 /* Block-transform form of a plan. NF_TRANSFORM_RADIX2 (default) evaluates
  * the reference's radix-2 codelets in the same binary32 operation order
  * (bit-identical results). NF_TRANSFORM_MFMA_F16X3 evaluates each spatial

@@ -178,6 +178,26 @@ struct nf_plan {
   // block transform form (nf_plan_set_transform) and its twiddle table
   int transform = NF_TRANSFORM_RADIX2;
   const float* d_tw = nullptr;
+  // FFT3D raw-spectrum reuse (nf_plan_set_spectrum_cache): per source frame
+  // [block][re bins][im bins], keyed by (source frame number, plane pointer,
+  // stride); least recently used entries are replaced.
+  struct SpectrumCache {
+    struct Entry {
+      long long frame = -1;
+      const void* src = nullptr;
+      long long stride = 0;
+      unsigned long long used = 0;
+    };
+    float* mem = nullptr;
+    size_t per_frame = 0;  // floats
+    std::vector<Entry> entries;
+    unsigned long long tick = 0;
+    ~SpectrumCache() {
+      if (mem) (void)hipFree(mem);
+    }
+  };
+  mutable std::unique_ptr<SpectrumCache> spectra;
+  mutable std::mutex spectra_mu;
 };
 
 struct nf_kalman_state {
@@ -758,6 +778,46 @@ Fft3dJob make_fft3d_job(const nf_plan& p, int T, int mode) {
 
 size_t contrib_floats(const nf_plan& p) { return (size_t)p.gx.count * p.gy.count * p.bw * p.bh; }
 
+// Points the job's slots at cached spectra of output frame `frame` (slot j
+// holds source frame frame - T/2 + j) or at the entries their spectra are
+// written to. Full temporal windows only; the end-of-clip fallback and calls
+// without a frame number compute everything.
+void use_spectrum_cache(const nf_plan& p, Fft3dJob& j, const nf_plane* sources, int n_sources, int frame) {
+  if (!p.spectra || n_sources != p.temporal || p.temporal < 2 || frame < 0) return;
+  std::lock_guard<std::mutex> lock(p.spectra_mu);
+  auto& C = *p.spectra;
+  bool taken[16] = {};
+  const int T = p.temporal;
+  ++C.tick;
+  for (int s = 0; s < T; ++s) {
+    const long long key = (long long)frame - T / 2 + s;
+    int hit = -1;
+    for (int e = 0; e < (int)C.entries.size(); ++e) {
+      const auto& E = C.entries[e];
+      if (E.frame == key && E.src == sources[s].data && E.stride == sources[s].stride_bytes) hit = e;
+    }
+    if (hit >= 0) {
+      taken[hit] = true;
+      C.entries[hit].used = C.tick;
+      j.cache_in[s] = C.mem + C.per_frame * hit;
+    }
+  }
+  for (int s = 0; s < T; ++s) {
+    if (j.cache_in[s]) continue;
+    int victim = -1;
+    for (int e = 0; e < (int)C.entries.size(); ++e)
+      if (!taken[e] && (victim < 0 || C.entries[e].used < C.entries[victim].used)) victim = e;
+    if (victim < 0) return;
+    taken[victim] = true;
+    auto& E = C.entries[victim];
+    E.frame = (long long)frame - T / 2 + s;
+    E.src = sources[s].data;
+    E.stride = sources[s].stride_bytes;
+    E.used = C.tick;
+    j.cache_out[s] = C.mem + C.per_frame * victim;
+  }
+}
+
 // Per-thread device word for the synchronous entry points.
 struct SyncFlag {
   unsigned* dev = nullptr;
@@ -1097,8 +1157,13 @@ void enqueue_fft3d_jobs(const std::vector<const nf_plan*>& plans, std::vector<Ff
       L.pixel_start[i] = pixels;
       groups += L.jobs[i].groups_x * p.gy.count;
       pixels += (long long)p.width * p.height;
-      lds = std::max(lds, fft3d_block_lds_bytes(bw, bh, jobs[first + i].mode == 0 ? jobs[first + i].T : 1,
-                                                L.jobs[i].blocks_per_group, p.gx.step));
+      int computed = 1;
+      if (jobs[first + i].mode == 0) {
+        computed = 0;
+        for (int s = 0; s < jobs[first + i].T; ++s) computed += jobs[first + i].cache_in[s] ? 0 : 1;
+        computed = std::max(computed, 1);
+      }
+      lds = std::max(lds, fft3d_block_lds_bytes(bw, bh, computed, L.jobs[i].blocks_per_group, p.gx.step));
     }
     L.group_start[n] = groups;
     L.pixel_start[n] = pixels;
@@ -1275,6 +1340,7 @@ nf_status enqueue_one(const nf_plan* plan, const nf_plane* sources, int n_source
     }
     j.dst = dst.data;
     j.dst_stride = dst.stride_bytes;
+    use_spectrum_cache(*plan, j, sources, n_sources, frame);
     enqueue_fft3d_jobs({plan}, {j}, flags, stream, true);
   }
   return NF_OK;
@@ -1305,7 +1371,6 @@ extern "C" nf_status nf_process_batch(const nf_plan* const* plans, int n_jobs, c
                                       const int* frames, const int* planes, unsigned* device_flags, void* stream) {
   return guarded([&]() -> nf_status {
     require(plans && n_jobs > 0 && sources && first_source && n_sources && dsts && device_flags, "null argument");
-    (void)frames;
     (void)planes;
     const int alg = plans[0]->algorithm;
     std::vector<const nf_plan*> pv;
@@ -1341,6 +1406,7 @@ extern "C" nf_status nf_process_batch(const nf_plan* const* plans, int n_jobs, c
         }
         j.dst = dsts[i].data;
         j.dst_stride = dsts[i].stride_bytes;
+        use_spectrum_cache(*p, j, s, ns, frames ? frames[i] : -1);
         fj.push_back(j);
       }
     }
@@ -1458,6 +1524,25 @@ extern "C" nf_status nf_dfttest_wiener_backward(const nf_plan* plan, nf_plane so
 }
 
 // ---------------------------------------------------------------- Kalman
+extern "C" nf_status nf_plan_set_spectrum_cache(nf_plan* plan, int frames) {
+  return guarded([&]() -> nf_status {
+    require(plan, "null argument");
+    require(frames >= 0 && frames <= 16, "spectrum cache holds 0..16 source frames");
+    std::lock_guard<std::mutex> lock(plan->spectra_mu);
+    plan->spectra.reset();
+    if (frames == 0) return NF_OK;
+    if (plan->algorithm != 1 || plan->kalman || plan->temporal < 2 || plan->preview)
+      throw Unsupported("the spectrum cache covers stateless temporal FFT3D plans (bt >= 2)");
+    require(frames >= plan->temporal, "the spectrum cache must hold at least bt frames");
+    auto c = std::make_unique<nf_plan::SpectrumCache>();
+    c->per_frame = (size_t)plan->gx.count * plan->gy.count * 2 * plan->bh * (plan->bw / 2 + 1);
+    hip_check(hipMalloc((void**)&c->mem, sizeof(float) * c->per_frame * frames), "hipMalloc(spectrum cache)");
+    c->entries.resize(frames);
+    plan->spectra = std::move(c);
+    return NF_OK;
+  });
+}
+
 extern "C" nf_status nf_plan_set_transform(nf_plan* plan, int transform) {
   return guarded([&]() -> nf_status {
     require(plan, "null argument");

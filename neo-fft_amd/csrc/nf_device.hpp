@@ -150,6 +150,12 @@ struct Fft3dJob {
   float k_r0, k_ratio2;
   int blocks_per_group;    // blocks one workgroup handles along a block row
   int groups_x;            // workgroups per block row
+  // Raw-spectrum reuse across output frames (stateless temporal mode): slot
+  // j's spectra are read from cache_in[j] when set, else computed and, when
+  // cache_out[j] is set, also written there. Layout per source frame:
+  // [block][re: bh*(bw/2+1)][im: bh*(bw/2+1)].
+  const float* cache_in[5];
+  float* cache_out[5];
 };
 
 struct Fft3dLaunch {

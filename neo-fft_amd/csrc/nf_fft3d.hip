@@ -80,20 +80,29 @@ __device__ __forceinline__ void temporal_bin(const Fft3dJob& J, int k, const flo
   out_i = yi * invT;
 }
 
-// Temporal Wiener over every bin of the group's blocks into slot 0.
+// Temporal Wiener over every bin of the group's blocks into LDS slot 0.
+// Slot j's spectra are in LDS slot lidx[j], or (lidx[j] < 0) in the
+// spectrum cache at J.cache_in[j].
 template <int T, int BINS, int BS>
 __device__ __forceinline__ void temporal_pass(const Fft3dJob& J, float* sre, float* sim, float* dc, int nb, int NB,
-                                              int tid, int nthreads, bool& bad) {
+                                              const int* lidx, long long block_base, int tid, int nthreads,
+                                              bool& bad) {
   const int c = J.c;
   for (int t = tid; t < nb * BINS; t += nthreads) {
     const int bi = t / BINS, k = t - bi * BINS;
     float xr[T], xi[T];
 #pragma unroll
     for (int j = 0; j < T; ++j) {
-      xr[j] = sre[(j * NB + bi) * BS + k];
-      xi[j] = sim[(j * NB + bi) * BS + k];
+      if (lidx[j] >= 0) {
+        xr[j] = sre[(lidx[j] * NB + bi) * BS + k];
+        xi[j] = sim[(lidx[j] * NB + bi) * BS + k];
+      } else {
+        const float* g = J.cache_in[j] + (block_base + bi) * 2 * BINS;
+        xr[j] = g[k];
+        xi[j] = g[BINS + k];
+      }
     }
-    const float dcc = sre[(c * NB + bi) * BS];
+    const float dcc = lidx[c] >= 0 ? sre[(lidx[c] * NB + bi) * BS] : J.cache_in[c][(block_base + bi) * 2 * BINS];
     const float g_ratio = (J.grid && J.degrid != 0.0f && J.grid[0].x != 0.0f) ? (J.degrid * dcc) / J.grid[0].x : 0.0f;
     bad |= !isfinite(g_ratio);
     float orr, oi;
@@ -156,7 +165,14 @@ __global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
   const int WW = (nb - 1) * Sx + BW;
   const int ww = odd_stride((NB - 1) * Sx + BW);
   const int tid = threadIdx.x, nthreads = blockDim.x;
-  const int Tst = J.mode == 0 ? J.T : 1;  // spectra held in LDS
+  const int nslots = J.mode == 0 ? J.T : 1;
+  // LDS slot of every temporal slot computed here (-1: read from the cache)
+  int lidx[5];
+  int ncomp = 0;
+#pragma unroll
+  for (int j = 0; j < 5; ++j) lidx[j] = (j < nslots && !(J.mode == 0 && J.cache_in[j])) ? ncomp++ : -1;
+  const int Tst = ncomp > 0 ? ncomp : 1;  // spectra held in LDS
+  const long long block_base = (long long)by * J.gx.count + bx0;
 
   float* sre = smem;                     // [Tst][NB][BS] real parts
   float* sim = sre + Tst * NB * BS;      // [Tst][NB][BS] imaginary parts
@@ -165,7 +181,8 @@ __global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
   bool bad = false;
 
   if (J.mode != 2) {
-    for (int j = 0; j < Tst; ++j) {
+    for (int j = 0; j < nslots; ++j) {
+      if (lidx[j] < 0) continue;
       // decoded, reflected source rows: (x - base) * 1 (pad.hpp:33-38)
       for (int idx = tid; idx < BH * WW; idx += nthreads) {
         const int r = idx / WW, c = idx - r * WW;
@@ -175,8 +192,8 @@ __global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
         win[r * ww + c] = v;
       }
       __syncthreads();
-      float* jre = sre + j * NB * BS;
-      float* jim = sim + j * NB * BS;
+      float* jre = sre + lidx[j] * NB * BS;
+      float* jim = sim + lidx[j] * NB * BS;
       for (int t = tid; t < nb * BH; t += nthreads) {
         const int bi = t / BH, y = t - bi * BH;
         float re[BW], im[BW];
@@ -213,6 +230,14 @@ __global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
           cr[y * K] = re[y];
           ci[y * K] = im[y];
         }
+        if (J.mode == 0 && J.cache_out[j]) {
+          float* go = J.cache_out[j] + (block_base + bi) * 2 * BINS + k;
+#pragma unroll
+          for (int y = 0; y < BH; ++y) {
+            go[y * K] = re[y];
+            go[BINS + y * K] = im[y];
+          }
+        }
         if (k == 0 && j == 0) {
           dc[2 * bi] = re[0];
           dc[2 * bi + 1] = im[0];
@@ -222,7 +247,6 @@ __global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
     }
   }
 
-  const long long block_base = (long long)by * J.gx.count + bx0;
   if (J.mode == 1) {
     // Kalman advance: update the recurrence with this frame's spectra.
     for (int t = tid; t < nb * BINS; t += nthreads) {
@@ -250,10 +274,10 @@ __global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
   } else if (J.T > 1) {
     // temporal Wiener per bin into slot 0 (spectral.cpp:162-232)
     switch (J.T) {
-      case 2: temporal_pass<2, BINS, BS>(J, sre, sim, dc, nb, NB, tid, nthreads, bad); break;
-      case 3: temporal_pass<3, BINS, BS>(J, sre, sim, dc, nb, NB, tid, nthreads, bad); break;
-      case 4: temporal_pass<4, BINS, BS>(J, sre, sim, dc, nb, NB, tid, nthreads, bad); break;
-      default: temporal_pass<5, BINS, BS>(J, sre, sim, dc, nb, NB, tid, nthreads, bad); break;
+      case 2: temporal_pass<2, BINS, BS>(J, sre, sim, dc, nb, NB, lidx, block_base, tid, nthreads, bad); break;
+      case 3: temporal_pass<3, BINS, BS>(J, sre, sim, dc, nb, NB, lidx, block_base, tid, nthreads, bad); break;
+      case 4: temporal_pass<4, BINS, BS>(J, sre, sim, dc, nb, NB, lidx, block_base, tid, nthreads, bad); break;
+      default: temporal_pass<5, BINS, BS>(J, sre, sim, dc, nb, NB, lidx, block_base, tid, nthreads, bad); break;
     }
     __syncthreads();
   } else if (J.denoise) {

@@ -117,6 +117,7 @@ _SIGNATURES = {
     "nf_dfttest_wiener_backward": (ctypes.c_int, [ctypes.c_void_p, Plane, Plane, MutPlane, ctypes.c_void_p,
                                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "nf_plan_set_transform": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "nf_plan_set_spectrum_cache": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "nf_plan_sampling": (ctypes.c_int, [ctypes.c_void_p, c_int_p, c_int_p]),
     "nf_dfttest_noise_locations": (ctypes.c_int, [ctypes.c_void_p, c_int_p, ctypes.c_int, c_int_p]),
     "nf_dfttest_prepare_noise": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Plane), ctypes.c_int,
@@ -393,8 +394,14 @@ def fft3d_plan(width, height, bits=8, floating=False, chroma=False, **params):
     return Plan(h, "FFT3D")
 
 
-def process_batch(jobs, flags, stream=None):
-    """jobs: list of (plan, [sources], dst). One launch per group of jobs."""
+def set_spectrum_cache(plan, frames):
+    """Keep the forward spectra of up to `frames` source frames (FFT3D bt >= 2)."""
+    _check(lib().nf_plan_set_spectrum_cache(plan.handle, int(frames)))
+
+
+def process_batch(jobs, flags, stream=None, frames=None):
+    """jobs: list of (plan, [sources], dst). One launch per group of jobs.
+    frames: output frame number per job (spectrum reuse), default none."""
     n = len(jobs)
     plans = (ctypes.c_void_p * n)(*[j[0].handle for j in jobs])
     all_src = []
@@ -407,8 +414,9 @@ def process_batch(jobs, flags, stream=None):
     src_arr = (Plane * len(all_src))(*[_plane(s) for s in all_src])
     dst_arr = (MutPlane * n)(*[_plane(j[2], MutPlane) for j in jobs])
     zeros = (ctypes.c_int * n)()
+    fr = (ctypes.c_int * n)(*([-1] * n if frames is None else frames))
     _check(lib().nf_process_batch(plans, n, src_arr, (ctypes.c_int * n)(*first), (ctypes.c_int * n)(*counts),
-                                  dst_arr, zeros, zeros, ctypes.c_void_p(flags.data_ptr()), _stream_handle(stream)))
+                                  dst_arr, fr, zeros, ctypes.c_void_p(flags.data_ptr()), _stream_handle(stream)))
 
 
 class KalmanState:

@@ -326,3 +326,56 @@ def test_fft3d_launch_shape_invariance(nf, oracle, monkeypatch, nb):
     got = _run(plan, frames)
     ref = oracle.fft3d_plane(frames, bits=8, **params)
     assert np.array_equal(got, ref), compare(got, ref)
+
+
+@pytest.mark.parametrize("bt,bw", [(3, 32), (5, 16), (2, 8)])
+def test_fft3d_spectrum_cache_sequences(nf, oracle, bt, bw):
+    """Reusing cached source spectra across outputs gives the same bits as
+    computing every slot: sequential order, seeks, and a frame number whose
+    plane moved to another buffer."""
+    N, h, w = 10, 96, 130
+    frames = _clip_frames(N, h, w, 8, 60 + bt)
+    dev = [to_device(f) for f in frames]
+    params = dict(bw=bw, bh=bw, bt=bt, sigma=3.0)
+    plan = nf.fft3d_plan(w, h, bits=8, **params)
+    nf.set_spectrum_cache(plan, bt + 1)
+    c = bt // 2
+    order = list(range(c, N - (bt - 1 - c))) + [6, 2, 7, 3]
+    for n in order:
+        slots = [n - c + j for j in range(bt)]
+        dst = empty_like_np(frames[0])
+        plan.process([dev[s] for s in slots], dst, frame=n)
+        got = to_host(dst, np.uint8)
+        ref = oracle.fft3d_plane([frames[s] for s in slots], bits=8, **params)
+        assert np.array_equal(got, ref), (n, compare(got, ref))
+    # frame c's plane now lives elsewhere with other content: recomputed
+    moved = np.ascontiguousarray(255 - frames[c])
+    dev_moved = to_device(moved)
+    slots = [c - c + j for j in range(bt)]
+    srcs = [dev_moved if s == c else dev[s] for s in slots]
+    dst = empty_like_np(frames[0])
+    plan.process(srcs, dst, frame=c)
+    ref = oracle.fft3d_plane([moved if s == c else frames[s] for s in slots], bits=8, **params)
+    assert np.array_equal(to_host(dst, np.uint8), ref)
+
+
+def test_fft3d_spectrum_cache_batch(nf, oracle):
+    import torch
+    frames = yuv420_sinusoids(160, 96, frames=6, bits=16, seed=9)
+    params = dict(bw=32, bh=32, ow=16, oh=16, bt=3, sigma=2.0)
+    plans = [nf.fft3d_plan(160 if p == 0 else 80, 96 if p == 0 else 48, bits=16, chroma=p > 0, **params)
+             for p in range(3)]
+    for pl in plans:
+        nf.set_spectrum_cache(pl, 4)
+    dev = [[to_device(pl) for pl in fr] for fr in frames]
+    flags = torch.zeros(1, dtype=torch.int32, device="cuda")
+    for n in range(1, 5):
+        outs = [empty_like_np(frames[0][p]) for p in range(3)]
+        jobs = [(plans[p], [dev[n - 1][p], dev[n][p], dev[n + 1][p]], outs[p]) for p in range(3)]
+        nf.process_batch(jobs, flags, frames=[n, n, n])
+        torch.cuda.synchronize()
+        assert int(flags.item()) == 0
+        for p in range(3):
+            ref = oracle.fft3d_plane([frames[n - 1][p], frames[n][p], frames[n + 1][p]], bits=16, chroma=p > 0,
+                                     **params)
+            assert np.array_equal(to_host(outs[p], np.uint16), ref), (n, p)
