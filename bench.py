@@ -161,7 +161,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": measured_traffic("fft3d_block_kernel<32,32>"),
             "bytes_per_launch": BYTES_PER_FRAME,
             "avg_launch_ms": round(blk_avg, 5),
         },
@@ -231,6 +231,18 @@ def main():
             result["gpu_over_cpu"] = round(fps / result["cpu_baseline"]["value"], 1)
     print(json.dumps(result))
     return 0
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the same counter
+    on a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["kernels"][kernel]["bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def mfma_error(nf, torch):

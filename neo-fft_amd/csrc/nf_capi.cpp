@@ -1121,6 +1121,10 @@ void enqueue_fft3d_jobs(const std::vector<const nf_plan*>& plans, std::vector<Ff
     for (size_t i = first; i < first + n; ++i) {
       stateless = stateless && jobs[i].mode == 0 && jobs[i].T == plans[i]->temporal;
       ncand = std::min(ncand, plans[i]->fft3d_nbs.size());
+      // with spectrum reuse, tune on a call that reuses (the steady state)
+      bool reuse = false;
+      for (int s = 0; s < jobs[i].T; ++s) reuse = reuse || jobs[i].cache_in[s];
+      if (plans[i]->spectra && !reuse) stateless = false;
     }
     if (stateless)
       autotune(group, (int)ncand, stream, [&](int c) {
