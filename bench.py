@@ -366,9 +366,85 @@ def other_configs(nf, torch, clip, stream, flags, args):
     c5(0)
     ms = _timed(torch, stream, c5, 10)
     out["wiener_adjoint_16x256x256"] = {"batches_per_s": round(10 / (ms / 1e3), 1), "ms_per_batch": round(ms / 10, 4)}
+    out["batched_64_clips"] = batched_clips(nf, torch, stream)
     if int(flags.item()) != 0:
         out["error"] = "non-finite flag raised"
     return out
+
+
+BATCH_SIZES = [(640, 360), (1280, 720), (1920, 1080), (3840, 2160)]
+
+
+def batched_clips(nf, torch, stream, frames=8, reps=2):
+    """64 clips of uneven size (16 each at 640x360, 1280x720, 1920x1080 and
+    3840x2160, YUV420P16, 8 frames, the sinusoid generator evaluated on the
+    device) through DFTTest tbsize=3 tmode=0 at sbsize 8/16/32 (sosize
+    6/12/24): per frame index ONE persistent launch over all 192 planes, the
+    8 launches of the frame group captured in one hipGraph and replayed;
+    frames resident in HBM. The schedule of the first launch is autotuned and
+    reused by the others. For sbsize 16 also the matrix-core form, and the
+    same frame index as 64 per-clip nf_process_batch calls (unbatched)."""
+    from synthetic import yuv420_sinusoids_device
+    dev = torch.device("cuda:0")
+    clips, sizes = [], []
+    for i in range(64):
+        w, h = BATCH_SIZES[i // 16]
+        sizes.append((w, h))
+        clips.append(yuv420_sinusoids_device(w, h, frames, bits=16, seed=1000 + i, device=dev))
+    outs = [[[torch.empty_like(clips[c][n][p]) for p in range(3)] for n in range(frames)] for c in range(64)]
+    flags = torch.zeros(1, dtype=torch.int32, device=dev)
+    bytes_per_index = sum(2 * 2 * (w * h + 2 * (w // 2) * (h // 2)) for w, h in sizes)
+    res = {"workload": "64 clips (16 each 640x360, 1280x720, 1920x1080, 3840x2160) YUV420P16 x 8 frames, "
+                       "DFTTest tbsize=3 tmode=0 sigma=8, one persistent launch per frame index, "
+                       "hipGraph per 8-frame group",
+           "bytes_per_frame_index": bytes_per_index}
+    torch.cuda.synchronize()
+    for B, O, form in ((8, 6, "radix2"), (16, 12, "radix2"), (16, 12, "mfma_f16x3"), (32, 24, "radix2")):
+        plans = {}
+        for w, h in BATCH_SIZES:
+            for ch in (False, True):
+                pl = nf.dfttest_plan(w // 2 if ch else w, h // 2 if ch else h, bits=16, chroma=ch, tbsize=3, tmode=0,
+                                     sbsize=B, sosize=O, ftype=0, sigma=8.0)
+                if form != "radix2":
+                    nf.set_transform(pl, nf.TRANSFORM_MFMA_F16X3)
+                plans[(w, h, ch)] = pl
+
+        def jobs_for(n, only=None):
+            idx = [min(max(n - 1 + z, 0), frames - 1) for z in range(3)]
+            cs = range(64) if only is None else [only]
+            return [(plans[sizes[c] + (p > 0,)], [clips[c][f][p] for f in idx], outs[c][n][p])
+                    for c in cs for p in range(3)]
+
+        t = time.perf_counter()
+        first = nf.Batch(jobs_for(0), stream=stream)
+        torch.cuda.synchronize()
+        tune_s = time.perf_counter() - t
+        sched = first.schedule()
+        batches = [first] + [nf.Batch(jobs_for(n), stream=stream, schedule=sched["choice"])
+                             for n in range(1, frames)]
+        graph = nf.Graph(batches, flags)
+        graph.launch(stream)
+        ms = _timed(torch, stream, lambda i: graph.launch(stream), reps) / reps
+        key = f"sbsize{B}_{form}"
+        res[key] = {"clip_frames_per_s": round(64 * frames / (ms / 1e3), 1),
+                    "ms_per_frame_index": round(ms / frames, 3),
+                    "achieved_gbs": round(bytes_per_index * frames / (ms / 1e3) / 1e9, 1),
+                    "hbm_frac": round(bytes_per_index * frames / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                    "schedule": sched, "tune_s": round(tune_s, 2)}
+        if B == 16 and form == "radix2":
+            def unbatched(i):
+                for c in range(64):
+                    nf.process_batch(jobs_for(i % frames, only=c), flags, stream=stream)
+            unbatched(0)
+            ums = _timed(torch, stream, unbatched, frames)
+            res[key]["unbatched_ms_per_frame_index"] = round(ums / frames, 3)
+            res[key]["batched_speedup"] = round(ums / frames / (ms / frames), 3)
+        del graph, batches, first
+    res["mfma_speedup_sbsize16"] = round(res["sbsize16_mfma_f16x3"]["clip_frames_per_s"]
+                                         / res["sbsize16_radix2"]["clip_frames_per_s"], 3)
+    if int(flags.item()) != 0:
+        res["error"] = "non-finite flag raised"
+    return res
 
 
 def _resident_frames(torch, clip, dev):

@@ -16,8 +16,13 @@
  * the calling thread's last failure. A frame that fails must not be used:
  * the destination plane may hold partial results.
  *
- * Threading: plans are immutable after creation and may be used from several
- * host threads at once (one stream per thread, or the default stream).
+ * Threading: a plan may be used from several host threads at once (one
+ * stream per thread, or the default stream). Its tables are immutable after
+ * creation; the only state set later is set once and guarded: the launch
+ * shape picked by the autotuner on the first call, and a sampled noise model
+ * published by nf_dfttest_prepare_noise / nf_fft3d_prepare_pattern. The
+ * optional FFT3D spectrum cache (nf_plan_set_spectrum_cache) is the
+ * exception: one stream at a time per plan while it is enabled.
  */
 #ifndef NEO_FFT_H
 #define NEO_FFT_H
@@ -143,6 +148,42 @@ nf_status nf_process_batch(const nf_plan* const* plans, int n_jobs, const nf_pla
                            const int* first_source, const int* n_sources, const nf_mut_plane* dsts,
                            const int* frames, const int* planes, unsigned* device_flags,
                            void* stream);
+
+/* Persistent work-list batch: every block of every DFTTest plane of a frame
+ * group (all planes of any number of clips of uneven size, tmode=0, one
+ * sbsize, tbsize and transform form) in ONE launch. Jobs are given as for
+ * nf_process_batch; the job descriptors and the tile work list are built and
+ * uploaded once. The launch shape (tile size, LDS chunk of blocks) is the
+ * candidate schedule `schedule` (an index, as reported by nf_batch_schedule
+ * for a batch of the same plans), or with schedule = -1 the fastest candidate,
+ * found by timing each on `stream` at creation (this runs the batch and writes
+ * dsts; NF_AUTOTUNE=0 keeps the first candidate instead). Each
+ * workgroup takes tiles from a device ticket until the list is exhausted.
+ * Results are bit-identical to nf_plan_process of every job. The planes'
+ * memory must stay valid while the batch is used; plans must outlive it; one
+ * stream at a time per batch. No counterpart in the reference, which runs one
+ * plane per Plan::run call (plan.cpp:331-640). */
+typedef struct nf_batch nf_batch;
+nf_status nf_batch_create(const nf_plan* const* plans, int n_jobs, const nf_plane* sources,
+                          const int* first_source, const int* n_sources, const nf_mut_plane* dsts,
+                          int schedule, void* stream, nf_batch** out);
+/* One launch of the batch; asynchronous like nf_plan_enqueue, allocates
+ * nothing, capturable. */
+nf_status nf_batch_enqueue(const nf_batch* batch, unsigned* device_flags, void* stream);
+/* Chosen schedule: out = {work items, workgroups, tile width, tile height,
+ * blocks per LDS chunk (0 = whole tile row), dynamic LDS bytes, candidate
+ * schedules, index of the chosen one}; *tuned_ms = its time per launch when
+ * tuned (else 0). */
+nf_status nf_batch_schedule(const nf_batch* batch, int out[8], float* tuned_ms);
+void nf_batch_destroy(nf_batch* batch);
+
+/* hipGraph of a frame group: the launches of `batches` in order, captured
+ * once and replayed by nf_graph_launch on any stream. Batches must outlive the
+ * graph; device_flags receives the NF_FLAG_* bits of every replay. */
+typedef struct nf_graph nf_graph;
+nf_status nf_graph_capture(const nf_batch* const* batches, int n, unsigned* device_flags, nf_graph** out);
+nf_status nf_graph_launch(const nf_graph* graph, void* stream);
+void nf_graph_destroy(nf_graph* graph);
 
 /* DFTTest tmode=1 with tosize=0: every frame of the lattice block
  * [s, s+tbsize) has that block as its only covering block (reference

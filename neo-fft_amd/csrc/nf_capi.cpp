@@ -1462,6 +1462,335 @@ extern "C" nf_status nf_dfttest_enqueue_volume(const nf_plan* plan, const nf_pla
   });
 }
 
+// ------------------------------------------------ persistent work-list batches
+// One launch over any number of DFTTest planes of clips of any size, tiles
+// taken from a device work list (nf_worklist.hpp). No counterpart in the
+// reference, which processes one plane per Plan::run call.
+namespace {
+struct BatchShape {
+  int tile_w = 0, tile_h = 0, chunk = 0, tile_index = -1;
+  size_t lds = 0;
+};
+
+// Device work list of one schedule: [jobs][temporal inputs][order] in one
+// allocation; the ticket words live in the batch.
+struct BatchTables {
+  char* base = nullptr;
+  DftWorkList W{};
+  int workgroups = 0;
+  size_t lds = 0;
+  BatchTables() = default;
+  BatchTables(const BatchTables&) = delete;
+  BatchTables& operator=(const BatchTables&) = delete;
+  BatchTables(BatchTables&& o) noexcept { *this = std::move(o); }
+  BatchTables& operator=(BatchTables&& o) noexcept {
+    std::swap(base, o.base);
+    W = o.W;
+    workgroups = o.workgroups;
+    lds = o.lds;
+    return *this;
+  }
+  ~BatchTables() {
+    if (base) (void)hipFree(base);
+  }
+};
+}  // namespace
+
+struct nf_batch {
+  int kind = 0;  // 0 spatial (tbsize 1), 1 temporal, 2 matrix-core form
+  int B = 0, T = 1, max_depth = 1;
+  const float* tw = nullptr;
+  std::vector<const nf_plan*> plans;
+  std::vector<DftJob> jobs;
+  std::vector<DftTemporal> tq;
+  std::vector<BatchShape> shapes;
+  int choice = 0;
+  float tuned_ms = 0.0f;
+  BatchTables tables;
+  unsigned* ticket = nullptr;
+  ~nf_batch() {
+    if (ticket) (void)hipFree(ticket);
+  }
+};
+
+struct nf_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~nf_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+namespace {
+// Candidate schedules of a batch: the tile shapes of the plans (spatial and
+// matrix-core forms) or tile shapes with an LDS chunk budget (temporal form,
+// per-frame ring accumulator).
+void batch_shapes(nf_batch& b) {
+  if (b.kind == 1) {
+    static const int cand[][3] = {{64, 128, 76}, {64, 64, 76}, {32, 128, 76}, {128, 128, 76},
+                                  {64, 256, 76}, {64, 128, 50}, {32, 64, 50}};
+    for (const auto& c : cand) {
+      BatchShape s;
+      s.tile_w = c[0];
+      s.tile_h = c[1];
+      s.chunk = 1 << 30;
+      bool fits = true;
+      for (auto* p : b.plans) {
+        if (dfttest_vol_lds_bytes(p->B, b.T, 1, p->S, 1, p->B, c[0]) > (size_t)kListLdsLimit) fits = false;
+        const int nbx = (c[0] + p->B - 2) / p->S + 1;
+        int k = 1;
+        while (k < nbx && dfttest_vol_lds_bytes(p->B, b.T, k + 1, p->S, 1, p->B, c[0]) <= (size_t)c[2] * 1024) ++k;
+        s.chunk = std::min(s.chunk, k);
+      }
+      if (!fits) continue;
+      for (auto* p : b.plans) s.lds = std::max(s.lds, dfttest_vol_lds_bytes(p->B, b.T, s.chunk, p->S, 1, p->B, c[0]));
+      b.shapes.push_back(s);
+    }
+  } else {
+    size_t ncand = b.plans[0]->dft_tiles.size();
+    for (auto* p : b.plans) ncand = std::min(ncand, p->dft_tiles.size());
+    for (size_t c = 0; c < ncand; ++c) {
+      BatchShape s;
+      s.tile_index = (int)c;
+      s.tile_w = b.plans[0]->dft_tiles[c].tile_w;
+      s.tile_h = b.plans[0]->dft_tiles[c].tile_h;
+      for (auto* p : b.plans) {
+        const auto& t = p->dft_tiles[c];
+        s.lds = std::max(s.lds, b.kind == 2 ? dfttest_mfma_lds_bytes(b.B, b.T, t.nbx_max, t.ww_max, t.tile_w) : t.lds);
+      }
+      if (s.lds <= (size_t)kListLdsLimit) b.shapes.push_back(s);
+    }
+  }
+  if (b.shapes.empty()) throw Unsupported("DFTTest batch: no tile schedule fits the LDS");
+}
+
+const void* batch_function(const nf_batch& b) {
+  return b.kind == 0 ? dfttest_t1_list_function(b.B)
+                     : b.kind == 1 ? dfttest_vol_list_function(b.B, b.max_depth) : dfttest_mfma_list_function(b.B, b.T);
+}
+
+// Builds and uploads the work list of schedule `s`: jobs tiled by the shape,
+// items ordered by decreasing tile area (full tiles in raster order first,
+// clipped edge tiles last), workgroups = resident workgroups of the device.
+BatchTables build_tables(const nf_batch& b, const BatchShape& s, unsigned* flags) {
+  std::vector<DftJob> jobs(b.jobs.size());
+  std::vector<DftTemporal> tq(b.tq);
+  struct Item {
+    int job, tile;
+    long long area;
+  };
+  std::vector<Item> items;
+  for (size_t i = 0; i < b.jobs.size(); ++i) {
+    const nf_plan& p = *b.plans[i];
+    DftJob j = b.jobs[i];
+    if (s.tile_index >= 0) {
+      j = apply_tile(j, p, s.tile_index);
+    } else {
+      j.tile_w = s.tile_w;
+      j.tile_h = s.tile_h;
+      j.tile_x0 = p.gx.offset / s.tile_w;
+      j.tile_y0 = p.gy.offset / s.tile_h;
+      j.tiles_x = (p.gx.offset + p.gx.length + s.tile_w - 1) / s.tile_w - j.tile_x0;
+      j.tiles_y = (p.gy.offset + p.gy.length + s.tile_h - 1) / s.tile_h - j.tile_y0;
+      tq[i].chunk = s.chunk;
+    }
+    jobs[i] = j;
+    for (int t = 0; t < j.tiles_x * j.tiles_y; ++t) {
+      const int tx = t % j.tiles_x, ty = t / j.tiles_x;
+      const int x0 = std::max(p.gx.offset, (j.tile_x0 + tx) * j.tile_w);
+      const int x1 = std::min(p.gx.offset + p.gx.length, (j.tile_x0 + tx + 1) * j.tile_w);
+      const int y0 = std::max(p.gy.offset, (j.tile_y0 + ty) * j.tile_h);
+      const int y1 = std::min(p.gy.offset + p.gy.length, (j.tile_y0 + ty + 1) * j.tile_h);
+      items.push_back({(int)i, t, (long long)(x1 - x0) * (y1 - y0)});
+    }
+  }
+  require(!items.empty() && items.size() < (size_t)INT32_MAX, "DFTTest batch: invalid work list size");
+  std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& c) { return a.area > c.area; });
+  std::vector<int2> order(items.size());
+  for (size_t k = 0; k < items.size(); ++k) order[k] = make_int2(items[k].job, items[k].tile);
+
+  const size_t jb = jobs.size() * sizeof(DftJob);
+  const size_t qo = (jb + 255) & ~size_t(255);
+  const size_t qb = b.kind == 0 ? 0 : tq.size() * sizeof(DftTemporal);
+  const size_t oo = (qo + qb + 255) & ~size_t(255);
+  const size_t total = oo + order.size() * sizeof(int2);
+  std::vector<char> host(total, 0);
+  std::memcpy(host.data(), jobs.data(), jb);
+  if (qb) std::memcpy(host.data() + qo, tq.data(), qb);
+  std::memcpy(host.data() + oo, order.data(), order.size() * sizeof(int2));
+  BatchTables t;
+  hip_check(hipMalloc((void**)&t.base, total), "hipMalloc(batch work list)");
+  hip_check(hipMemcpy(t.base, host.data(), total, hipMemcpyHostToDevice), "hipMemcpy(batch work list)");
+  t.W.jobs = reinterpret_cast<const DftJob*>(t.base);
+  t.W.tq = qb ? reinterpret_cast<const DftTemporal*>(t.base + qo) : nullptr;
+  t.W.order = reinterpret_cast<const int2*>(t.base + oo);
+  t.W.items = (int)order.size();
+  t.W.ticket = b.ticket;
+  t.W.flags = flags;
+  t.lds = s.lds;
+  int dev = 0, cus = 0, per_cu = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+  hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_function(b), 256, s.lds),
+            "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+  require(per_cu > 0, "DFTTest batch: schedule does not fit a compute unit");
+  t.workgroups = (int)std::min<long long>((long long)per_cu * cus, order.size());
+  return t;
+}
+
+void launch_batch(const nf_batch& b, const BatchTables& t, unsigned* flags, hipStream_t stream) {
+  DftWorkList W = t.W;
+  W.flags = flags;
+  hipError_t e;
+  if (b.kind == 0) e = launch_dfttest_t1_list(b.B, W, t.workgroups, t.lds, stream);
+  else if (b.kind == 1) e = launch_dfttest_vol_list(b.B, b.max_depth, W, t.workgroups, t.lds, stream);
+  else e = launch_dfttest_mfma_list(b.B, b.T, W, t.workgroups, t.lds, b.tw, stream);
+  hip_check(e, "dfttest batch launch");
+}
+}  // namespace
+
+extern "C" nf_status nf_batch_create(const nf_plan* const* plans, int n_jobs, const nf_plane* sources,
+                                     const int* first_source, const int* n_sources, const nf_mut_plane* dsts,
+                                     int schedule, void* stream, nf_batch** out) {
+  return guarded([&]() -> nf_status {
+    require(out != nullptr, "null argument");
+    *out = nullptr;
+    require(plans && n_jobs > 0 && sources && first_source && n_sources && dsts, "null argument");
+    ensure_lds_limits();
+    auto b = std::make_unique<nf_batch>();
+    const nf_plan* p0 = plans[0];
+    require(p0 && p0->algorithm == 0, "batches take DFTTest plans");
+    b->B = p0->B;
+    b->T = p0->temporal;
+    b->kind = p0->transform == NF_TRANSFORM_MFMA_F16X3 ? 2 : (p0->temporal > 1 ? 1 : 0);
+    if (b->kind == 2) {
+      require(dfttest_mfma_supported(b->B, b->T), "matrix-core form needs sbsize 16 and tbsize 1, 3 or 5");
+      b->tw = p0->d_tw;
+    }
+    for (int i = 0; i < n_jobs; ++i) {
+      const nf_plan* p = plans[i];
+      require(p && p->algorithm == 0, "batches take DFTTest plans");
+      require(p->B == b->B, "batched DFTTest jobs must share sbsize");
+      require(p->temporal == b->T, "batched DFTTest jobs must share tbsize");
+      require((p->transform == NF_TRANSFORM_MFMA_F16X3) == (b->kind == 2),
+              "batched DFTTest jobs must share the transform form");
+      require(p->tmode == 0 || p->temporal == 1, "batched DFTTest jobs take tmode=0 planes (use nf_plan_enqueue)");
+      require(!(p->dither > 0 && p->storage == kU8), "dithered DFTTest planes use nf_plan_enqueue");
+      const nf_plane* s = sources + first_source[i];
+      const int ns = n_sources[i];
+      require(ns == p->temporal, "DFTTest sources size must match plan temporal_size");
+      check_plane(*p, dsts[i].width, dsts[i].height, dsts[i].stride_bytes, dsts[i].data);
+      for (int k = 0; k < ns; ++k) check_plane(*p, s[k].width, s[k].height, s[k].stride_bytes, s[k].data);
+      b->plans.push_back(p);
+      b->jobs.push_back(make_dft_job(*p, s[0], dsts[i]));
+      if (b->kind != 0) {
+        b->tq.push_back(make_temporal(*p, s, ns, nullptr, 0, 0));
+        b->max_depth = std::max(b->max_depth, b->tq.back().T);
+      }
+    }
+    hip_check(hipMalloc((void**)&b->ticket, 2 * sizeof(unsigned)), "hipMalloc(batch ticket)");
+    hip_check(hipMemset(b->ticket, 0, 2 * sizeof(unsigned)), "hipMemset(batch ticket)");
+    batch_shapes(*b);
+    require(schedule >= -1 && schedule < (int)b->shapes.size(), "DFTTest batch: schedule index out of range");
+    if (schedule >= 0) b->choice = schedule;
+    // Autotuner: time every schedule (one warm-up and three timed launches,
+    // identical outputs) and keep the fastest.
+    const hipStream_t st = (hipStream_t)stream;
+    if (schedule < 0 && b->shapes.size() > 1 && autotune_enabled(st)) {
+      unsigned* flags = nullptr;
+      hip_check(hipMalloc((void**)&flags, sizeof(unsigned)), "hipMalloc(tuning flags)");
+      hipEvent_t e0, e1;
+      hip_check(hipEventCreate(&e0), "hipEventCreate");
+      hip_check(hipEventCreate(&e1), "hipEventCreate");
+      float best = 0.0f;
+      for (size_t c = 0; c < b->shapes.size(); ++c) {
+        BatchTables t = build_tables(*b, b->shapes[c], flags);
+        launch_batch(*b, t, flags, st);
+        hip_check(hipEventRecord(e0, st), "hipEventRecord");
+        for (int r = 0; r < 3; ++r) launch_batch(*b, t, flags, st);
+        hip_check(hipEventRecord(e1, st), "hipEventRecord");
+        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        if (c == 0 || ms < best) {
+          best = ms;
+          b->choice = (int)c;
+        }
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      (void)hipFree(flags);
+      b->tuned_ms = best / 3.0f;
+    }
+    b->tables = build_tables(*b, b->shapes[b->choice], nullptr);
+    *out = b.release();
+    return NF_OK;
+  });
+}
+
+extern "C" nf_status nf_batch_enqueue(const nf_batch* batch, unsigned* device_flags, void* stream) {
+  return guarded([&]() -> nf_status {
+    require(batch && device_flags, "null argument");
+    timed(0, (hipStream_t)stream, [&] { launch_batch(*batch, batch->tables, device_flags, (hipStream_t)stream); });
+    return NF_OK;
+  });
+}
+
+extern "C" nf_status nf_batch_schedule(const nf_batch* batch, int out[8], float* tuned_ms) {
+  return guarded([&]() -> nf_status {
+    require(batch && out, "null argument");
+    const BatchShape& s = batch->shapes[batch->choice];
+    const int v[8] = {batch->tables.W.items, batch->tables.workgroups, s.tile_w, s.tile_h, s.chunk > (1 << 29) ? 0 : s.chunk,
+                      (int)s.lds, (int)batch->shapes.size(), batch->choice};
+    std::memcpy(out, v, sizeof(v));
+    if (tuned_ms) *tuned_ms = batch->tuned_ms;
+    return NF_OK;
+  });
+}
+
+extern "C" void nf_batch_destroy(nf_batch* batch) { delete batch; }
+
+extern "C" nf_status nf_graph_capture(const nf_batch* const* batches, int n, unsigned* device_flags, nf_graph** out) {
+  return guarded([&]() -> nf_status {
+    require(out != nullptr, "null argument");
+    *out = nullptr;
+    require(batches && n > 0 && device_flags, "null argument");
+    for (int i = 0; i < n; ++i) require(batches[i] != nullptr, "null batch");
+    auto g = std::make_unique<nf_graph>();
+    hipStream_t s = nullptr;
+    hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      try {
+        for (int i = 0; i < n; ++i) launch_batch(*batches[i], batches[i]->tables, device_flags, s);
+      } catch (...) {
+        hipGraph_t broken = nullptr;
+        (void)hipStreamEndCapture(s, &broken);
+        if (broken) (void)hipGraphDestroy(broken);
+        (void)hipStreamDestroy(s);
+        throw;
+      }
+      e = hipStreamEndCapture(s, &g->graph);
+    }
+    (void)hipStreamDestroy(s);
+    hip_check(e, "hipStreamCapture");
+    hip_check(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    *out = g.release();
+    return NF_OK;
+  });
+}
+
+extern "C" nf_status nf_graph_launch(const nf_graph* graph, void* stream) {
+  return guarded([&]() -> nf_status {
+    require(graph != nullptr, "null argument");
+    hip_check(hipGraphLaunch(graph->exec, (hipStream_t)stream), "hipGraphLaunch");
+    return NF_OK;
+  });
+}
+
+extern "C" void nf_graph_destroy(nf_graph* graph) { delete graph; }
+
 // ------------------------------------------------- differentiable Wiener path
 namespace {
 void require_wiener(const nf_plan* plan) {

@@ -106,6 +106,25 @@ struct DftVolLaunch {
   unsigned* flags;
 };
 
+// Work list of one persistent DFTTest launch over any number of jobs (planes
+// of clips of any size): job descriptors live in device memory, `order` lists
+// the (job, tile) work items in schedule order, and each workgroup takes the
+// next item from ticket[0] until the list is exhausted. The last workgroup to
+// finish returns ticket[0] and ticket[1] (finished workgroups) to zero, so the
+// list can be launched again, also from a hipGraph, without a reset.
+// dynamic LDS bound of the work-list kernels (their loop holds a few bytes of
+// static LDS)
+constexpr int kListLdsLimit = 160 * 1024 - 256;
+
+struct DftWorkList {
+  const DftJob* jobs;
+  const DftTemporal* tq;  // tbsize > 1 and matrix-core lists, else nullptr
+  const int2* order;      // (job, tile) per ticket
+  int items;
+  unsigned* ticket;       // [2], zero between launches
+  unsigned* flags;
+};
+
 // One FFT3D plane of one output frame: block stage writes windowed
 // contributions, the synthesis stage sums them in the canonical order.
 // LDS layout helpers. Spectra are kept as separate real and imaginary planes

@@ -14,17 +14,18 @@
 #include "nf_fft.hpp"
 #include "nf_gain.hpp"
 #include "nf_launch.hpp"
+#include "nf_worklist.hpp"
 
 namespace nf {
 
+namespace {
+
+// One output tile of job J; ORs NF_FLAG_NONFINITE into *flags on a non-finite
+// sample or intermediate.
 template <int B>
-__global__ __launch_bounds__(256) void dfttest_t1_kernel(const DftLaunch L) {
+__device__ __forceinline__ void t1_tile(const DftJob& J, int tile, unsigned* flags) {
   extern __shared__ float smem[];
   constexpr int K = B / 2 + 1;
-  int jid = 0;
-  while (jid + 1 < L.njobs && (int)blockIdx.x >= L.tile_start[jid + 1]) ++jid;
-  const DftJob& J = L.jobs[jid];
-  const int tile = (int)blockIdx.x - L.tile_start[jid];
   const int tx = tile % J.tiles_x, ty = tile / J.tiles_x;
   const int S = J.S;
   const int tid = threadIdx.x, nthreads = blockDim.x;
@@ -223,7 +224,24 @@ __global__ __launch_bounds__(256) void dfttest_t1_kernel(const DftLaunch L) {
     // The next block row's stages A..E touch neither the ring nor the rows
     // cleared above; stage F runs after four more barriers.
   }
-  if (bad) atomicOr(L.flags, NF_FLAG_NONFINITE);
+  if (bad) atomicOr(flags, NF_FLAG_NONFINITE);
+}
+
+}  // namespace
+
+template <int B>
+__global__ __launch_bounds__(256) void dfttest_t1_kernel(const DftLaunch L) {
+  int jid = 0;
+  while (jid + 1 < L.njobs && (int)blockIdx.x >= L.tile_start[jid + 1]) ++jid;
+  t1_tile<B>(L.jobs[jid], (int)blockIdx.x - L.tile_start[jid], L.flags);
+}
+
+// sbsize 32 holds two waves per SIMD; the explicit budget keeps the loop
+// form free of spills
+template <int B>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(B == 32 ? 2 : 1))) void dfttest_t1_list_kernel(
+    const DftWorkList W) {
+  for_each_work_item(W, [&](int j, int tile) { t1_tile<B>(list_job(W, j), tile, W.flags); });
 }
 
 size_t dfttest_t1_lds_bytes(int B, const DftJob& j) {
@@ -252,12 +270,47 @@ hipError_t launch_dfttest_t1(int B, const DftLaunch& L, int total_tiles, size_t 
   return hipGetLastError();
 }
 
+hipError_t launch_dfttest_t1_list(int B, const DftWorkList& W, int workgroups, size_t lds, hipStream_t stream) {
+  dim3 grid(workgroups), block(256);
+  switch (B) {
+    case 8:
+      hipLaunchKernelGGL(dfttest_t1_list_kernel<8>, grid, block, lds, stream, W);
+      break;
+    case 16:
+      hipLaunchKernelGGL(dfttest_t1_list_kernel<16>, grid, block, lds, stream, W);
+      break;
+    case 32:
+      hipLaunchKernelGGL(dfttest_t1_list_kernel<32>, grid, block, lds, stream, W);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+const void* dfttest_t1_list_function(int B) {
+  switch (B) {
+    case 8: return (const void*)dfttest_t1_list_kernel<8>;
+    case 16: return (const void*)dfttest_t1_list_kernel<16>;
+    case 32: return (const void*)dfttest_t1_list_kernel<32>;
+    default: return nullptr;
+  }
+}
+
 hipError_t dfttest_t1_set_lds_limit() {
-  hipError_t e = hipFuncSetAttribute((const void*)dfttest_t1_kernel<8>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void*)dfttest_t1_kernel<16>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void*)dfttest_t1_kernel<32>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  const void* fns[] = {(const void*)dfttest_t1_kernel<8>, (const void*)dfttest_t1_kernel<16>,
+                       (const void*)dfttest_t1_kernel<32>};
+  const void* lists[] = {(const void*)dfttest_t1_list_kernel<8>, (const void*)dfttest_t1_list_kernel<16>,
+                         (const void*)dfttest_t1_list_kernel<32>};
+  for (const void* f : fns) {
+    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+  }
+  for (const void* f : lists) {
+    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kListLdsLimit);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace nf

@@ -21,6 +21,7 @@
 #include "nf_fft.hpp"
 #include "nf_gain.hpp"
 #include "nf_launch.hpp"
+#include "nf_worklist.hpp"
 
 namespace nf {
 
@@ -140,17 +141,12 @@ __device__ __forceinline__ void temporal_stages(const DftJob& J, const DftTempor
   }
 }
 
-}  // namespace
-
+// One output tile of job J with temporal inputs Q; ORs NF_FLAG_NONFINITE
+// into *flags on a non-finite sample or intermediate.
 template <int B, bool kAnyDepth>
-__global__ __launch_bounds__(256) void dfttest_vol_kernel(const DftVolLaunch L) {
+__device__ __forceinline__ void vol_tile(const DftJob& J, const DftTemporal& Q, int tile, unsigned* flags) {
   extern __shared__ float smem[];
   constexpr int K = B / 2 + 1;
-  int jid = 0;
-  while (jid + 1 < L.njobs && (int)blockIdx.x >= L.tile_start[jid + 1]) ++jid;
-  const DftJob& J = L.jobs[jid];
-  const DftTemporal& Q = L.tq[jid];
-  const int tile = (int)blockIdx.x - L.tile_start[jid];
   const int tx = tile % J.tiles_x, ty = tile / J.tiles_x;
   const int S = J.S, T = Q.T, NBc = Q.chunk, nout = Q.nout;
   const int tid = threadIdx.x, nthreads = blockDim.x;
@@ -378,7 +374,23 @@ __global__ __launch_bounds__(256) void dfttest_vol_kernel(const DftVolLaunch L) 
       else store_dft_output(J, J.dst, y - vy0, x - vx0, a);
     }
   }
-  if (bad) atomicOr(L.flags, NF_FLAG_NONFINITE);
+  if (bad) atomicOr(flags, NF_FLAG_NONFINITE);
+}
+
+}  // namespace
+
+template <int B, bool kAnyDepth>
+__global__ __launch_bounds__(256) void dfttest_vol_kernel(const DftVolLaunch L) {
+  int jid = 0;
+  while (jid + 1 < L.njobs && (int)blockIdx.x >= L.tile_start[jid + 1]) ++jid;
+  vol_tile<B, kAnyDepth>(L.jobs[jid], L.tq[jid], (int)blockIdx.x - L.tile_start[jid], L.flags);
+}
+
+template <int B, bool kAnyDepth>
+__global__ __launch_bounds__(256) void dfttest_vol_list_kernel(const DftWorkList W) {
+  for_each_work_item(W, [&](int j, int tile) {
+    vol_tile<B, kAnyDepth>(list_job(W, j), list_temporal(W, j), tile, W.flags);
+  });
 }
 
 size_t dfttest_vol_lds_bytes(int B, int T, int chunk, int S, int nout, int acc_rows, int tile_w) {
@@ -412,6 +424,25 @@ hipError_t launch_dfttest_vol(int B, int max_depth, const DftVolLaunch& L, int t
   return hipGetLastError();
 }
 
+hipError_t launch_dfttest_vol_list(int B, int max_depth, const DftWorkList& W, int workgroups, size_t lds,
+                                   hipStream_t stream) {
+  dim3 grid(workgroups), block(256);
+  const void* f = dfttest_vol_list_function(B, max_depth);
+  if (!f) return hipErrorInvalidValue;
+  void* args[] = {const_cast<DftWorkList*>(&W)};
+  return hipLaunchKernel(f, grid, block, args, lds, stream);
+}
+
+const void* dfttest_vol_list_function(int B, int max_depth) {
+  const bool any = max_depth > 7;
+  switch (B) {
+    case 8: return any ? (const void*)dfttest_vol_list_kernel<8, true> : (const void*)dfttest_vol_list_kernel<8, false>;
+    case 16: return any ? (const void*)dfttest_vol_list_kernel<16, true> : (const void*)dfttest_vol_list_kernel<16, false>;
+    case 32: return any ? (const void*)dfttest_vol_list_kernel<32, true> : (const void*)dfttest_vol_list_kernel<32, false>;
+    default: return nullptr;
+  }
+}
+
 hipError_t dfttest_vol_set_lds_limit() {
   const void* fns[] = {(const void*)dfttest_vol_kernel<8, false>,  (const void*)dfttest_vol_kernel<8, true>,
                        (const void*)dfttest_vol_kernel<16, false>, (const void*)dfttest_vol_kernel<16, true>,
@@ -420,6 +451,12 @@ hipError_t dfttest_vol_set_lds_limit() {
     const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;
   }
+  for (int B : {8, 16, 32})
+    for (int d : {3, 8}) {
+      const hipError_t e =
+          hipFuncSetAttribute(dfttest_vol_list_function(B, d), hipFuncAttributeMaxDynamicSharedMemorySize, kListLdsLimit);
+      if (e != hipSuccess) return e;
+    }
   return hipSuccess;
 }
 

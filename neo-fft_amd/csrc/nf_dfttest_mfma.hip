@@ -30,6 +30,7 @@
 #include "nf_fft.hpp"
 #include "nf_gain.hpp"
 #include "nf_launch.hpp"
+#include "nf_worklist.hpp"
 
 namespace nf {
 
@@ -86,18 +87,14 @@ __device__ __forceinline__ Hl split_scaled(const float* v, float scale) {
   return split(t);
 }
 
-}  // namespace
-
-// tw: cos/sin of 2*pi*k/16, k = 0..15, as (cos, sin) pairs (binary64 -> float).
+// One output tile of job J with temporal inputs Q. tw: cos/sin of 2*pi*k/16,
+// k = 0..15, as (cos, sin) pairs (binary64 -> float). ORs NF_FLAG_NONFINITE
+// into *flags on a non-finite sample or intermediate.
 template <int TT>
-__global__ __launch_bounds__(256) void dfttest_mfma16_kernel(const DftVolLaunch L, const float* tw) {
+__device__ __forceinline__ void mfma16_tile(const DftJob& J, const DftTemporal& Q, int tile, const float* tw,
+                                            unsigned* flags) {
   constexpr int B = 16, K = B / 2 + 1, OS = B + 1, BINS2 = B * K;
   extern __shared__ float smem[];
-  int jid = 0;
-  while (jid + 1 < L.njobs && (int)blockIdx.x >= L.tile_start[jid + 1]) ++jid;
-  const DftJob& J = L.jobs[jid];
-  const DftTemporal& Q = L.tq[jid];
-  const int tile = (int)blockIdx.x - L.tile_start[jid];
   const int tx = tile % J.tiles_x, ty = tile / J.tiles_x;
   const int S = J.S;
   const int tid = threadIdx.x, nthreads = blockDim.x;
@@ -348,7 +345,28 @@ __global__ __launch_bounds__(256) void dfttest_mfma16_kernel(const DftVolLaunch 
       }
     }
   }
-  if (bad) atomicOr(L.flags, NF_FLAG_NONFINITE);
+  if (bad) atomicOr(flags, NF_FLAG_NONFINITE);
+}
+
+}  // namespace
+
+// Register budget: the compiler's default allocation for depths 3 and 5
+// exceeds 256 VGPRs (one wave per SIMD); capped at two waves per SIMD (four
+// for depth 1) it fits without spills.
+template <int TT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TT == 1 ? 4 : 2))) void dfttest_mfma16_kernel(
+    const DftVolLaunch L, const float* tw) {
+  int jid = 0;
+  while (jid + 1 < L.njobs && (int)blockIdx.x >= L.tile_start[jid + 1]) ++jid;
+  mfma16_tile<TT>(L.jobs[jid], L.tq[jid], (int)blockIdx.x - L.tile_start[jid], tw, L.flags);
+}
+
+template <int TT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TT == 1 ? 4 : 2))) void dfttest_mfma16_list_kernel(
+    const DftWorkList W, const float* tw) {
+  for_each_work_item(W, [&](int j, int tile) {
+    mfma16_tile<TT>(list_job(W, j), list_temporal(W, j), tile, tw, W.flags);
+  });
 }
 
 size_t dfttest_mfma_lds_bytes(int B, int T, int nbx_max, int ww, int tile_w) {
@@ -379,11 +397,35 @@ hipError_t launch_dfttest_mfma(int B, int T, const DftVolLaunch& L, int total_ti
   return hipGetLastError();
 }
 
+const void* dfttest_mfma_list_function(int B, int T) {
+  if (B != 16) return nullptr;
+  switch (T) {
+    case 1: return (const void*)dfttest_mfma16_list_kernel<1>;
+    case 3: return (const void*)dfttest_mfma16_list_kernel<3>;
+    case 5: return (const void*)dfttest_mfma16_list_kernel<5>;
+    default: return nullptr;
+  }
+}
+
+hipError_t launch_dfttest_mfma_list(int B, int T, const DftWorkList& W, int workgroups, size_t lds, const float* tw,
+                                    hipStream_t stream) {
+  const void* f = dfttest_mfma_list_function(B, T);
+  if (!f) return hipErrorInvalidValue;
+  dim3 grid(workgroups), block(256);
+  void* args[] = {const_cast<DftWorkList*>(&W), const_cast<const float**>(&tw)};
+  return hipLaunchKernel(f, grid, block, args, lds, stream);
+}
+
 hipError_t dfttest_mfma_set_lds_limit() {
   const void* fns[] = {(const void*)dfttest_mfma16_kernel<1>, (const void*)dfttest_mfma16_kernel<3>,
                        (const void*)dfttest_mfma16_kernel<5>};
   for (const void* f : fns) {
     const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+  }
+  for (int T : {1, 3, 5}) {
+    const hipError_t e =
+        hipFuncSetAttribute(dfttest_mfma_list_function(16, T), hipFuncAttributeMaxDynamicSharedMemorySize, kListLdsLimit);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;

@@ -6,10 +6,16 @@ namespace nf {
 size_t dfttest_t1_lds_bytes(int B, const DftJob& j);
 hipError_t launch_dfttest_t1(int B, const DftLaunch& L, int total_tiles, size_t lds, hipStream_t stream);
 hipError_t dfttest_t1_set_lds_limit();
+// persistent work-list forms (grid = `workgroups`, items taken from W.ticket)
+hipError_t launch_dfttest_t1_list(int B, const DftWorkList& W, int workgroups, size_t lds, hipStream_t stream);
+const void* dfttest_t1_list_function(int B);
 
 size_t dfttest_vol_lds_bytes(int B, int T, int chunk, int S, int nout, int acc_rows, int tile_w);
 hipError_t launch_dfttest_vol(int B, int max_depth, const DftVolLaunch& L, int total_tiles, size_t lds, hipStream_t stream);
 hipError_t dfttest_vol_set_lds_limit();
+hipError_t launch_dfttest_vol_list(int B, int max_depth, const DftWorkList& W, int workgroups, size_t lds,
+                                   hipStream_t stream);
+const void* dfttest_vol_list_function(int B, int max_depth);
 
 hipError_t launch_dither(const float* acc, long long acc_stride, int w, int h, uint8_t* dst, long long dst_stride,
                          int mode, int seed, int frame, int plane, hipStream_t stream);
@@ -31,6 +37,9 @@ bool dfttest_mfma_supported(int B, int T);
 hipError_t launch_dfttest_mfma(int B, int T, const DftVolLaunch& L, int total_tiles, size_t lds, const float* tw,
                                hipStream_t stream);
 hipError_t dfttest_mfma_set_lds_limit();
+hipError_t launch_dfttest_mfma_list(int B, int T, const DftWorkList& W, int workgroups, size_t lds, const float* tw,
+                                    hipStream_t stream);
+const void* dfttest_mfma_list_function(int B, int T);
 
 size_t dft_noise_lds_bytes(int B, int T);
 hipError_t launch_dft_noise(int B, const DftNoiseArgs& A, int nloc, float calibration, float* out,

@@ -1,0 +1,47 @@
+// nf_worklist.hpp -- persistent work-list loop of the DFTTest kernels.
+#pragma once
+#include "nf_device.hpp"
+
+namespace nf {
+
+// Job descriptors of a work list, read through the constant address space:
+// the tables never change during a launch, so the compiler may keep fields
+// in scalar registers or reload them instead of holding them live.
+__device__ __forceinline__ const DftJob& list_job(const DftWorkList& W, int j) {
+  typedef const __attribute__((address_space(4))) DftJob CJob;
+  return (const DftJob&)((CJob*)W.jobs)[j];
+}
+__device__ __forceinline__ const DftTemporal& list_temporal(const DftWorkList& W, int j) {
+  typedef const __attribute__((address_space(4))) DftTemporal CTq;
+  return (const DftTemporal&)((CTq*)W.tq)[j];
+}
+
+// Runs body(job, tile) for work items taken from W.ticket[0] until the list is
+// exhausted. Every workgroup leaves the loop after one ticket past the end, so
+// the loop ends for every wave; the workgroup that finishes last resets both
+// counters. The barrier at the top of each turn also separates one item's LDS
+// use from the next one's. Job and tile are read into scalar registers so the
+// descriptor fields are uniform loads.
+template <class Body>
+__device__ __forceinline__ void for_each_work_item(const DftWorkList& W, Body&& body) {
+  __shared__ int s_item;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(&W.ticket[0], 1u);
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(s_item);
+    if (t >= W.items) break;
+    const int2 o = W.order[t];
+    body(__builtin_amdgcn_readfirstlane(o.x), __builtin_amdgcn_readfirstlane(o.y));
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&W.ticket[1], 1u) == gridDim.x - 1) {
+      __threadfence();
+      atomicExch(&W.ticket[0], 0u);
+      atomicExch(&W.ticket[1], 0u);
+    }
+  }
+}
+
+}  // namespace nf

@@ -109,6 +109,16 @@ _SIGNATURES = {
     "nf_process_batch": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(Plane), c_int_p,
                                         c_int_p, ctypes.POINTER(MutPlane), c_int_p, c_int_p, ctypes.c_void_p,
                                         ctypes.c_void_p]),
+    "nf_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(Plane), c_int_p,
+                                       c_int_p, ctypes.POINTER(MutPlane), ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.POINTER(ctypes.c_void_p)]),
+    "nf_batch_enqueue": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "nf_batch_schedule": (ctypes.c_int, [ctypes.c_void_p, c_int_p, c_float_p]),
+    "nf_batch_destroy": (None, [ctypes.c_void_p]),
+    "nf_graph_capture": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_void_p,
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    "nf_graph_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "nf_graph_destroy": (None, [ctypes.c_void_p]),
     "nf_dfttest_enqueue_volume": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Plane), ctypes.c_int,
                                                  ctypes.POINTER(MutPlane), ctypes.c_int, ctypes.c_int,
                                                  ctypes.c_void_p, ctypes.c_void_p]),
@@ -417,6 +427,74 @@ def process_batch(jobs, flags, stream=None, frames=None):
     fr = (ctypes.c_int * n)(*([-1] * n if frames is None else frames))
     _check(lib().nf_process_batch(plans, n, src_arr, (ctypes.c_int * n)(*first), (ctypes.c_int * n)(*counts),
                                   dst_arr, fr, zeros, ctypes.c_void_p(flags.data_ptr()), _stream_handle(stream)))
+
+
+def _job_arrays(jobs):
+    n = len(jobs)
+    plans = (ctypes.c_void_p * n)(*[j[0].handle for j in jobs])
+    all_src, first, counts = [], [], []
+    for _, srcs, _ in jobs:
+        first.append(len(all_src))
+        counts.append(len(srcs))
+        all_src.extend(srcs)
+    src_arr = (Plane * len(all_src))(*[_plane(s) for s in all_src])
+    dst_arr = (MutPlane * n)(*[_plane(j[2], MutPlane) for j in jobs])
+    return plans, src_arr, (ctypes.c_int * n)(*first), (ctypes.c_int * n)(*counts), dst_arr
+
+
+class Batch:
+    """Persistent work-list launch over DFTTest jobs [(plan, [sources], dst)]
+    of any plane sizes (one sbsize/tbsize/transform form, tmode=0).
+    schedule=None tunes the launch shape on `stream` (runs the batch, writing
+    the dsts); an index reuses a schedule reported by schedule()["choice"] of
+    a batch of the same plans. Tensors and plans must stay alive while the
+    batch is used."""
+
+    def __init__(self, jobs, stream=None, schedule=None):
+        plans, src_arr, first, counts, dst_arr = _job_arrays(jobs)
+        h = ctypes.c_void_p()
+        _check(lib().nf_batch_create(plans, len(jobs), src_arr, first, counts, dst_arr,
+                                     -1 if schedule is None else int(schedule), _stream_handle(stream),
+                                     ctypes.byref(h)))
+        self.handle = h.value
+        self._keep = [j[0] for j in jobs], [t for j in jobs for t in j[1]], [j[2] for j in jobs]
+
+    def enqueue(self, flags, stream=None):
+        _check(lib().nf_batch_enqueue(self.handle, ctypes.c_void_p(flags.data_ptr()), _stream_handle(stream)))
+
+    def schedule(self):
+        """{items, workgroups, tile_w, tile_h, chunk, lds, candidates, choice, tuned_ms}"""
+        out = (ctypes.c_int * 8)()
+        ms = ctypes.c_float()
+        _check(lib().nf_batch_schedule(self.handle, out, ctypes.byref(ms)))
+        keys = ("items", "workgroups", "tile_w", "tile_h", "chunk", "lds", "candidates", "choice")
+        d = dict(zip(keys, list(out)))
+        d["tuned_ms"] = ms.value
+        return d
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.nf_batch_destroy(self.handle)
+            self.handle = None
+
+
+class Graph:
+    """hipGraph replaying the launches of `batches` (a frame group)."""
+
+    def __init__(self, batches, flags):
+        arr = (ctypes.c_void_p * len(batches))(*[b.handle for b in batches])
+        h = ctypes.c_void_p()
+        _check(lib().nf_graph_capture(arr, len(batches), ctypes.c_void_p(flags.data_ptr()), ctypes.byref(h)))
+        self.handle = h.value
+        self._keep = list(batches), flags
+
+    def launch(self, stream=None):
+        _check(lib().nf_graph_launch(self.handle, _stream_handle(stream)))
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.nf_graph_destroy(self.handle)
+            self.handle = None
 
 
 class KalmanState:

@@ -1,0 +1,191 @@
+"""Persistent work-list batches (nf_batch_*) and frame-group graphs
+(nf_graph_*): all planes of several clips of uneven size in one launch must give
+the same bits as the oracle (radix-2 form) and as per-plane processing (both
+forms); graph replays must match the eager launches on every replay."""
+import numpy as np
+import pytest
+
+from gpu_util import empty_like_np, to_device, to_host
+from synthetic import yuv420_sinusoids
+
+pytestmark = pytest.mark.gpu
+
+# (width, height) of the clips: uneven sizes whose planes end in clipped edge
+# tiles (the smallest chroma planes, 59x41, still admit sbsize 32)
+CLIPS = [(160, 96), (142, 82), (200, 120), (118, 86)]
+
+
+def _clips(bits, frames, seed):
+    return [yuv420_sinusoids(w, h, frames=frames, bits=bits, seed=seed + i) for i, (w, h) in enumerate(CLIPS)]
+
+
+def _slots(n, T, nframes):
+    return [min(max(n - T // 2 + z, 0), nframes - 1) for z in range(T)]
+
+
+def _jobs(nf, clips, n, T, B, O, plans_cache, transform=None, bits=16):
+    """One job per (clip, plane) for output frame n; returns jobs and host slot lists."""
+    jobs, slots_host = [], []
+    for ci, clip in enumerate(clips):
+        nframes = len(clip)
+        for p in range(3):
+            pl = clip[0][p]
+            h, w = pl.shape
+            key = (ci, p)
+            if key not in plans_cache:
+                plan = nf.dfttest_plan(w, h, bits=bits, chroma=p > 0, tbsize=T, sbsize=B, sosize=O, sigma=8.0)
+                if transform is not None:
+                    nf.set_transform(plan, transform)
+                plans_cache[key] = (plan, [to_device(clip[f][p]) for f in range(nframes)])
+            plan, dev = plans_cache[key]
+            idx = _slots(n, T, nframes)
+            jobs.append((plan, [dev[f] for f in idx], empty_like_np(pl)))
+            slots_host.append([clip[f][p] for f in idx])
+    return jobs, slots_host
+
+
+@pytest.mark.parametrize("T", [1, 3])
+@pytest.mark.parametrize("B,O", [(8, 6), (16, 12), (32, 24)])
+def test_batch_matches_oracle(nf, oracle, B, O, T):
+    import torch
+    clips = _clips(16, 3, seed=70 + B + T)
+    cache = {}
+    jobs, slots = _jobs(nf, clips, 1, T, B, O, cache)
+    print(f"batch B={B} T={T} jobs={len(jobs)}", flush=True)
+    batch = nf.Batch(jobs)
+    sched = batch.schedule()
+    assert sched["items"] > 0 and 0 < sched["workgroups"] <= sched["items"]
+    assert 0 <= sched["choice"] < sched["candidates"]
+    for _, _, dst in jobs:
+        dst.zero_()
+    flags = torch.zeros(1, dtype=torch.int32, device="cuda")
+    batch.enqueue(flags)
+    torch.cuda.synchronize()
+    assert int(flags.item()) == 0
+    for (plan, _, dst), sl in zip(jobs, slots):
+        h, w = sl[0].shape
+        ref = oracle.dfttest_plane(sl, bits=16, tbsize=T, sbsize=B, sosize=O, sigma=8.0)
+        assert np.array_equal(to_host(dst, np.uint16), ref), f"B={B} T={T} plane {w}x{h}"
+
+
+@pytest.mark.parametrize("T", [1, 3])
+def test_batch_matrix_core_matches_single(nf, T):
+    import torch
+    clips = _clips(16, 3, seed=90 + T)
+    cache = {}
+    jobs, _ = _jobs(nf, clips, 1, T, 16, 12, cache, transform=nf.TRANSFORM_MFMA_F16X3)
+    print(f"matrix-core batch T={T} jobs={len(jobs)}", flush=True)
+    batch = nf.Batch(jobs)
+    for _, _, dst in jobs:
+        dst.zero_()
+    flags = torch.zeros(1, dtype=torch.int32, device="cuda")
+    batch.enqueue(flags)
+    torch.cuda.synchronize()
+    assert int(flags.item()) == 0
+    for plan, srcs, dst in jobs:
+        ref = torch.empty_like(dst)
+        plan.process(srcs, ref)
+        assert torch.equal(dst, ref)
+
+
+def test_batch_tuned_and_untuned_agree(nf, monkeypatch):
+    import torch
+    clips = _clips(16, 3, seed=11)
+    cache = {}
+    jobs, _ = _jobs(nf, clips, 2, 3, 16, 12, cache)
+    flags = torch.zeros(1, dtype=torch.int32, device="cuda")
+    tuned = nf.Batch(jobs)
+    assert tuned.schedule()["candidates"] > 1
+    tuned.enqueue(flags)
+    torch.cuda.synchronize()
+    first = [dst.clone() for _, _, dst in jobs]
+    monkeypatch.setenv("NF_AUTOTUNE", "0")
+    plain = nf.Batch(jobs)
+    assert plain.schedule()["choice"] == 0 and plain.schedule()["tuned_ms"] == 0.0
+    for _, _, dst in jobs:
+        dst.zero_()
+    plain.enqueue(flags)
+    torch.cuda.synchronize()
+    for (_, _, dst), ref in zip(jobs, first):
+        assert torch.equal(dst, ref)
+    # every explicit schedule gives the same bits; out of range is rejected
+    n = tuned.schedule()["candidates"]
+    for c in range(n):
+        other = nf.Batch(jobs, schedule=c)
+        assert other.schedule()["choice"] == c
+        for _, _, dst in jobs:
+            dst.zero_()
+        other.enqueue(flags)
+        torch.cuda.synchronize()
+        for (_, _, dst), ref in zip(jobs, first):
+            assert torch.equal(dst, ref), f"schedule {c}"
+    with pytest.raises(nf.InvalidArgument):
+        nf.Batch(jobs, schedule=n)
+    assert int(flags.item()) == 0
+
+
+def test_graph_replays_match_eager(nf):
+    """A frame group of three batches captured once; every replay, with eager
+    work and synchronisations in between, reproduces the eager outputs, and
+    relaunching the same batches again (ticket reset) does too."""
+    import torch
+    clips = _clips(16, 3, seed=31)
+    cache = {}
+    groups = [_jobs(nf, clips, n, 3, 16, 12, cache)[0] for n in range(3)]
+    batches = [nf.Batch(jobs) for jobs in groups]
+    flags = torch.zeros(1, dtype=torch.int32, device="cuda")
+    for b in batches:
+        b.enqueue(flags)
+    torch.cuda.synchronize()
+    eager = [[dst.clone() for _, _, dst in jobs] for jobs in groups]
+    graph = nf.Graph(batches, flags)
+    stream = torch.cuda.Stream()
+    for replay in range(4):
+        for jobs in groups:
+            for _, _, dst in jobs:
+                dst.fill_(-1)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(stream):
+            graph.launch(stream)
+        stream.synchronize()
+        print(f"replay {replay}", flush=True)
+        for jobs, ref in zip(groups, eager):
+            for (_, _, dst), r in zip(jobs, ref):
+                assert torch.equal(dst, r), f"replay {replay}"
+        # eager work between replays: the same batches outside the graph
+        batches[replay % 3].enqueue(flags)
+        torch.cuda.synchronize()
+    assert int(flags.item()) == 0
+
+
+def test_batch_nonfinite_flag(nf):
+    import torch
+    rng = np.random.default_rng(3)
+    planes = [rng.random((40, 56), dtype=np.float32), rng.random((33, 70), dtype=np.float32)]
+    planes[1][5, 9] = np.inf
+    jobs = []
+    for pl in planes:
+        h, w = pl.shape
+        plan = nf.dfttest_plan(w, h, bits=32, floating=True, tbsize=1, sbsize=16, sosize=12)
+        jobs.append((plan, [to_device(pl)], empty_like_np(pl)))
+    batch = nf.Batch(jobs)
+    flags = torch.zeros(1, dtype=torch.int32, device="cuda")
+    batch.enqueue(flags)
+    torch.cuda.synchronize()
+    assert int(flags.item()) & nf.NF_FLAG_NONFINITE
+
+
+def test_batch_rejects_mixed_jobs(nf):
+    rng = np.random.default_rng(5)
+    pl = (rng.random((48, 64)) * 65535).astype(np.uint16)
+    a = nf.dfttest_plan(64, 48, bits=16, tbsize=1, sbsize=16, sosize=12)
+    b = nf.dfttest_plan(64, 48, bits=16, tbsize=1, sbsize=8, sosize=6)
+    src = to_device(pl)
+    with pytest.raises(nf.InvalidArgument):
+        nf.Batch([(a, [src], empty_like_np(pl)), (b, [src], empty_like_np(pl))])
+    t1 = nf.dfttest_plan(64, 48, bits=16, tbsize=3, tmode=1, sbsize=16, sosize=12)
+    with pytest.raises(nf.InvalidArgument):
+        nf.Batch([(t1, [src, src, src], empty_like_np(pl))])
+    f3 = nf.fft3d_plan(64, 48, bits=16, bt=1, bw=16, bh=16)
+    with pytest.raises(nf.InvalidArgument):
+        nf.Batch([(f3, [src], empty_like_np(pl))])
