@@ -19,8 +19,12 @@
  *   plane pipelines ...................... pinned by identity, constant-gain,
  *                                         worked-example and numpy-composition
  *                                         checks
- *   sampled noise models (nlocation,
- *   pfactor, pshow) ...................... not restated here
+ *   sampled noise models (DFTNoise,
+ *   FFT3D pattern search/model, pshow) ... pinned by the reference's
+ *                                         dft_noise / fft3d_models KATs against
+ *                                         a direct binary64 DFT; the bit-level
+ *                                         result of the reference's ordered
+ *                                         float sums is "parity unpinned"
  */
 #include "nf_oracle.h"
 
