@@ -66,6 +66,7 @@ void ensure_lds_limits() {
     if (g_lds_status == hipSuccess) g_lds_status = dfttest_adjoint_set_lds_limit();
     if (g_lds_status == hipSuccess) g_lds_status = fft3d_set_lds_limit();
     if (g_lds_status == hipSuccess) g_lds_status = dfttest_mfma_set_lds_limit();
+    if (g_lds_status == hipSuccess) g_lds_status = dfttest_generic_set_lds_limit();
   });
   hip_check(g_lds_status, "hipFuncSetAttribute");
 }
@@ -151,6 +152,11 @@ struct nf_plan {
   mutable std::atomic<int> mfma_tuned{0};
   mutable std::mutex tune_mu;
   int tmode = 0, tosize = 0;
+  // DFTTest blocks outside the 8/16/32-point codelets or smode=0: direct-DFT
+  // kernels (nf_generic.hip)
+  bool generic = false;
+  int smode = 1;
+  size_t generic_lds = 0;
   const float* d_ttw = nullptr;
   // temporal kernel tiling per mode: 0 per-frame ring, 1 per-frame tile (several
   // covering temporal blocks), 2 all slices of one volume
@@ -162,6 +168,7 @@ struct nf_plan {
   GainParams enh{};
   const float *d_wxa = nullptr, *d_wxs = nullptr, *d_wya = nullptr, *d_wys = nullptr;
   const float *d_ftw = nullptr, *d_itw = nullptr, *d_primary = nullptr;
+  const float *d_twx = nullptr, *d_twy = nullptr;  // direct-DFT tables (generic FFT3D blocks)
   int NB = 4, groups_x = 0;
   // sampled noise models: DFTTest nlocation (DFTNoise) and FFT3D pfactor /
   // pshow; the model is published into `model` by the prepare entry points
@@ -396,12 +403,20 @@ extern "C" nf_status nf_dfttest_plan_create(int width, int height, nf_sample_for
     p->dither_seed = c.dither_seed;
     p->tmode = c.tmode;
     p->tosize = c.tmode ? c.tosize : 0;
-    // Device coverage of this path: smode=1 with 8/16/32-point blocks.
-    if (c.smode != 1) throw Unsupported("DFTTest smode=0 is not available on this device path yet");
-    if (!(c.sbsize == 8 || c.sbsize == 16 || c.sbsize == 32))
-      throw Unsupported("DFTTest sbsize must be 8, 16 or 32 on this device path");
+    // smode=1 with 8/16/32-point blocks runs the codelet-order kernels; other
+    // block sizes and the centre mode run the direct-DFT kernels
     const int T = c.tbsize, B = c.sbsize;
+    p->smode = c.smode;
+    p->generic = c.smode == 0 || !(B == 8 || B == 16 || B == 32);
     p->sampled = c.ftype < 2 && c.n_nlocation > 0;  // plan.cpp:194
+    if (p->generic) {
+      if (p->sampled)
+        throw Unsupported("DFTTest nlocation needs sbsize 8, 16 or 32 with smode=1 on this device path");
+      if (B > 256) throw Unsupported("DFTTest sbsize above 256 is not available on this device path");
+      p->generic_lds = dfttest_generic_lds_bytes(B, T);
+      if (p->generic_lds > 160 * 1024)
+        throw Unsupported("DFTTest block volume does not fit the LDS at this sbsize/tbsize");
+    }
     // plan.cpp:188-226
     const auto win = dft_window_3d(T, B, c.smode == 0 ? 0 : c.sosize, c.smode, c.swin, c.twin, c.sbeta, c.tbeta,
                                    c.tmode, c.tmode ? c.tosize : 0);
@@ -476,7 +491,7 @@ extern "C" nf_status nf_dfttest_plan_create(int width, int height, nf_sample_for
     p->dscale = fmt.floating ? 255.0f : 1.0f / float(1 << (fmt.bits - 8));  // pad.hpp:96-101
     p->oscale = fmt.floating ? 1.0f / 255.0f : float(1 << (fmt.bits - 8));  // plan.cpp:611-639
     p->peak = fmt.floating ? 1.0f : float((1 << fmt.bits) - 1);
-    choose_dft_tiling(*p);
+    if (!p->generic) choose_dft_tiling(*p);
     *out = p.release();
     return NF_OK;
   });
@@ -517,8 +532,14 @@ extern "C" nf_status nf_fft3d_plan_create(int width, int height, nf_sample_forma
     p->bh = c.bh;
     p->bt = c.bt;
     p->temporal = std::max(1, c.bt);
-    if (!(c.bw == c.bh && (c.bw == 8 || c.bw == 16 || c.bw == 32)))
-      throw Unsupported("FFT3D bw=bh in {8,16,32} required on this device path");
+    // square 8/16/32-point blocks run the codelet-order kernel; any other
+    // shape runs the generic block kernel
+    p->generic = !(c.bw == c.bh && (c.bw == 8 || c.bw == 16 || c.bw == 32));
+    if (p->generic) {
+      if (c.bw > 256 || c.bh > 256) throw Unsupported("FFT3D blocks above 256 are not available on this device path");
+      p->generic_lds = fft3d_generic_lds_bytes(c.bw, c.bh, p->temporal);
+      if (p->generic_lds > 160 * 1024) throw Unsupported("FFT3D block volume does not fit the LDS at this bw/bh/bt");
+    }
     const auto wx = fft3d_window(c.bw, p->gx.overlap, c.wintype);
     const auto wy = fft3d_window(c.bh, p->gy.overlap, c.wintype);
     // plan.cpp:101-169
@@ -535,6 +556,8 @@ extern "C" nf_status nf_fft3d_plan_create(int width, int height, nf_sample_forma
     }
     p->varying = sig[1] != sig[0] || sig[2] != sig[0] || sig[3] != sig[0];
     p->preview = c.pshow && (p->sampled || p->varying);  // plan.cpp:113-115
+    if (p->generic && (p->preview || (p->sampled && c.bt != -1)))
+      throw Unsupported("FFT3D pfactor/pshow need bw=bh in {8,16,32} on this device path");
     p->denoise = c.bt != -1 && !p->preview;
     p->kalman = c.bt == 0 && !p->preview;
     p->tabled = p->varying || (p->sampled && p->denoise);
@@ -623,7 +646,10 @@ extern "C" nf_status nf_fft3d_plan_create(int width, int height, nf_sample_forma
     const size_t o_sh = et.sharpen.empty() ? 0 : p->tables.add(et.sharpen);
     const size_t o_ha = et.halo.empty() ? 0 : p->tables.add(et.halo);
     const size_t o_w = weights.empty() ? 0 : p->tables.add(weights);
+    const size_t o_twx = p->tables.add(temporal_twiddles(c.bw)), o_twy = p->tables.add(temporal_twiddles(c.bh));
     p->tables.upload();
+    p->d_twx = p->tables.at<float>(o_twx);
+    p->d_twy = p->tables.at<float>(o_twy);
     if (!weights.empty()) {
       p->d_weights = p->tables.at<float>(o_w);
       p->model.alloc(weights.size());
@@ -662,6 +688,14 @@ extern "C" nf_status nf_fft3d_plan_create(int width, int height, nf_sample_forma
     p->lower = (c.beta - 1.0f) / c.beta;
     p->base = (!fmt.floating && fmt.chroma) ? float(1 << (fmt.bits - 1)) : 0.0f;
     p->peak = fmt.floating ? 1.0f : float((1 << fmt.bits) - 1);
+    if (p->generic) {
+      p->fft3d_nbs.push_back(1);
+      p->NB = 1;
+      p->groups_x = p->gx.count;
+      p->lds = p->generic_lds;
+      *out = p.release();
+      return NF_OK;
+    }
     if (const char* nb = std::getenv("NF_FFT3D_NB")) {
       p->fft3d_nbs.push_back(std::max(1, std::atoi(nb)));
     } else {
@@ -723,7 +757,8 @@ DftJob make_dft_job(const nf_plan& p, const nf_plane& src, const nf_mut_plane& d
   j.mean_scale = p.mean_scale;
   j.g = p.g;
   j.h = p.d_h;
-  j.hsyn = p.d_hsyn;
+  j.tmode1 = p.tmode == 1 ? 1 : 0;
+  j.hsyn = j.tmode1 ? p.d_h : p.d_hsyn;
   j.grid = p.d_grid;
   j.src = src.data;
   j.src_stride = src.stride_bytes;
@@ -1099,7 +1134,29 @@ void enqueue_dft_vol_jobs(const std::vector<const nf_plan*>& plans, const std::v
   }
 }
 
-void enqueue_fft3d_jobs(const std::vector<const nf_plan*>& plans, std::vector<Fft3dJob> jobs, unsigned* flags,
+// Generic block kernel of one plane, then (unless Kalman advance) its
+// synthesis.
+void enqueue_fft3d_generic(const nf_plan& p, const Fft3dJob& j, unsigned* flags, hipStream_t stream, bool synthesize) {
+  GenericFft3dArgs A{};
+  A.J = j;
+  A.twx = p.d_twx;
+  A.twy = p.d_twy;
+  A.flags = flags;
+  timed(1, stream, [&] { hip_check(launch_fft3d_generic_block(A, p.lds, stream), "fft3d generic block launch"); });
+  if (synthesize && j.mode != 1) {
+    Fft3dLaunch L{};
+    L.jobs[0] = j;
+    L.group_start[0] = 0;
+    L.group_start[1] = p.gx.count * p.gy.count;
+    L.pixel_start[0] = 0;
+    L.pixel_start[1] = (long long)p.width * p.height;
+    L.njobs = 1;
+    L.flags = flags;
+    timed(2, stream, [&] { hip_check(launch_fft3d_synthesis(L, stream), "fft3d synthesis kernel launch"); });
+  }
+}
+
+void enqueue_fft3d_jobs(const std::vector<const nf_plan*>& all_plans, std::vector<Fft3dJob> jobs, unsigned* flags,
                         hipStream_t stream, bool synthesize) {
   ensure_lds_limits();
   // One stream-ordered contribution buffer for every job of the call.
@@ -1107,11 +1164,25 @@ void enqueue_fft3d_jobs(const std::vector<const nf_plan*>& plans, std::vector<Ff
   std::vector<size_t> off(jobs.size());
   for (size_t i = 0; i < jobs.size(); ++i) {
     off[i] = total;
-    if (jobs[i].mode != 1) total += (contrib_floats(*plans[i]) + 63) & ~size_t(63);
+    if (jobs[i].mode != 1) total += (contrib_floats(*all_plans[i]) + 63) & ~size_t(63);
   }
   float* contrib = nullptr;
   if (total) hip_check(hipMallocAsync((void**)&contrib, total * sizeof(float), stream), "hipMallocAsync(contrib)");
   for (size_t i = 0; i < jobs.size(); ++i) jobs[i].contrib = contrib ? contrib + off[i] : nullptr;
+  // blocks outside the codelet sizes run one plane at a time
+  std::vector<const nf_plan*> plans;
+  {
+    std::vector<Fft3dJob> rest;
+    for (size_t i = 0; i < jobs.size(); ++i) {
+      if (all_plans[i]->generic) {
+        enqueue_fft3d_generic(*all_plans[i], jobs[i], flags, stream, synthesize);
+      } else {
+        plans.push_back(all_plans[i]);
+        rest.push_back(jobs[i]);
+      }
+    }
+    jobs.swap(rest);
+  }
   for (size_t first = 0; first < jobs.size(); first += kMaxInlineJobs) {
     // tuned on stateless block passes only (Kalman state updates must run once)
     const size_t n = std::min<size_t>(kMaxInlineJobs, jobs.size() - first);
@@ -1289,6 +1360,29 @@ void enqueue_dft_mfma_jobs(const std::vector<const nf_plan*>& plans, const std::
   }
 }
 
+// DFTTest direct-DFT path: block kernel into a stream-ordered contribution
+// buffer, then the synthesis kernel.
+void enqueue_dft_generic(const nf_plan& p, const DftJob& j, const nf_plane* sources, int n_sources,
+                         const int* targets, int n_targets, unsigned* flags, hipStream_t stream) {
+  ensure_lds_limits();
+  GenericDftArgs A{};
+  A.J = j;
+  const int zero = 0;
+  if (p.tmode == 1 && p.temporal == 1 && n_targets == 0) {  // the one covering block of tbsize=1
+    targets = &zero;
+    n_targets = 1;
+  }
+  A.Q = make_temporal(p, sources, n_sources, targets, n_targets, 0);
+  A.tw = p.d_tw;
+  A.B = p.B;
+  A.center_only = p.smode == 0;
+  A.flags = flags;
+  const size_t n = dfttest_generic_contrib_floats(A);
+  hip_check(hipMallocAsync((void**)&A.contrib, n * sizeof(float), stream), "hipMallocAsync(contributions)");
+  timed(0, stream, [&] { hip_check(launch_dfttest_generic(A, p.generic_lds, stream), "dfttest generic launch"); });
+  hip_check(hipFreeAsync(A.contrib, stream), "hipFreeAsync(contributions)");
+}
+
 nf_status enqueue_one(const nf_plan* plan, const nf_plane* sources, int n_sources, const int* targets, int n_targets,
                       nf_mut_plane dst, int frame, int plane, unsigned* flags, hipStream_t stream) {
   require(plan && sources && n_sources > 0, "sources must not be empty");
@@ -1312,7 +1406,9 @@ nf_status enqueue_one(const nf_plan* plan, const nf_plane* sources, int n_source
       j.acc_out = acc;
       j.acc_stride = plan->width;
     }
-    if (plan->temporal == 1) {
+    if (plan->generic) {
+      enqueue_dft_generic(*plan, j, sources, n_sources, targets, n_targets, flags, stream);
+    } else if (plan->temporal == 1) {
       require(n_targets == 0 || (plan->tmode == 1 && n_targets == 1 && targets[0] == 0), "invalid temporal block inputs");
       require(n_sources == 1, "DFTTest sources size must match plan temporal_size");
       enqueue_dft_jobs({plan}, {j}, flags, stream);
@@ -1389,6 +1485,14 @@ extern "C" nf_status nf_process_batch(const nf_plan* const* plans, int n_jobs, c
       check_plane(*p, dsts[i].width, dsts[i].height, dsts[i].stride_bytes, dsts[i].data);
       for (int k = 0; k < ns; ++k) check_plane(*p, s[k].width, s[k].height, s[k].stride_bytes, s[k].data);
       pv.push_back(p);
+      if (alg == 0 && p->generic) {  // direct-DFT planes run one by one
+        pv.pop_back();
+        require(p->tmode == 0 || p->temporal == 1, "batched DFTTest jobs take tmode=0 planes (use nf_plan_enqueue)");
+        require(!(p->dither > 0 && p->storage == kU8), "dithered DFTTest planes use nf_plan_enqueue");
+        require(ns == p->temporal, "DFTTest sources size must match plan temporal_size");
+        enqueue_dft_generic(*p, make_dft_job(*p, s[0], dsts[i]), s, ns, nullptr, 0, device_flags, (hipStream_t)stream);
+        continue;
+      }
       if (alg == 0) {
         require(p->tmode == 0 || p->temporal == 1, "batched DFTTest jobs take tmode=0 planes (use nf_plan_enqueue)");
         require(!(p->dither > 0 && p->storage == kU8), "dithered DFTTest planes use nf_plan_enqueue");
@@ -1414,7 +1518,7 @@ extern "C" nf_status nf_process_batch(const nf_plan* const* plans, int n_jobs, c
         fj.push_back(j);
       }
     }
-    if (alg == 0) {
+    if (alg == 0 && !dj.empty()) {
       const bool temporal = pv[0]->temporal > 1;
       const bool mfma = pv[0]->transform == NF_TRANSFORM_MFMA_F16X3;
       for (auto* p : pv) {
@@ -1438,6 +1542,7 @@ extern "C" nf_status nf_dfttest_enqueue_volume(const nf_plan* plan, const nf_pla
     require(plan && plan->algorithm == 0 && sources && dsts && device_flags, "null argument");
     require(plan->tmode == 1 && plan->tosize == 0 && plan->temporal > 1,
             "volume processing needs a tmode=1, tosize=0, tbsize>1 DFTTest plan");
+    if (plan->generic) throw Unsupported("volume processing covers sbsize 8, 16 or 32 with smode=1");
     require(!(plan->dither > 0 && plan->storage == kU8), "dithered DFTTest planes use nf_plan_enqueue");
     (void)first_frame;
     (void)plane;
@@ -1661,6 +1766,7 @@ extern "C" nf_status nf_batch_create(const nf_plan* const* plans, int n_jobs, co
     auto b = std::make_unique<nf_batch>();
     const nf_plan* p0 = plans[0];
     require(p0 && p0->algorithm == 0, "batches take DFTTest plans");
+    if (p0->generic) throw Unsupported("batches cover sbsize 8, 16 or 32 with smode=1");
     b->B = p0->B;
     b->T = p0->temporal;
     b->kind = p0->transform == NF_TRANSFORM_MFMA_F16X3 ? 2 : (p0->temporal > 1 ? 1 : 0);
@@ -1671,6 +1777,7 @@ extern "C" nf_status nf_batch_create(const nf_plan* const* plans, int n_jobs, co
     for (int i = 0; i < n_jobs; ++i) {
       const nf_plan* p = plans[i];
       require(p && p->algorithm == 0, "batches take DFTTest plans");
+      if (p->generic) throw Unsupported("batches cover sbsize 8, 16 or 32 with smode=1");
       require(p->B == b->B, "batched DFTTest jobs must share sbsize");
       require(p->temporal == b->T, "batched DFTTest jobs must share tbsize");
       require((p->transform == NF_TRANSFORM_MFMA_F16X3) == (b->kind == 2),
@@ -1795,9 +1902,10 @@ extern "C" void nf_graph_destroy(nf_graph* graph) { delete graph; }
 namespace {
 void require_wiener(const nf_plan* plan) {
   require(plan && plan->algorithm == 0, "not a DFTTest plan");
-  require(plan->temporal == 1 && plan->g.type == 0 && std::fabs(plan->g.exponent - 1.0f) < 0.00005f,
-          "the differentiable path needs tbsize=1, ftype=0 and f0beta=1");
+  require(plan->temporal == 1 && plan->tmode == 0 && plan->g.type == 0 && std::fabs(plan->g.exponent - 1.0f) < 0.00005f,
+          "the differentiable path needs tbsize=1, tmode=0, ftype=0 and f0beta=1");
   require(plan->storage == kF32, "the differentiable path needs float32 planes");
+  if (plan->generic) throw Unsupported("the differentiable path covers sbsize 8, 16 or 32 with smode=1");
 }
 }  // namespace
 

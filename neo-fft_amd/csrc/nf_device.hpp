@@ -68,6 +68,7 @@ struct DftJob {
   long long acc_stride;
   int tile_x0, tile_y0, tiles_x, tiles_y, tile_w, tile_h;
   int nbx_max, ww_max;
+  int tmode1;  // tmode=1 synthesis (U*V)*h; hsyn then holds h (plan.cpp:594-595)
 };
 
 constexpr int kMaxInlineJobs = 4;
@@ -106,16 +107,28 @@ struct DftVolLaunch {
   unsigned* flags;
 };
 
+// DFTTest blocks outside the 8/16/32-point codelets or in the centre mode
+// (smode=0): direct-DFT block kernel plus a synthesis kernel.
+struct GenericDftArgs {
+  DftJob J;
+  DftTemporal Q;
+  const float* tw;  // (cos, sin) of 2*pi*e/B, e = 0..B-1
+  float* contrib;   // [ntb][count_y][count_x][B*B], or [..][1] in the centre mode
+  int B;
+  int center_only;  // smode=0
+  unsigned* flags;
+};
+
+// dynamic LDS bound of the work-list kernels (their loop holds a few bytes of
+// static LDS)
+constexpr int kListLdsLimit = 160 * 1024 - 256;
+
 // Work list of one persistent DFTTest launch over any number of jobs (planes
 // of clips of any size): job descriptors live in device memory, `order` lists
 // the (job, tile) work items in schedule order, and each workgroup takes the
 // next item from ticket[0] until the list is exhausted. The last workgroup to
 // finish returns ticket[0] and ticket[1] (finished workgroups) to zero, so the
 // list can be launched again, also from a hipGraph, without a reset.
-// dynamic LDS bound of the work-list kernels (their loop holds a few bytes of
-// static LDS)
-constexpr int kListLdsLimit = 160 * 1024 - 256;
-
 struct DftWorkList {
   const DftJob* jobs;
   const DftTemporal* tq;  // tbsize > 1 and matrix-core lists, else nullptr
@@ -175,6 +188,16 @@ struct Fft3dJob {
   // [block][re: bh*(bw/2+1)][im: bh*(bw/2+1)].
   const float* cache_in[5];
   float* cache_out[5];
+};
+
+// FFT3D blocks outside the square 8/16/32-point codelets: one workgroup per
+// block; each axis runs the codelet order when its length is 8, 16 or 32,
+// else a direct DFT with (cos, sin) tables of 2*pi*e/n.
+struct GenericFft3dArgs {
+  Fft3dJob J;
+  const float* twx;  // bw entries
+  const float* twy;  // bh entries
+  unsigned* flags;
 };
 
 struct Fft3dLaunch {

@@ -63,6 +63,7 @@ __device__ __forceinline__ void t1_tile(const DftJob& J, int tile, unsigned* fla
   }
   for (int i = tid; i < B * J.tile_w; i += nthreads) ring[i] = 0.0f;
   const float fct = 1.0f / ((float)B * (float)B);
+  const float volume = 1.0f * (float)B * (float)B;
   __syncthreads();
 
   for (int by = by_lo; by <= by_hi; ++by) {
@@ -167,8 +168,9 @@ __device__ __forceinline__ void t1_tile(const DftJob& J, int tile, unsigned* fla
       float* o = out + (bi * B + y) * OS;
 #pragma unroll
       for (int x = 0; x < B; ++x) {
-        const float v = re[x] * fct;
+        float v = re[x] * fct;
         bad |= !isfinite(v);
+        if (J.tmode1) v = volume * v;
         o[x] = v * hs[y * OS + x];
       }
     }

@@ -288,12 +288,35 @@ NF_HD Cpx dft_term(int n, const Cpx* x, int k, const float* tw, bool fwd) {
   return {sr, si};
 }
 
-// Host-side dispatch by runtime length (plan creation only).
+// Host-side dispatch by runtime length (plan creation only): the radix-2
+// codelet order for 8/16/32 points, else a direct DFT with binary64 twiddles
+// rounded to binary32 and index-order sums (the order of the device path for
+// those lengths).
 inline bool cfft_supported(int n) { return n == 8 || n == 16 || n == 32; }
 inline void cfft_host(int n, float* r, float* i, bool fwd) {
-  if (n == 8) fwd ? fft8<true>(r, i) : fft8<false>(r, i);
-  else if (n == 16) fwd ? fft16<true>(r, i) : fft16<false>(r, i);
-  else fwd ? fft32<true>(r, i) : fft32<false>(r, i);
+  if (n == 8) {
+    fwd ? fft8<true>(r, i) : fft8<false>(r, i);
+  } else if (n == 16) {
+    fwd ? fft16<true>(r, i) : fft16<false>(r, i);
+  } else if (n == 32) {
+    fwd ? fft32<true>(r, i) : fft32<false>(r, i);
+  } else if (n > 1) {
+    const double pi = 3.1415926535897932384626433832795;
+    float tw[2 * 256], xr[256], xi[256];
+    for (int e = 0; e < n; ++e) {
+      tw[2 * e] = (float)std::cos(2.0 * pi * (double)e / (double)n);
+      tw[2 * e + 1] = (float)std::sin(2.0 * pi * (double)e / (double)n);
+      xr[e] = r[e];
+      xi[e] = i[e];
+    }
+    Cpx x[256];
+    for (int e = 0; e < n; ++e) x[e] = Cpx{xr[e], xi[e]};
+    for (int k = 0; k < n; ++k) {
+      const Cpx v = dft_term(n, x, k, tw, fwd);
+      r[k] = v.r;
+      i[k] = v.i;
+    }
+  }
 }
 
 }  // namespace nf

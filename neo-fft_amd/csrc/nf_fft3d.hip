@@ -378,6 +378,324 @@ __global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
   if (bad) atomicOr(L.flags, NF_FLAG_NONFINITE);
 }
 
+// ---------------------------------------------------------------- generic
+// Blocks outside the square 8/16/32-point codelets (any bw x bh up to the LDS
+// budget). One workgroup per block; the per-bin filter is the one above.
+namespace {
+
+__device__ __forceinline__ bool radix_length(int n) { return n == 8 || n == 16 || n == 32; }
+
+// Output k of a direct DFT of n values x[j * st] (real when xi is null):
+// binary64 twiddles rounded to binary32, index-order sums.
+__device__ __forceinline__ void direct_term(int n, const float* xr, const float* xi, int st, int k, const float* tw,
+                                            bool fwd, float& out_r, float& out_i) {
+  float sr = 0.0f, si = 0.0f;
+  for (int j = 0; j < n; ++j) {
+    const int e = (j * k) % n;
+    const float c = tw[2 * e];
+    const float s = fwd ? -tw[2 * e + 1] : tw[2 * e + 1];
+    const float r = xr[j * st], i = xi ? xi[j * st] : 0.0f;
+    const float pr = r * c - i * s;
+    const float pi = r * s + i * c;
+    sr = sr + pr;
+    si = si + pi;
+  }
+  out_r = sr;
+  out_i = si;
+}
+
+template <int N>
+__device__ __forceinline__ void row_forward_radix(const float* x, float* orr, float* oi) {
+  float re[N], im[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    re[j] = x[j];
+    im[j] = 0.0f;
+  }
+  cfft<N, true>(re, im);
+#pragma unroll
+  for (int k = 0; k < N / 2 + 1; ++k) {
+    orr[k] = re[k];
+    oi[k] = (k == N / 2) ? 0.0f : im[k];
+  }
+}
+
+template <int N, bool FWD>
+__device__ __forceinline__ void line_radix(float* lr, float* li, int st) {
+  float re[N], im[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    re[j] = lr[j * st];
+    im[j] = li[j * st];
+  }
+  cfft<N, FWD>(re, im);
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    lr[j * st] = re[j];
+    li[j * st] = im[j];
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void row_inverse_radix(const float* zr, const float* zi, float fct, const float* wxs,
+                                                  float* out, bool& bad) {
+  constexpr int K = N / 2 + 1;
+  float re[N], im[N];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    re[k] = zr[k];
+    im[k] = zi[k];
+  }
+#pragma unroll
+  for (int k = K; k < N; ++k) {
+    re[k] = re[N - k];
+    im[k] = 0.0f - im[N - k];
+  }
+  cfft<N, false>(re, im);
+#pragma unroll
+  for (int x = 0; x < N; ++x) {
+    const float v = re[x] * fct;
+    bad |= !isfinite(v);
+    out[x] = v * wxs[x];
+  }
+}
+
+// In-place transform of every column (n = bh values, stride K) of one block.
+__device__ __forceinline__ void columns(int BH, int K, float* re, float* im, float* tr, float* ti, const float* twy,
+                                        bool fwd, int tid, int nthreads) {
+  if (radix_length(BH)) {
+    for (int k = tid; k < K; k += nthreads) {
+      if (BH == 8) fwd ? line_radix<8, true>(re + k, im + k, K) : line_radix<8, false>(re + k, im + k, K);
+      else if (BH == 16) fwd ? line_radix<16, true>(re + k, im + k, K) : line_radix<16, false>(re + k, im + k, K);
+      else fwd ? line_radix<32, true>(re + k, im + k, K) : line_radix<32, false>(re + k, im + k, K);
+    }
+    __syncthreads();
+  } else {
+    for (int t = tid; t < BH * K; t += nthreads) {
+      const int ky = t / K, k = t - ky * K;
+      direct_term(BH, re + k, im + k, K, ky, twy, fwd, tr[t], ti[t]);
+    }
+    __syncthreads();
+    for (int t = tid; t < BH * K; t += nthreads) {
+      re[t] = tr[t];
+      im[t] = ti[t];
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(128) void fft3d_generic_block_kernel(const GenericFft3dArgs A) {
+  extern __shared__ float smem[];
+  const Fft3dJob& J = A.J;
+  const int BW = J.gx.block, BH = J.gy.block, K = BW / 2 + 1, BINS = BH * K;
+  const long long blk = blockIdx.x;
+  const int by = (int)(blk / J.gx.count), bx = (int)(blk - (long long)by * J.gx.count);
+  const int oy = by * J.gy.step, ox = bx * J.gx.step;
+  const int tid = threadIdx.x, nthreads = blockDim.x;
+  const int nslots = J.mode == 0 ? J.T : 1;
+
+  float* twx = smem;                   // [BW][2]
+  float* twy = twx + 2 * BW;           // [BH][2]
+  float* x = twy + 2 * BH;             // [BH][BW]
+  float* tre = x + BH * BW;            // [BINS] scratch
+  float* tim = tre + BINS;
+  float* sre = tim + BINS;             // [nslots][BINS]
+  float* sim = sre + nslots * BINS;
+  float* dc = sim + nslots * BINS;     // [2]
+  bool bad = false;
+
+  for (int i = tid; i < 2 * BW; i += nthreads) twx[i] = A.twx[i];
+  for (int i = tid; i < 2 * BH; i += nthreads) twy[i] = A.twy[i];
+  __syncthreads();
+  if (J.mode != 2) {
+    for (int j = 0; j < nslots; ++j) {
+      float* jre = sre + j * BINS;
+      float* jim = sim + j * BINS;
+      if (J.mode == 0 && J.cache_in[j]) {
+        const float* g = J.cache_in[j] + blk * 2 * BINS;
+        for (int t = tid; t < BINS; t += nthreads) {
+          jre[t] = g[t];
+          jim[t] = g[BINS + t];
+        }
+        __syncthreads();
+        continue;
+      }
+      // decoded, reflected, windowed samples: ((x - base) * wy) * wx
+      for (int t = tid; t < BH * BW; t += nthreads) {
+        const int y = t / BW, xx = t - y * BW;
+        const int sy = reflect(oy + y, J.gy), sx = reflect(ox + xx, J.gx);
+        const float v = load_sample(J.src[j], J.src_stride[j], J.storage, sy, sx) - J.base;
+        bad |= !isfinite(v);
+        x[t] = (v * J.wya[y]) * J.wxa[xx];
+      }
+      __syncthreads();
+      // rows (real input, bins 0..bw/2; Nyquist imaginary part 0 for even bw)
+      if (radix_length(BW)) {
+        for (int y = tid; y < BH; y += nthreads) {
+          if (BW == 8) row_forward_radix<8>(x + y * BW, jre + y * K, jim + y * K);
+          else if (BW == 16) row_forward_radix<16>(x + y * BW, jre + y * K, jim + y * K);
+          else row_forward_radix<32>(x + y * BW, jre + y * K, jim + y * K);
+        }
+      } else {
+        for (int t = tid; t < BINS; t += nthreads) {
+          const int y = t / K, k = t - y * K;
+          float r, i;
+          direct_term(BW, x + y * BW, nullptr, 1, k, twx, true, r, i);
+          jre[t] = r;
+          jim[t] = (BW % 2 == 0 && k == BW / 2) ? 0.0f : i;
+        }
+      }
+      __syncthreads();
+      columns(BH, K, jre, jim, tre, tim, twy, true, tid, nthreads);
+      if (J.mode == 0 && J.cache_out[j]) {
+        float* g = J.cache_out[j] + blk * 2 * BINS;
+        for (int t = tid; t < BINS; t += nthreads) {
+          g[t] = jre[t];
+          g[BINS + t] = jim[t];
+        }
+      }
+    }
+  }
+  if (J.mode == 1) {
+    // Kalman advance
+    for (int k = tid; k < BINS; k += nthreads) kalman_bin(J, blk * BINS + k, k, sre[k], sim[k], bad);
+    if (bad) atomicOr(A.flags, NF_FLAG_NONFINITE);
+    return;
+  }
+  if (J.mode == 2) {
+    // Kalman render: private copy of L (plan.cpp:471-474)
+    for (int k = tid; k < BINS; k += nthreads) {
+      const float2 l = J.k_last[blk * BINS + k];
+      sre[k] = l.x;
+      sim[k] = l.y;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    dc[0] = sre[0];
+    dc[1] = sim[0];
+  }
+  __syncthreads();
+  if (J.mode == 0 && J.T > 1) {
+    // temporal Wiener per bin into slot 0 (spectral.cpp:162-232)
+    const int c = J.c;
+    for (int k = tid; k < BINS; k += nthreads) {
+      float xr[5], xi[5];
+      for (int j = 0; j < J.T; ++j) {
+        xr[j] = sre[j * BINS + k];
+        xi[j] = sim[j * BINS + k];
+      }
+      const float dcc = sre[c * BINS];
+      const float g_ratio =
+          (J.grid && J.degrid != 0.0f && J.grid[0].x != 0.0f) ? (J.degrid * dcc) / J.grid[0].x : 0.0f;
+      bad |= !isfinite(g_ratio);
+      float orr, oi;
+      switch (J.T) {
+        case 2: temporal_bin<2>(J, k, xr, xi, g_ratio, orr, oi, bad); break;
+        case 3: temporal_bin<3>(J, k, xr, xi, g_ratio, orr, oi, bad); break;
+        case 4: temporal_bin<4>(J, k, xr, xi, g_ratio, orr, oi, bad); break;
+        default: temporal_bin<5>(J, k, xr, xi, g_ratio, orr, oi, bad); break;
+      }
+      tre[k] = orr;
+      tim[k] = oi;
+    }
+    __syncthreads();
+    for (int k = tid; k < BINS; k += nthreads) {
+      sre[k] = tre[k];
+      sim[k] = tim[k];
+    }
+    if (tid == 0) {
+      dc[0] = tre[0];
+      dc[1] = tim[0];
+    }
+    __syncthreads();
+  } else if (J.mode == 0 && J.denoise) {
+    // spatial Wiener (type -1) with fused enhancement when uniform
+    const float scale = J.grid ? (J.degrid * dc[0]) / J.grid[0].x : 0.0f;
+    for (int k = tid; k < BINS; k += nthreads) {
+      float mr = 0.0f, mi = 0.0f;
+      if (J.grid) {
+        mr = scale * J.grid[k].x;
+        mi = scale * J.grid[k].y;
+      }
+      float re = sre[k], im = sim[k];
+      apply_gain(J.g, k, re, im, mr, mi, bad);
+      sre[k] = re;
+      sim[k] = im;
+    }
+    __syncthreads();
+    if (J.enh_separate && tid == 0) {
+      dc[0] = sre[0];
+      dc[1] = sim[0];
+    }
+    __syncthreads();
+  }
+  // enhancement-only pass (plan.cpp:325-329)
+  const bool enhance = J.mode == 2 || J.T > 1 || !J.denoise ? (J.enh.sharpen != 0.0f || J.enh.dehalo != 0.0f)
+                                                            : J.enh_separate != 0;
+  if (enhance) {
+    const float scale = J.grid ? (J.degrid * dc[0]) / J.grid[0].x : 0.0f;
+    bad |= !isfinite(scale);
+    for (int k = tid; k < BINS; k += nthreads) {
+      float mr = 0.0f, mi = 0.0f;
+      if (J.grid) {
+        mr = scale * J.grid[k].x;
+        mi = scale * J.grid[k].y;
+      }
+      float re = sre[k], im = sim[k];
+      apply_gain(J.enh, k, re, im, mr, mi, bad);
+      sre[k] = re;
+      sim[k] = im;
+    }
+    __syncthreads();
+  }
+  columns(BH, K, sre, sim, tre, tim, twy, false, tid, nthreads);
+  // inverse rows with the Hermitian extension, normalisation, horizontal
+  // synthesis window
+  const float fct = 1.0f / ((float)BW * (float)BH);
+  float* out = J.contrib + blk * BH * BW;
+  if (radix_length(BW)) {
+    for (int y = tid; y < BH; y += nthreads) {
+      if (BW == 8) row_inverse_radix<8>(sre + y * K, sim + y * K, fct, J.wxs, out + y * BW, bad);
+      else if (BW == 16) row_inverse_radix<16>(sre + y * K, sim + y * K, fct, J.wxs, out + y * BW, bad);
+      else row_inverse_radix<32>(sre + y * K, sim + y * K, fct, J.wxs, out + y * BW, bad);
+    }
+  } else {
+    for (int t = tid; t < BH * BW; t += nthreads) {
+      const int y = t / BW, xx = t - y * BW;
+      const float* zr = sre + y * K;
+      const float* zi = sim + y * K;
+      float sr = 0.0f;
+      for (int k = 0; k < BW; ++k) {
+        const float br = k < K ? zr[k] : zr[BW - k];
+        const float bi = k < K ? zi[k] : 0.0f - zi[BW - k];
+        const int e = (k * xx) % BW;
+        const float cc = twx[2 * e], ss = twx[2 * e + 1];
+        const float pr = br * cc - bi * ss;
+        sr = sr + pr;
+      }
+      const float v = sr * fct;
+      bad |= !isfinite(v);
+      out[t] = v * J.wxs[xx];
+    }
+  }
+  if (bad) atomicOr(A.flags, NF_FLAG_NONFINITE);
+}
+
+size_t fft3d_generic_lds_bytes(int BW, int BH, int T) {
+  const size_t K = (size_t)BW / 2 + 1, bins = (size_t)BH * K;
+  return sizeof(float) * (2 * (size_t)BW + 2 * (size_t)BH + (size_t)BW * BH + 2 * bins + 2 * (size_t)T * bins + 2);
+}
+
+hipError_t launch_fft3d_generic_block(const GenericFft3dArgs& A, size_t lds, hipStream_t stream) {
+  const long long blocks = (long long)A.J.gx.count * A.J.gy.count;
+  if (blocks <= 0 || blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(fft3d_generic_block_kernel, dim3((unsigned)blocks), dim3(128), lds, stream, A);
+  return hipGetLastError();
+}
+
 // One lane per visible pixel of every job (jobs laid out back to back).
 __global__ __launch_bounds__(256) void fft3d_synthesis_kernel(const Fft3dLaunch L) {
   const long long* pixel_start = L.pixel_start;
@@ -437,7 +755,10 @@ hipError_t launch_fft3d_synthesis(const Fft3dLaunch& L, hipStream_t stream) {
 }
 
 hipError_t fft3d_set_lds_limit() {
-  hipError_t e = hipFuncSetAttribute((const void*)fft3d_block_kernel<8, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  hipError_t e = hipFuncSetAttribute((const void*)fft3d_generic_block_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     160 * 1024);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)fft3d_block_kernel<8, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
   if (e != hipSuccess) return e;
   e = hipFuncSetAttribute((const void*)fft3d_block_kernel<16, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
   if (e != hipSuccess) return e;

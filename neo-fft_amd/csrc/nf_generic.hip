@@ -1,0 +1,270 @@
+// nf_generic.hip -- DFTTest for block sizes outside the 8/16/32-point
+// codelets and for the centre mode (smode=0), on gfx950.
+//
+// The reference transforms such blocks with its generic FFT (PocketFFT,
+// src/spectral/fft_backend.cpp:276-303); this path evaluates each axis as a
+// direct DFT (twiddles cos/sin(2 pi e / n) in binary64 rounded to binary32,
+// products and sums in index order). Stage 1: one workgroup per block of
+// every covering temporal block -- gather with the analysis window
+// (scalar/spatial.cpp:18-28), spatial row and column DFTs, temporal
+// butterflies or direct DFT, gain with the 3D mean (scalar/spectral.cpp:5-53),
+// inverse of the target slice (fft_backend.cpp:526-586) and synthesis window
+// -- writes the block's contribution to a buffer; in the centre mode only the
+// centre sample is inverted and kept (plan.cpp:596-599). Stage 2: one lane per
+// visible pixel adds the contributions in the canonical order (temporal
+// block, block row, block column) and converts (scalar/spatial.cpp:48-88).
+#include "nf_device.hpp"
+#include "nf_fft.hpp"
+#include "nf_gain.hpp"
+#include "nf_launch.hpp"
+
+namespace nf {
+
+namespace {
+
+// Output k of a direct DFT of n values x[j * st] (real when xi is null).
+__device__ __forceinline__ void direct_term(int n, const float* xr, const float* xi, int st, int k, const float* tw,
+                                            bool fwd, float& out_r, float& out_i) {
+  float sr = 0.0f, si = 0.0f;
+  for (int j = 0; j < n; ++j) {
+    const int e = (j * k) % n;
+    const float c = tw[2 * e];
+    const float s = fwd ? -tw[2 * e + 1] : tw[2 * e + 1];
+    const float r = xr[j * st], i = xi ? xi[j * st] : 0.0f;
+    const float pr = r * c - i * s;
+    const float pi = r * s + i * c;
+    sr = sr + pr;
+    si = si + pi;
+  }
+  out_r = sr;
+  out_i = si;
+}
+
+__device__ __forceinline__ void store_output(const DftJob& J, int oyy, int oxx, float a) {
+  char* drow = static_cast<char*>(J.dst) + (long long)oyy * J.dst_stride;
+  if (J.storage == kF32) {
+    reinterpret_cast<float*>(drow)[oxx] = a * J.oscale;
+  } else {
+    const float v = (a * J.oscale) + 0.5f;
+    const float cl = v < 0.0f ? 0.0f : (J.peak < v ? J.peak : v);
+    if (J.storage == kU8) reinterpret_cast<uint8_t*>(drow)[oxx] = (uint8_t)cl;
+    else reinterpret_cast<uint16_t*>(drow)[oxx] = (uint16_t)cl;
+  }
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(128) void dfttest_generic_block_kernel(const GenericDftArgs A) {
+  extern __shared__ float smem[];
+  const DftJob& J = A.J;
+  const DftTemporal& Q = A.Q;
+  const int B = A.B, K = B / 2 + 1, T = Q.T;
+  const int BINS2 = B * K;
+  const int cx = J.gx.count, cy = J.gy.count;
+  const long long g = blockIdx.x;
+  const int tb = (int)(g / ((long long)cx * cy));
+  const int rem = (int)(g - (long long)tb * cx * cy);
+  const int by = rem / cx, bx = rem - by * cx;
+  const int oy = by * J.gy.step, ox = bx * J.gx.step;
+  const int tid = threadIdx.x, nthreads = blockDim.x;
+  const int target = Q.tmode1 ? Q.targets[tb] : T / 2;
+  const int cyy = B / 2, cxx = B / 2;
+
+  float* tw = smem;                    // [B][2]
+  float* x = tw + 2 * B;               // [T][B][B]
+  float* rre = x + T * B * B;          // [T][B][K] row transforms
+  float* rim = rre + T * BINS2;
+  float* xre = rim + T * BINS2;        // [T][B][K] spectra
+  float* xim = xre + T * BINS2;
+  float* zre = xim + T * BINS2;        // [B][K] inverse columns
+  float* zim = zre + BINS2;
+  float* dc = zim + BINS2;             // [2]
+  bool bad = false;
+
+  for (int i = tid; i < 2 * B; i += nthreads) tw[i] = A.tw[i];
+  // gather: decoded, reflected samples times the analysis window
+  for (int i = tid; i < T * B * B; i += nthreads) {
+    const int z = i / (B * B), e = i - z * (B * B);
+    const int y = e / B, xx = e - y * B;
+    const int f = Q.slot_frame[tb * T + z];
+    const int sy = reflect(oy + y, J.gy), sx = reflect(ox + xx, J.gx);
+    const float v = load_sample(Q.frames[f], Q.frame_stride[f], J.storage, sy, sx) * J.dscale;
+    bad |= !isfinite(v);
+    x[i] = v * J.h[i];
+  }
+  __syncthreads();
+  // rows: real input, bins 0..B/2 (Nyquist imaginary part 0 for even B)
+  for (int t = tid; t < T * B * K; t += nthreads) {
+    const int zy = t / K, k = t - zy * K;
+    float r, i;
+    direct_term(B, x + zy * B, nullptr, 1, k, tw, true, r, i);
+    rre[zy * K + k] = r;
+    rim[zy * K + k] = (B % 2 == 0 && k == B / 2) ? 0.0f : i;
+  }
+  __syncthreads();
+  // columns
+  for (int t = tid; t < T * B * K; t += nthreads) {
+    const int z = t / BINS2, e = t - z * BINS2;
+    const int ky = e / K, k = e - ky * K;
+    float r, i;
+    direct_term(B, rre + z * BINS2 + k, rim + z * BINS2 + k, K, ky, tw, true, r, i);
+    xre[z * BINS2 + ky * K + k] = r;
+    xim[z * BINS2 + ky * K + k] = i;
+  }
+  __syncthreads();
+  // temporal forward transform per bin (depth 3..5 butterflies, else direct)
+  if (T > 1) {
+    for (int b = tid; b < BINS2; b += nthreads) {
+      Cpx v[15];
+      for (int z = 0; z < T; ++z) v[z] = Cpx{xre[z * BINS2 + b], xim[z * BINS2 + b]};
+      if (T >= 3 && T <= 5) {
+        temporal_butterfly(T, v, true);
+      } else {
+        Cpx w[15];
+        for (int m = 0; m < T; ++m) w[m] = dft_term(T, v, m, Q.ttw, true);
+        for (int m = 0; m < T; ++m) v[m] = w[m];
+      }
+      for (int z = 0; z < T; ++z) {
+        xre[z * BINS2 + b] = v[z].r;
+        xim[z * BINS2 + b] = v[z].i;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    dc[0] = xre[0];
+    dc[1] = xim[0];
+  }
+  __syncthreads();
+  // gain over the (3D) spectrum, then the inverse temporal transform of the
+  // target slice into rre/rim
+  const bool use_grid = J.grid && (T == 1 || (J.mean_scale != 0.0f && J.grid[0].x != 0.0f));
+  const float gs = use_grid ? (J.mean_scale * dc[0]) / J.grid[0].x : 0.0f;
+  for (int b = tid; b < BINS2; b += nthreads) {
+    Cpx v[15];
+    for (int z = 0; z < T; ++z) {
+      const int k3 = z * BINS2 + b;
+      float re = xre[k3], im = xim[k3], mr = 0.0f, mi = 0.0f;
+      if (J.grid) {
+        const float2 gv = J.grid[k3];
+        mr = gs * gv.x;
+        mi = gs * gv.y;
+      }
+      apply_gain(J.g, k3, re, im, mr, mi, bad);
+      v[z] = Cpx{re, im};
+    }
+    Cpx r = v[0];
+    if (T > 1) {
+      if ((T == 3 || T == 5) && target == T / 2) {
+        r = temporal_inverse_center(T, v);
+      } else if (T >= 3 && T <= 5) {
+        temporal_butterfly(T, v, false);
+        r = v[target];
+      } else {
+        r = dft_term(T, v, target, Q.ttw, false);
+      }
+    }
+    rre[b] = r.r;
+    rim[b] = r.i;
+  }
+  __syncthreads();
+  // inverse columns (only the centre row in the centre mode)
+  const int nrows = A.center_only ? 1 : B;
+  for (int t = tid; t < nrows * K; t += nthreads) {
+    const int yi = t / K, k = t - yi * K;
+    const int yy = A.center_only ? cyy : yi;
+    float r, i;
+    direct_term(B, rre + k, rim + k, K, yy, tw, false, r, i);
+    zre[yy * K + k] = r;
+    zim[yy * K + k] = i;
+  }
+  __syncthreads();
+  // inverse rows with the Hermitian extension, normalisation, synthesis
+  const float fct = T > 1 ? 1.0f / ((float)T * (float)B * (float)B) : 1.0f / ((float)B * (float)B);
+  const float volume = (float)T * (float)B * (float)B;
+  const float* syn = (Q.tmode1 ? J.h : J.hsyn) + (size_t)target * B * B;
+  const int nout = A.center_only ? 1 : B * B;
+  float* out = A.contrib + ((long long)(tb * cy + by) * cx + bx) * nout;
+  for (int t = tid; t < nout; t += nthreads) {
+    const int yy = A.center_only ? cyy : t / B, xx = A.center_only ? cxx : t - (t / B) * B;
+    const float* zr = zre + yy * K;
+    const float* zi = zim + yy * K;
+    float sr = 0.0f;
+    for (int k = 0; k < B; ++k) {
+      const float br = k < K ? zr[k] : zr[B - k];
+      const float bi = k < K ? zi[k] : 0.0f - zi[B - k];
+      const int e = (k * xx) % B;
+      const float c = tw[2 * e], s = tw[2 * e + 1];
+      const float pr = br * c - bi * s;
+      sr = sr + pr;
+    }
+    float v = sr * fct;
+    bad |= !isfinite(v);
+    if (Q.tmode1) v = volume * v;
+    out[t] = v * syn[yy * B + xx];
+  }
+  if (bad) atomicOr(A.flags, NF_FLAG_NONFINITE);
+}
+
+__global__ __launch_bounds__(256) void dfttest_generic_synthesis_kernel(const GenericDftArgs A) {
+  const DftJob& J = A.J;
+  const long long w = J.gx.length, h = J.gy.length;
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= w * h) return;
+  const int vy = (int)(p / w), vx = (int)(p - (long long)vy * w);
+  const int B = A.B, S = J.gx.step, Sy = J.gy.step;
+  const int cx = J.gx.count, cy = J.gy.count;
+  const int ntb = A.Q.ntb;
+  float acc;
+  if (A.center_only) {
+    // one block per pixel: block (vy, vx) is centred on it
+    if (!A.Q.tmode1) {
+      acc = A.contrib[(long long)vy * cx + vx];
+    } else {
+      acc = 0.0f;
+      for (int tb = 0; tb < ntb; ++tb) acc = acc + A.contrib[((long long)tb * cy + vy) * cx + vx];
+    }
+  } else {
+    const int y = vy + J.gy.offset, x = vx + J.gx.offset;
+    const int by_first = max(0, ceil_div_int(y - B + 1, Sy)), by_last = min(cy - 1, y / Sy);
+    const int bx_first = max(0, ceil_div_int(x - B + 1, S)), bx_last = min(cx - 1, x / S);
+    acc = 0.0f;
+    for (int tb = 0; tb < ntb; ++tb)
+      for (int by = by_first; by <= by_last; ++by)
+        for (int bx = bx_first; bx <= bx_last; ++bx) {
+          const float* blk = A.contrib + ((long long)(tb * cy + by) * cx + bx) * B * B;
+          acc = acc + blk[(y - by * Sy) * B + (x - bx * S)];
+        }
+  }
+  if (!isfinite(acc)) atomicOr(A.flags, NF_FLAG_NONFINITE);
+  if (J.acc_out) J.acc_out[(long long)vy * J.acc_stride + vx] = acc;
+  else store_output(J, vy, vx, acc);
+}
+
+size_t dfttest_generic_lds_bytes(int B, int T) {
+  const size_t K = (size_t)B / 2 + 1, bins2 = (size_t)B * K;
+  return sizeof(float) * (2 * (size_t)B + (size_t)T * B * B + 4 * (size_t)T * bins2 + 2 * bins2 + 2);
+}
+
+size_t dfttest_generic_contrib_floats(const GenericDftArgs& A) {
+  return (size_t)A.Q.ntb * A.J.gy.count * A.J.gx.count * (A.center_only ? 1 : (size_t)A.B * A.B);
+}
+
+hipError_t launch_dfttest_generic(const GenericDftArgs& A, size_t lds, hipStream_t stream) {
+  const long long blocks = (long long)A.Q.ntb * A.J.gy.count * A.J.gx.count;
+  if (blocks <= 0 || blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(dfttest_generic_block_kernel, dim3((unsigned)blocks), dim3(128), lds, stream, A);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const long long pixels = (long long)A.J.gx.length * A.J.gy.length;
+  hipLaunchKernelGGL(dfttest_generic_synthesis_kernel, dim3((unsigned)((pixels + 255) / 256)), dim3(256), 0, stream,
+                     A);
+  return hipGetLastError();
+}
+
+hipError_t dfttest_generic_set_lds_limit() {
+  return hipFuncSetAttribute((const void*)dfttest_generic_block_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             160 * 1024);
+}
+
+}  // namespace nf

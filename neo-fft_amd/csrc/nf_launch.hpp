@@ -31,6 +31,8 @@ size_t fft3d_block_lds_bytes(int BW, int BH, int T, int NB, int Sx);
 hipError_t launch_fft3d_block(int BW, int BH, const Fft3dLaunch& L, int total_groups, size_t lds, hipStream_t stream);
 hipError_t launch_fft3d_synthesis(const Fft3dLaunch& L, hipStream_t stream);
 hipError_t fft3d_set_lds_limit();
+size_t fft3d_generic_lds_bytes(int BW, int BH, int T);
+hipError_t launch_fft3d_generic_block(const GenericFft3dArgs& A, size_t lds, hipStream_t stream);
 
 size_t dfttest_mfma_lds_bytes(int B, int T, int nbx_max, int ww, int tile_w);
 bool dfttest_mfma_supported(int B, int T);
@@ -40,6 +42,11 @@ hipError_t dfttest_mfma_set_lds_limit();
 hipError_t launch_dfttest_mfma_list(int B, int T, const DftWorkList& W, int workgroups, size_t lds, const float* tw,
                                     hipStream_t stream);
 const void* dfttest_mfma_list_function(int B, int T);
+
+size_t dfttest_generic_lds_bytes(int B, int T);
+size_t dfttest_generic_contrib_floats(const GenericDftArgs& A);
+hipError_t launch_dfttest_generic(const GenericDftArgs& A, size_t lds, hipStream_t stream);
+hipError_t dfttest_generic_set_lds_limit();
 
 size_t dft_noise_lds_bytes(int B, int T);
 hipError_t launch_dft_noise(int B, const DftNoiseArgs& A, int nloc, float calibration, float* out,

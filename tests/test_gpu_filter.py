@@ -154,3 +154,51 @@ def test_kalman_replay_policy(nf, oracle, order):
         for p in range(3):
             assert np.array_equal(to_host(got[p], np.uint16), ref[p]), (order, n, p)
     assert out.checkpoints() == sorted(e[0] for e in policy.entries)
+
+
+def _field_rows(h):
+    """Packed row y <- ROI row y < h/2 ? 2y : 2(h-1-y)+1 (reference roi.hpp:9-11)."""
+    return np.array([2 * y if y < h // 2 else 2 * (h - 1 - y) + 1 for y in range(h)])
+
+
+@pytest.mark.parametrize("bt", [1, 3])
+def test_fft3d_interlaced_roi(nf, oracle, bt):
+    N, w, h, bits = 3, 96, 64, 16
+    frames = _yuv_clip(N, w, h, bits, 12 + bt)
+    params = dict(bt=bt, sigma=3.0, bw=16, bh=16)
+    out = nf.FFT3D(_device_clip(nf, frames, bits), l=8, t=4, r=16, b=12, interlaced=True, **params)
+    got = out.get_frame(1)
+    slots = [1] if bt == 1 else [0, 1, 2]
+    for p in range(3):
+        a = 2 if p else 1
+        L, Tt, R, Bm = 8 // a, 4 // a, 16 // a, 12 // a
+        ph, pw = frames[1][p].shape
+        rows = _field_rows(ph - Tt - Bm)
+        packed = [frames[j][p][Tt:ph - Bm, L:pw - R][rows] for j in slots]
+        res = oracle.fft3d_plane(packed, bits=bits, chroma=p > 0, **params)
+        ref = frames[1][p].copy()
+        roi = ref[Tt:ph - Bm, L:pw - R]
+        roi[rows] = res
+        assert np.array_equal(to_host(got[p], np.uint16), ref), p
+
+
+def test_fft3d_interlaced_kalman(nf, oracle):
+    N, w, h, bits = 6, 64, 48, 10
+    frames = _yuv_clip(N, w, h, bits, 33)
+    params = dict(bt=0, sigma=2.0, bw=16, bh=16, kratio=2.0)
+    out = nf.FFT3D(_device_clip(nf, frames, bits), kalman_warmup=2, interlaced=True, **params)
+    packed = [[pl[_field_rows(pl.shape[0])] for pl in fr] for fr in frames]
+    policy = _KalmanPolicy(oracle, params, [p.shape for p in frames[0]], bits, 2)
+    for n in range(N):
+        got = out.get_frame(n)
+        ref = policy.render(packed, n)
+        for p in range(3):
+            want = np.empty_like(ref[p])
+            want[_field_rows(want.shape[0])] = ref[p]
+            assert np.array_equal(to_host(got[p], np.uint16), want), (n, p)
+
+
+def test_fft3d_interlaced_needs_even_height(nf):
+    frames = _yuv_clip(2, 64, 48, 8, 1)
+    with pytest.raises(nf.InvalidArgument):
+        nf.FFT3D(_device_clip(nf, frames, 8), t=2, b=0, interlaced=True, bt=1, bw=16, bh=16)
