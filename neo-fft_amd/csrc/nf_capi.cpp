@@ -1149,7 +1149,7 @@ void enqueue_fft3d_generic(const nf_plan& p, const Fft3dJob& j, unsigned* flags,
     L.group_start[0] = 0;
     L.group_start[1] = p.gx.count * p.gy.count;
     L.pixel_start[0] = 0;
-    L.pixel_start[1] = (long long)p.width * p.height;
+    L.pixel_start[1] = fft3d_synthesis_units(j);
     L.njobs = 1;
     L.flags = flags;
     timed(2, stream, [&] { hip_check(launch_fft3d_synthesis(L, stream), "fft3d synthesis kernel launch"); });
@@ -1231,7 +1231,7 @@ void enqueue_fft3d_jobs(const std::vector<const nf_plan*>& all_plans, std::vecto
       L.group_start[i] = groups;
       L.pixel_start[i] = pixels;
       groups += L.jobs[i].groups_x * p.gy.count;
-      pixels += (long long)p.width * p.height;
+      pixels += fft3d_synthesis_units(jobs[first + i]);
       int computed = 1;
       if (jobs[first + i].mode == 0) {
         computed = 0;

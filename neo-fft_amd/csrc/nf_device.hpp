@@ -200,10 +200,19 @@ struct GenericFft3dArgs {
   unsigned* flags;
 };
 
+// Synthesis in groups of four cover columns applies when the block step and
+// width are multiples of four; groups per row cover the visible columns.
+__host__ __device__ inline bool synthesis_vector(const Fft3dJob& J) {
+  return J.gx.step % 4 == 0 && J.gx.block % 4 == 0;
+}
+__host__ __device__ inline int synthesis_groups(const Fft3dJob& J) {
+  return (J.gx.offset + J.gx.length - 1) / 4 - J.gx.offset / 4 + 1;
+}
+
 struct Fft3dLaunch {
   Fft3dJob jobs[kMaxInlineJobs];
   int group_start[kMaxInlineJobs + 1];
-  long long pixel_start[kMaxInlineJobs + 1];
+  long long pixel_start[kMaxInlineJobs + 1];  // synthesis units (fft3d_synthesis_units) per job, prefix
   int njobs;
   unsigned* flags;
 };

@@ -145,8 +145,10 @@ __device__ __forceinline__ void kalman_bin(const Fft3dJob& J, long long s, int k
 
 }  // namespace
 
+// One wavefront per workgroup: the barriers between stages are cheap and every
+// resident wave works in the row and column stages.
 template <int BW, int BH>
-__global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
+__global__ __launch_bounds__(64) void fft3d_block_kernel(const Fft3dLaunch L) {
   extern __shared__ float smem[];
   constexpr int K = BW / 2 + 1;
   constexpr int BINS = BH * K;
@@ -310,9 +312,30 @@ __global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
     }
   }
 
-  // enhancement-only pass (plan.cpp:325-329) fused into the inverse columns
+  // enhancement-only pass (plan.cpp:325-329), a separate sweep over the bins
+  // so the inverse column transforms keep a small register footprint
   const bool enhance = J.mode == 2 || J.T > 1 || !J.denoise ? (J.enh.sharpen != 0.0f || J.enh.dehalo != 0.0f)
                                                             : J.enh_separate != 0;
+  if (enhance) {
+    for (int t = tid; t < nb * BINS; t += nthreads) {
+      const int bi = t / BINS, bin = t - bi * BINS;
+      const int y = bin / K, k = bin - y * K;
+      const float scale = J.grid ? (J.degrid * dc[2 * bi]) / J.grid[0].x : 0.0f;
+      bad |= !isfinite(scale);
+      float mr = 0.0f, mi = 0.0f;
+      if (J.grid) {
+        mr = scale * J.grid[bin].x;
+        mi = scale * J.grid[bin].y;
+      }
+      float* cr = sre + bi * BS + y * K + k;
+      float* ci = sim + bi * BS + y * K + k;
+      float re = *cr, im = *ci;
+      apply_gain(J.enh, bin, re, im, mr, mi, bad);
+      *cr = re;
+      *ci = im;
+    }
+    __syncthreads();
+  }
   for (int t = tid; t < nb * K; t += nthreads) {
     const int bi = t / K, k = t - bi * K;
     float re[BH], im[BH];
@@ -322,20 +345,6 @@ __global__ __launch_bounds__(128) void fft3d_block_kernel(const Fft3dLaunch L) {
     for (int y = 0; y < BH; ++y) {
       re[y] = cr[y * K];
       im[y] = ci[y * K];
-    }
-    if (enhance) {
-      const float scale = J.grid ? (J.degrid * dc[2 * bi]) / J.grid[0].x : 0.0f;
-      bad |= !isfinite(scale);
-#pragma unroll
-      for (int y = 0; y < BH; ++y) {
-        const int bin = y * K + k;
-        float mr = 0.0f, mi = 0.0f;
-        if (J.grid) {
-          mr = scale * J.grid[bin].x;
-          mi = scale * J.grid[bin].y;
-        }
-        apply_gain(J.enh, bin, re[y], im[y], mr, mi, bad);
-      }
     }
     cfft<BH, false>(re, im);
 #pragma unroll
@@ -696,18 +705,10 @@ hipError_t launch_fft3d_generic_block(const GenericFft3dArgs& A, size_t lds, hip
   return hipGetLastError();
 }
 
-// One lane per visible pixel of every job (jobs laid out back to back).
-__global__ __launch_bounds__(256) void fft3d_synthesis_kernel(const Fft3dLaunch L) {
-  const long long* pixel_start = L.pixel_start;
-  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  int jid = 0;
-  while (jid + 1 < L.njobs && gid >= pixel_start[jid + 1]) ++jid;
-  if (gid >= pixel_start[L.njobs]) return;
-  const Fft3dJob& J = L.jobs[jid];
-  const long long p = gid - pixel_start[jid];
-  const int w = J.gx.length;
-  const int oy = (int)(p / w), ox = (int)(p - (long long)oy * w);
-  const int y = oy + J.gy.offset, x = ox + J.gx.offset;
+// Vertical synthesis of one cover pixel (x, y): block columns left to right
+// into a row value, times the vertical synthesis weight, block rows top to
+// bottom.
+__device__ __forceinline__ float synthesize_pixel(const Fft3dJob& J, int x, int y) {
   const int BW = J.gx.block, BH = J.gy.block, Sx = J.gx.step, Sy = J.gy.step;
   const int by_first = max(0, ceil_div_int(y - BH + 1, Sy)), by_last = min(J.gy.count - 1, y / Sy);
   const int bx_first = max(0, ceil_div_int(x - BW + 1, Sx)), bx_last = min(J.gx.count - 1, x / Sx);
@@ -721,7 +722,10 @@ __global__ __launch_bounds__(256) void fft3d_synthesis_kernel(const Fft3dLaunch 
     }
     acc = acc + r * J.wys[yy];
   }
-  if (!isfinite(acc)) atomicOr(L.flags, NF_FLAG_NONFINITE);
+  return acc;
+}
+
+__device__ __forceinline__ void store_fft3d_output(const Fft3dJob& J, int oy, int ox, float acc) {
   char* drow = static_cast<char*>(J.dst) + (long long)oy * J.dst_stride;
   if (J.storage == kF32) {
     reinterpret_cast<float*>(drow)[ox] = acc < 0.0f ? 0.0f : (1.0f < acc ? 1.0f : acc);
@@ -733,13 +737,72 @@ __global__ __launch_bounds__(256) void fft3d_synthesis_kernel(const Fft3dLaunch 
   }
 }
 
+// One lane per synthesis unit of every job (jobs laid out back to back): a
+// group of four cover columns of one row when the block step and width are
+// multiples of four (the four columns then share their covering blocks and
+// their contributions are one aligned float4 per block), else one pixel. The
+// per-pixel sums are the same in both forms.
+__global__ __launch_bounds__(256) void fft3d_synthesis_kernel(const Fft3dLaunch L) {
+  const long long* unit_start = L.pixel_start;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int jid = 0;
+  while (jid + 1 < L.njobs && gid >= unit_start[jid + 1]) ++jid;
+  if (gid >= unit_start[L.njobs]) return;
+  const Fft3dJob& J = L.jobs[jid];
+  const long long u = gid - unit_start[jid];
+  const int w = J.gx.length;
+  if (!synthesis_vector(J)) {
+    const int oy = (int)(u / w), ox = (int)(u - (long long)oy * w);
+    const float acc = synthesize_pixel(J, ox + J.gx.offset, oy + J.gy.offset);
+    if (!isfinite(acc)) atomicOr(L.flags, NF_FLAG_NONFINITE);
+    store_fft3d_output(J, oy, ox, acc);
+    return;
+  }
+  const int gpr = synthesis_groups(J);
+  const int oy = (int)(u / gpr), gi = (int)(u - (long long)oy * gpr);
+  const int x0 = (J.gx.offset / 4 + gi) * 4;
+  const int y = oy + J.gy.offset;
+  const int BW = J.gx.block, BH = J.gy.block, Sx = J.gx.step, Sy = J.gy.step;
+  const int by_first = max(0, ceil_div_int(y - BH + 1, Sy)), by_last = min(J.gy.count - 1, y / Sy);
+  const int bx_first = max(0, ceil_div_int(x0 - BW + 1, Sx)), bx_last = min(J.gx.count - 1, x0 / Sx);
+  float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int by = by_first; by <= by_last; ++by) {
+    const int yy = y - by * Sy;
+    float r[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int bx = bx_first; bx <= bx_last; ++bx) {
+      const long long blk = (long long)by * J.gx.count + bx;
+      const float4 c = *reinterpret_cast<const float4*>(J.contrib + (blk * BH + yy) * BW + (x0 - bx * Sx));
+      r[0] = r[0] + c.x;
+      r[1] = r[1] + c.y;
+      r[2] = r[2] + c.z;
+      r[3] = r[3] + c.w;
+    }
+    const float wy = J.wys[yy];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) acc[e] = acc[e] + r[e] * wy;
+  }
+  bool bad = false;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int ox = x0 + e - J.gx.offset;
+    if (ox < 0 || ox >= w) continue;
+    bad |= !isfinite(acc[e]);
+    store_fft3d_output(J, oy, ox, acc[e]);
+  }
+  if (bad) atomicOr(L.flags, NF_FLAG_NONFINITE);
+}
+
+long long fft3d_synthesis_units(const Fft3dJob& J) {
+  return synthesis_vector(J) ? (long long)J.gy.length * synthesis_groups(J) : (long long)J.gy.length * J.gx.length;
+}
+
 size_t fft3d_block_lds_bytes(int BW, int BH, int T, int NB, int Sx) {
   const int ww = odd_stride((NB - 1) * Sx + BW);
   return sizeof(float) * ((size_t)T * NB * spec_block_stride(BW) * 2 + (size_t)BH * ww + 2 * (size_t)NB);
 }
 
 hipError_t launch_fft3d_block(int BW, int BH, const Fft3dLaunch& L, int total_groups, size_t lds, hipStream_t stream) {
-  dim3 grid(total_groups), block(128);
+  dim3 grid(total_groups), block(64);
   if (BW == 8 && BH == 8) hipLaunchKernelGGL((fft3d_block_kernel<8, 8>), grid, block, lds, stream, L);
   else if (BW == 16 && BH == 16) hipLaunchKernelGGL((fft3d_block_kernel<16, 16>), grid, block, lds, stream, L);
   else if (BW == 32 && BH == 32) hipLaunchKernelGGL((fft3d_block_kernel<32, 32>), grid, block, lds, stream, L);
@@ -748,8 +811,8 @@ hipError_t launch_fft3d_block(int BW, int BH, const Fft3dLaunch& L, int total_gr
 }
 
 hipError_t launch_fft3d_synthesis(const Fft3dLaunch& L, hipStream_t stream) {
-  const long long total_pixels = L.pixel_start[L.njobs];
-  const long long blocks = (total_pixels + 255) / 256;
+  const long long total_units = L.pixel_start[L.njobs];
+  const long long blocks = (total_units + 255) / 256;
   hipLaunchKernelGGL(fft3d_synthesis_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, L);
   return hipGetLastError();
 }
