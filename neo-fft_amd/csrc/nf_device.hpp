@@ -146,6 +146,13 @@ struct DftWorkList {
 // per-lane row and column walks of the transforms spread over the LDS banks.
 __host__ __device__ constexpr int spec_block_stride(int B) { return (B + 1) * (B / 2 + 1); }
 __host__ __device__ constexpr int odd_stride(int n) { return n | 1; }
+// Spectrum layout of the DFTTest spatial kernel: row stride and block stride
+// for which one wave's row transforms (lanes = rows of consecutive blocks)
+// and column transforms (lanes = columns of consecutive blocks) both address
+// 32 distinct banks per half-wave; each row slot also holds the block's B
+// output samples (2 * row stride >= B).
+__host__ __device__ constexpr int t1_row_stride(int B) { return B == 8 ? 12 : B == 16 ? 10 : B / 2 + 1; }
+__host__ __device__ constexpr int t1_block_stride(int B) { return B == 8 ? 101 : B == 16 ? 169 : (B + 1) * (B / 2 + 1); }
 
 struct Fft3dJob {
   Axis gx, gy;

@@ -42,14 +42,17 @@ __device__ __forceinline__ void t1_tile(const DftJob& J, int tile, unsigned* fla
   const int TW = X1 - X0;
 
   // LDS carve-up (floats). ww_max is an odd row stride; spectra are real and
-  // imaginary planes with block stride BS; block outputs and windows use a
-  // row stride of B + 1.
-  constexpr int BS = spec_block_stride(B);
+  // imaginary planes with row stride KS and block stride BS chosen so that
+  // the row and the column transforms both touch 32 distinct banks; windows
+  // use a row stride of B + 1. A block's inverse rows overwrite its own
+  // spectrum rows: output x of row y sits at re[y*KS + x] for x < KS, else
+  // at im[y*KS + x - KS].
+  constexpr int KS = t1_row_stride(B);
+  constexpr int BS = t1_block_stride(B);
   constexpr int OS = B + 1;
   float* sre = smem;                                   // [nbx_max][BS]
   float* sim = sre + J.nbx_max * BS;                   // [nbx_max][BS]
-  float* out = sim + J.nbx_max * BS;                   // [nbx_max][B][OS]
-  float* win = out + J.nbx_max * B * OS;               // [B][ww_max]
+  float* win = sim + J.nbx_max * BS;                   // [B][ww_max]
   float* ring = win + B * J.ww_max;                    // [B][tile_w]
   float* dc = ring + B * J.tile_w;                     // [nbx_max][2]
   float* hw = dc + 2 * J.nbx_max;                      // [B][OS]
@@ -88,8 +91,8 @@ __device__ __forceinline__ void t1_tile(const DftJob& J, int tile, unsigned* fla
         im[x] = 0.0f;
       }
       cfft<B, true>(re, im);
-      float* dr = sre + bi * BS + y * K;
-      float* di = sim + bi * BS + y * K;
+      float* dr = sre + bi * BS + y * KS;
+      float* di = sim + bi * BS + y * KS;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         dr[k] = re[k];
@@ -105,14 +108,14 @@ __device__ __forceinline__ void t1_tile(const DftJob& J, int tile, unsigned* fla
       float* ci = sim + bi * BS + k;
 #pragma unroll
       for (int y = 0; y < B; ++y) {
-        re[y] = cr[y * K];
-        im[y] = ci[y * K];
+        re[y] = cr[y * KS];
+        im[y] = ci[y * KS];
       }
       cfft<B, true>(re, im);
 #pragma unroll
       for (int y = 0; y < B; ++y) {
-        cr[y * K] = re[y];
-        ci[y * K] = im[y];
+        cr[y * KS] = re[y];
+        ci[y * KS] = im[y];
       }
       if (k == 0) {
         dc[2 * bi] = re[0];
@@ -129,8 +132,8 @@ __device__ __forceinline__ void t1_tile(const DftJob& J, int tile, unsigned* fla
       float* ci = sim + bi * BS + k;
 #pragma unroll
       for (int y = 0; y < B; ++y) {
-        re[y] = cr[y * K];
-        im[y] = ci[y * K];
+        re[y] = cr[y * KS];
+        im[y] = ci[y * KS];
         const int bin = y * K + k;
         float mr = 0.0f, mi = 0.0f;
         if (J.grid) {
@@ -143,8 +146,8 @@ __device__ __forceinline__ void t1_tile(const DftJob& J, int tile, unsigned* fla
       cfft<B, false>(re, im);
 #pragma unroll
       for (int y = 0; y < B; ++y) {
-        cr[y * K] = re[y];
-        ci[y * K] = im[y];
+        cr[y * KS] = re[y];
+        ci[y * KS] = im[y];
       }
     }
     __syncthreads();
@@ -152,8 +155,8 @@ __device__ __forceinline__ void t1_tile(const DftJob& J, int tile, unsigned* fla
     for (int t = tid; t < nbx * B; t += nthreads) {
       const int bi = t / B, y = t - bi * B;
       float re[B], im[B];
-      const float* rr = sre + bi * BS + y * K;
-      const float* ri = sim + bi * BS + y * K;
+      float* rr = sre + bi * BS + y * KS;
+      float* ri = sim + bi * BS + y * KS;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         re[k] = rr[k];
@@ -165,13 +168,13 @@ __device__ __forceinline__ void t1_tile(const DftJob& J, int tile, unsigned* fla
         im[k] = 0.0f - im[B - k];
       }
       cfft<B, false>(re, im);
-      float* o = out + (bi * B + y) * OS;
 #pragma unroll
       for (int x = 0; x < B; ++x) {
         float v = re[x] * fct;
         bad |= !isfinite(v);
         if (J.tmode1) v = volume * v;
-        o[x] = v * hs[y * OS + x];
+        if (x < KS) rr[x] = v * hs[y * OS + x];
+        else ri[x - KS] = v * hs[y * OS + x];
       }
     }
     __syncthreads();
@@ -189,7 +192,9 @@ __device__ __forceinline__ void t1_tile(const DftJob& J, int tile, unsigned* fla
         const int b_last = min(bx_hi, x / S);
         for (int bx = b_first; bx <= b_last; ++bx) {
           const int bi = bx - bx_lo;
-          sum = sum + out[(bi * B + (y - oy)) * OS + (x - bx * S)];
+          const int xx = x - bx * S;
+          const int row = bi * BS + (y - oy) * KS;
+          sum = sum + (xx < KS ? sre[row + xx] : sim[row + xx - KS]);
         }
         *cell = sum;
       }
@@ -247,11 +252,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(B == 32 ? 2
 }
 
 size_t dfttest_t1_lds_bytes(int B, const DftJob& j) {
-  const int K = B / 2 + 1;
-  (void)K;
-  return sizeof(float) * ((size_t)j.nbx_max * spec_block_stride(B) * 2 + (size_t)j.nbx_max * B * (B + 1) +
-                          (size_t)B * j.ww_max + (size_t)B * j.tile_w + 2 * (size_t)j.nbx_max +
-                          2 * (size_t)B * (B + 1));
+  return sizeof(float) * ((size_t)j.nbx_max * t1_block_stride(B) * 2 + (size_t)B * j.ww_max + (size_t)B * j.tile_w +
+                          2 * (size_t)j.nbx_max + 2 * (size_t)B * (B + 1));
 }
 
 hipError_t launch_dfttest_t1(int B, const DftLaunch& L, int total_tiles, size_t lds, hipStream_t stream) {
