@@ -65,15 +65,17 @@ __device__ __forceinline__ void temporal_stages(const DftJob& J, const DftTempor
                                                 int Tdyn, int target, int tid, int nthreads, bool& bad) {
   constexpr int K = B / 2 + 1;
   constexpr int BINS2 = B * K;
-  constexpr int BS = spec_block_stride(B);
+  constexpr int KS = vol_row_stride(B);
+  constexpr int BS = vol_block_stride(B);
   constexpr int NT = TT > 0 ? TT : 15;
   const int T = TT > 0 ? TT : Tdyn;
   // D: forward temporal transform per bin
   for (int t = tid; t < nb * BINS2; t += nthreads) {
     const int bi = t / BINS2, bin = t - bi * BINS2;
+    const int at = (bin / K) * KS + (bin - (bin / K) * K);
     Cpx v[NT];
-    float* br = sre + (bi * T) * BS + bin;
-    float* bim = sim + (bi * T) * BS + bin;
+    float* br = sre + (bi * T) * BS + at;
+    float* bim = sim + (bi * T) * BS + at;
 #pragma unroll
     for (int z = 0; z < NT; ++z)
       if (z < T) v[z] = Cpx{br[z * BS], bim[z * BS]};
@@ -105,9 +107,10 @@ __device__ __forceinline__ void temporal_stages(const DftJob& J, const DftTempor
   for (int t = tid; t < nb * BINS2; t += nthreads) {
     const int bi = t / BINS2, bin = t - bi * BINS2;
     const float gs = use_grid ? (J.mean_scale * dc[2 * bi]) / J.grid[0].x : 0.0f;
+    const int at = (bin / K) * KS + (bin - (bin / K) * K);
     Cpx v[NT];
-    float* br = sre + (bi * T) * BS + bin;
-    float* bim = sim + (bi * T) * BS + bin;
+    float* br = sre + (bi * T) * BS + at;
+    float* bim = sim + (bi * T) * BS + at;
 #pragma unroll
     for (int z = 0; z < NT; ++z) {
       if (z >= T) continue;
@@ -160,14 +163,18 @@ __device__ __forceinline__ void vol_tile(const DftJob& J, const DftTemporal& Q, 
   const int by_lo = ceil_div_int(Y0 - B + 1, S), by_hi = (Y1 - 1) / S;
   const int TW = X1 - X0, TH = Y1 - Y0;
   const int ww_c = odd_stride((NBc - 1) * S + B);
-  constexpr int BS = spec_block_stride(B);
+  // Spectra: real and imaginary planes, row stride KS, block stride BS (bank
+  // spread of the row and column transforms). A slice's inverse rows
+  // overwrite its own spectrum rows: output x of row y sits at
+  // re[y*KS + x] for x < KS, else at im[y*KS + x - KS].
+  constexpr int KS = vol_row_stride(B);
+  constexpr int BS = vol_block_stride(B);
   constexpr int OS = B + 1;
   const int acc_rows = ring_mode ? B : J.tile_h;
 
   float* sre = smem;                                  // [NBc][T][BS] real parts
   float* sim = sre + NBc * T * BS;                    // [NBc][T][BS] imaginary parts
-  float* out = sim + NBc * T * BS;                    // [NBc][B][OS]
-  float* win = out + NBc * B * OS;                    // [T][B][ww_c]
+  float* win = sim + NBc * T * BS;                    // [T][B][ww_c]
   float* acc = win + T * B * ww_c;                    // [nout][acc_rows][tile_w]
   float* dc = acc + nout * acc_rows * J.tile_w;       // [NBc][2]
   float* hw = dc + 2 * NBc;                           // [T][B][OS]
@@ -220,8 +227,8 @@ __device__ __forceinline__ void vol_tile(const DftJob& J, const DftTemporal& Q, 
             im[x] = 0.0f;
           }
           cfft<B, true>(re, im);
-          float* dr = sre + (bi * T + z) * BS + y * K;
-          float* di = sim + (bi * T + z) * BS + y * K;
+          float* dr = sre + (bi * T + z) * BS + y * KS;
+          float* di = sim + (bi * T + z) * BS + y * KS;
 #pragma unroll
           for (int k = 0; k < K; ++k) {
             dr[k] = re[k];
@@ -239,14 +246,14 @@ __device__ __forceinline__ void vol_tile(const DftJob& J, const DftTemporal& Q, 
           float* ci = sim + (bi * T + z) * BS + k;
 #pragma unroll
           for (int y = 0; y < B; ++y) {
-            re[y] = cr[y * K];
-            im[y] = ci[y * K];
+            re[y] = cr[y * KS];
+            im[y] = ci[y * KS];
           }
           cfft<B, true>(re, im);
 #pragma unroll
           for (int y = 0; y < B; ++y) {
-            cr[y * K] = re[y];
-            ci[y * K] = im[y];
+            cr[y * KS] = re[y];
+            ci[y * KS] = im[y];
           }
         }
         __syncthreads();
@@ -278,14 +285,14 @@ __device__ __forceinline__ void vol_tile(const DftJob& J, const DftTemporal& Q, 
             float* ci = sim + (bi * T + zs) * BS + k;
 #pragma unroll
             for (int y = 0; y < B; ++y) {
-              re[y] = cr[y * K];
-              im[y] = ci[y * K];
+              re[y] = cr[y * KS];
+              im[y] = ci[y * KS];
             }
             cfft<B, false>(re, im);
 #pragma unroll
             for (int y = 0; y < B; ++y) {
-              cr[y * K] = re[y];
-              ci[y * K] = im[y];
+              cr[y * KS] = re[y];
+              ci[y * KS] = im[y];
             }
           }
           __syncthreads();
@@ -293,8 +300,8 @@ __device__ __forceinline__ void vol_tile(const DftJob& J, const DftTemporal& Q, 
           for (int t = tid; t < nb * B; t += nthreads) {
             const int bi = t / B, y = t - bi * B;
             float re[B], im[B];
-            const float* rr = sre + (bi * T + zs) * BS + y * K;
-            const float* ri = sim + (bi * T + zs) * BS + y * K;
+            float* rr = sre + (bi * T + zs) * BS + y * KS;
+            float* ri = sim + (bi * T + zs) * BS + y * KS;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
               re[k] = rr[k];
@@ -306,14 +313,14 @@ __device__ __forceinline__ void vol_tile(const DftJob& J, const DftTemporal& Q, 
               im[k] = 0.0f - im[B - k];
             }
             cfft<B, false>(re, im);
-            float* ob = out + (bi * B + y) * OS;
             const float* hsrow = hs + (o * B + y) * OS;
 #pragma unroll
             for (int x = 0; x < B; ++x) {
               float v = re[x] * fct;
               bad |= !isfinite(v);
               if (Q.tmode1) v = volume * v;
-              ob[x] = v * hsrow[x];
+              if (x < KS) rr[x] = v * hsrow[x];
+              else ri[x - KS] = v * hsrow[x];
             }
           }
           __syncthreads();
@@ -331,8 +338,11 @@ __device__ __forceinline__ void vol_tile(const DftJob& J, const DftTemporal& Q, 
               float sum = *cell;
               const int b_first = max(cb, ceil_div_int(x - B + 1, S));
               const int b_last = min(cb + nb - 1, x / S);
-              for (int bx = b_first; bx <= b_last; ++bx)
-                sum = sum + out[((bx - cb) * B + (y - oy)) * OS + (x - bx * S)];
+              for (int bx = b_first; bx <= b_last; ++bx) {
+                const int xx = x - bx * S;
+                const int row = ((bx - cb) * T + zs) * BS + (y - oy) * KS;
+                sum = sum + (xx < KS ? sre[row + xx] : sim[row + xx - KS]);
+              }
               *cell = sum;
             }
           }
@@ -396,7 +406,7 @@ __global__ __launch_bounds__(256) void dfttest_vol_list_kernel(const DftWorkList
 size_t dfttest_vol_lds_bytes(int B, int T, int chunk, int S, int nout, int acc_rows, int tile_w) {
   const size_t ww_c = (size_t)odd_stride((chunk - 1) * S + B);
   const size_t OS = (size_t)B + 1;
-  return sizeof(float) * ((size_t)chunk * T * spec_block_stride(B) * 2 + (size_t)chunk * B * OS + (size_t)T * B * ww_c +
+  return sizeof(float) * ((size_t)chunk * T * vol_block_stride(B) * 2 + (size_t)T * B * ww_c +
                           (size_t)nout * acc_rows * tile_w + 2 * (size_t)chunk + (size_t)T * B * OS +
                           (size_t)nout * B * OS);
 }
