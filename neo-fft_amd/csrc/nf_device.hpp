@@ -153,6 +153,9 @@ __host__ __device__ constexpr int odd_stride(int n) { return n | 1; }
 // output samples (2 * row stride >= B).
 __host__ __device__ constexpr int t1_row_stride(int B) { return B == 8 ? 12 : B == 16 ? 10 : B / 2 + 1; }
 __host__ __device__ constexpr int t1_block_stride(int B) { return B == 8 ? 101 : B == 16 ? 169 : (B + 1) * (B / 2 + 1); }
+// Source-window row stride of the matrix-core kernel: at least n, a multiple
+// of four with stride/4 odd (float4 loads of eight rows hit distinct banks).
+__host__ __device__ constexpr int mfma_window_stride(int n) { return ((n + 3) / 8) * 8 + 4; }
 // The same for the temporal kernel, whose consecutive blocks are tbsize
 // slices apart and whose temporal stages walk the bins of a block in order
 // (one layout chosen over depths 2..7, weighting every stage's accesses).

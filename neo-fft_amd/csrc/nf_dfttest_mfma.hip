@@ -93,8 +93,11 @@ __device__ __forceinline__ Hl split_scaled(const float* v, float scale) {
 template <int TT>
 __device__ __forceinline__ void mfma16_tile(const DftJob& J, const DftTemporal& Q, int tile, const float* tw,
                                             unsigned* flags) {
-  constexpr int B = 16, K = B / 2 + 1, OS = B + 1, BINS2 = B * K;
-  extern __shared__ float smem[];
+  // OS: row stride of the windows and block outputs, a multiple of four with
+  // OS/4 odd, so the float4 accesses of eight lanes (rows r..r+7) fall on
+  // distinct bank groups; the source window uses the same rule (ww).
+  constexpr int B = 16, K = B / 2 + 1, OS = 20, BINS2 = B * K;
+  extern __shared__ __align__(16) float smem[];
   const int tx = tile % J.tiles_x, ty = tile / J.tiles_x;
   const int S = J.S;
   const int tid = threadIdx.x, nthreads = blockDim.x;
@@ -109,7 +112,8 @@ __device__ __forceinline__ void mfma16_tile(const DftJob& J, const DftTemporal& 
   const int by_lo = ceil_div_int(Y0 - B + 1, S), by_hi = (Y1 - 1) / S;
   const int nbx = bx_hi - bx_lo + 1;
   const int WW = (nbx - 1) * S + B;
-  const int ww = J.ww_max;
+  const int ww = mfma_window_stride(J.ww_max);
+  const bool vec = (S & 3) == 0;  // block origins on float4 boundaries
   const int cx0 = bx_lo * S;
   const int TW = X1 - X0;
   const int target = TT / 2;
@@ -194,10 +198,19 @@ __device__ __forceinline__ void mfma16_tile(const DftJob& J, const DftTemporal& 
         for (int p = 0; p < NP; ++p) {
           const int bi = min(b0 + p, nbx - 1);
           f4 xv;
+          if (vec) {
+            const float4 a = *reinterpret_cast<const float4*>(win + (z * B + r) * ww + bi * S + 4 * q);
+            const float4 h = *reinterpret_cast<const float4*>(hw + (z * B + r) * OS + 4 * q);
+            xv[0] = a.x * h.x;
+            xv[1] = a.y * h.y;
+            xv[2] = a.z * h.z;
+            xv[3] = a.w * h.w;
+          } else {
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int x = 4 * q + e;
-            xv[e] = win[(z * B + r) * ww + bi * S + x] * hw[(z * B + r) * OS + x];
+            for (int e = 0; e < 4; ++e) {
+              const int x = 4 * q + e;
+              xv[e] = win[(z * B + r) * ww + bi * S + x] * hw[(z * B + r) * OS + x];
+            }
           }
           xs[p] = split(xv);
         }
@@ -291,13 +304,11 @@ __device__ __forceinline__ void mfma16_tile(const DftJob& J, const DftTemporal& 
         const int bi = b0 + p;
         if (bi < nbx) {
           // yT: column yy = r, rows xx = 4q+e
-          float* ob = out + (bi * B + r) * OS;
+          const float4 h = *reinterpret_cast<const float4*>(hs + r * OS + 4 * q);
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int x = 4 * q + e;
-            bad |= !isfinite(yT[e]);
-            ob[x] = yT[e] * hs[r * OS + x];
-          }
+          for (int e = 0; e < 4; ++e) bad |= !isfinite(yT[e]);
+          *reinterpret_cast<float4*>(out + (bi * B + r) * OS + 4 * q) =
+              make_float4(yT[0] * h.x, yT[1] * h.y, yT[2] * h.z, yT[3] * h.w);
         }
       }
     }
@@ -370,9 +381,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TT == 1 ? 4
 }
 
 size_t dfttest_mfma_lds_bytes(int B, int T, int nbx_max, int ww, int tile_w) {
-  const size_t OS = (size_t)B + 1;
-  return sizeof(float) * ((size_t)T * B * ww + (size_t)nbx_max * B * OS + (size_t)B * tile_w + (size_t)T * B * OS +
-                          (size_t)B * OS);
+  const size_t OS = 20;
+  return sizeof(float) * ((size_t)T * B * mfma_window_stride(ww) + (size_t)nbx_max * B * OS + (size_t)B * tile_w +
+                          (size_t)T * B * OS + (size_t)B * OS);
 }
 
 bool dfttest_mfma_supported(int B, int T) { return B == 16 && (T == 1 || T == 3 || T == 5); }
