@@ -702,12 +702,12 @@ extern "C" nf_status nf_fft3d_plan_create(int width, int height, nf_sample_forma
       const int first = c.bw == 32 ? 2 : c.bw == 16 ? 4 : 8;
       p->fft3d_nbs.push_back(first);
       for (int nb : {1, 2, 3, 4, 8})
-        if (nb != first && fft3d_block_lds_bytes(c.bw, c.bh, p->temporal, nb, p->gx.step) <= 160 * 1024)
+        if (nb != first && fft3d_block_lds_bytes(c.bw, c.bh, p->temporal, nb, p->gx.step, p->storage) <= 160 * 1024)
           p->fft3d_nbs.push_back(nb);
     }
     p->NB = p->fft3d_nbs[0];
     p->groups_x = (p->gx.count + p->NB - 1) / p->NB;
-    p->lds = fft3d_block_lds_bytes(c.bw, c.bh, p->temporal, p->NB, p->gx.step);
+    p->lds = fft3d_block_lds_bytes(c.bw, c.bh, p->temporal, p->NB, p->gx.step, p->storage);
     require(p->lds <= 160 * 1024, "FFT3D block group does not fit the LDS");
     *out = p.release();
     return NF_OK;
@@ -1208,7 +1208,7 @@ void enqueue_fft3d_jobs(const std::vector<const nf_plan*>& all_plans, std::vecto
           L.group_start[i] = groups;
           groups += L.jobs[i].groups_x * p.gy.count;
           lds = std::max(lds, fft3d_block_lds_bytes(p.bw, p.bh, jobs[first + i].T, L.jobs[i].blocks_per_group,
-                                                    p.gx.step));
+                                                    p.gx.step, p.storage));
         }
         L.group_start[n] = groups;
         L.njobs = (int)n;
@@ -1238,7 +1238,7 @@ void enqueue_fft3d_jobs(const std::vector<const nf_plan*>& all_plans, std::vecto
         for (int s = 0; s < jobs[first + i].T; ++s) computed += jobs[first + i].cache_in[s] ? 0 : 1;
         computed = std::max(computed, 1);
       }
-      lds = std::max(lds, fft3d_block_lds_bytes(bw, bh, computed, L.jobs[i].blocks_per_group, p.gx.step));
+      lds = std::max(lds, fft3d_block_lds_bytes(bw, bh, computed, L.jobs[i].blocks_per_group, p.gx.step, p.storage));
     }
     L.group_start[n] = groups;
     L.pixel_start[n] = pixels;

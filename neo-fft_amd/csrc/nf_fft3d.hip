@@ -145,6 +145,52 @@ __device__ __forceinline__ void kalman_bin(const Fft3dJob& J, long long s, int k
 
 }  // namespace
 
+// Stages the reflected source rows of one temporal slot in LDS and runs the
+// windowed forward row transforms ((x - base) * wy) * wx (pad.hpp:33-38,
+// scalar/spatial.cpp:12-16). Integer samples are staged raw (two or one
+// bytes) and decoded when read, which is exact and keeps the LDS footprint
+// small; float samples are staged decoded.
+template <int BW, int BH, int BS, class Sample>
+__device__ __forceinline__ void forward_rows(const Fft3dJob& J, int j, Sample* win, int ww, int oy, int ox0, int WW,
+                                             int nb, float* jre, float* jim, int tid, int nthreads, bool& bad) {
+  constexpr int K = BW / 2 + 1;
+  const int Sx = J.gx.step;
+  for (int idx = tid; idx < BH * WW; idx += nthreads) {
+    const int r = idx / WW, c = idx - r * WW;
+    const int sy = reflect(oy + r, J.gy), sx = reflect(ox0 + c, J.gx);
+    const char* row = static_cast<const char*>(J.src[j]) + (long long)sy * J.src_stride[j];
+    const Sample raw = reinterpret_cast<const Sample*>(row)[sx];
+    if constexpr (sizeof(Sample) == 4) {
+      const float v = raw - J.base;
+      bad |= !isfinite(v);
+      win[r * ww + c] = v;
+    } else {
+      win[r * ww + c] = raw;
+    }
+  }
+  __syncthreads();
+  for (int t = tid; t < nb * BH; t += nthreads) {
+    const int bi = t / BH, y = t - bi * BH;
+    float re[BW], im[BW];
+    const Sample* src = win + y * ww + bi * Sx;
+    const float wy = J.wya[y];
+#pragma unroll
+    for (int x = 0; x < BW; ++x) {
+      const float v = sizeof(Sample) == 4 ? (float)src[x] : (float)src[x] - J.base;
+      re[x] = (v * wy) * J.wxa[x];
+      im[x] = 0.0f;
+    }
+    cfft<BW, true>(re, im);
+    float* dr = jre + bi * BS + y * K;
+    float* di = jim + bi * BS + y * K;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      dr[k] = re[k];
+      di[k] = (k == BW / 2) ? 0.0f : im[k];
+    }
+  }
+}
+
 // One wavefront per workgroup: the barriers between stages are cheap and every
 // resident wave works in the row and column stages.
 template <int BW, int BH>
@@ -178,43 +224,25 @@ __global__ __launch_bounds__(64) void fft3d_block_kernel(const Fft3dLaunch L) {
 
   float* sre = smem;                     // [Tst][NB][BS] real parts
   float* sim = sre + Tst * NB * BS;      // [Tst][NB][BS] imaginary parts
-  float* win = sim + Tst * NB * BS;      // [BH][ww]
-  float* dc = win + BH * ww;             // [NB][2]
+  float* dc = sim + Tst * NB * BS;       // [NB][2]
+  void* win = dc + 2 * NB;               // [BH][ww] source samples (raw integers, or decoded floats)
   bool bad = false;
 
   if (J.mode != 2) {
     for (int j = 0; j < nslots; ++j) {
       if (lidx[j] < 0) continue;
-      // decoded, reflected source rows: (x - base) * 1 (pad.hpp:33-38)
-      for (int idx = tid; idx < BH * WW; idx += nthreads) {
-        const int r = idx / WW, c = idx - r * WW;
-        const int sy = reflect(oy + r, J.gy), sx = reflect(ox0 + c, J.gx);
-        const float v = load_sample(J.src[j], J.src_stride[j], J.storage, sy, sx) - J.base;
-        bad |= !isfinite(v);
-        win[r * ww + c] = v;
-      }
-      __syncthreads();
       float* jre = sre + lidx[j] * NB * BS;
       float* jim = sim + lidx[j] * NB * BS;
-      for (int t = tid; t < nb * BH; t += nthreads) {
-        const int bi = t / BH, y = t - bi * BH;
-        float re[BW], im[BW];
-        const float* src = win + y * ww + bi * Sx;
-        const float wy = J.wya[y];
-#pragma unroll
-        for (int x = 0; x < BW; ++x) {
-          re[x] = (src[x] * wy) * J.wxa[x];
-          im[x] = 0.0f;
-        }
-        cfft<BW, true>(re, im);
-        float* dr = jre + bi * BS + y * K;
-        float* di = jim + bi * BS + y * K;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          dr[k] = re[k];
-          di[k] = (k == BW / 2) ? 0.0f : im[k];
-        }
-      }
+      // reflected source rows, then windowed rows -> forward row transforms
+      if (J.storage == kU16)
+        forward_rows<BW, BH, BS, uint16_t>(J, j, static_cast<uint16_t*>(win), ww, oy, ox0, WW, nb, jre, jim, tid,
+                                           nthreads, bad);
+      else if (J.storage == kU8)
+        forward_rows<BW, BH, BS, uint8_t>(J, j, static_cast<uint8_t*>(win), ww, oy, ox0, WW, nb, jre, jim, tid,
+                                          nthreads, bad);
+      else
+        forward_rows<BW, BH, BS, float>(J, j, static_cast<float*>(win), ww, oy, ox0, WW, nb, jre, jim, tid, nthreads,
+                                        bad);
       __syncthreads();
       for (int t = tid; t < nb * K; t += nthreads) {
         const int bi = t / K, k = t - bi * K;
@@ -796,9 +824,10 @@ long long fft3d_synthesis_units(const Fft3dJob& J) {
   return synthesis_vector(J) ? (long long)J.gy.length * synthesis_groups(J) : (long long)J.gy.length * J.gx.length;
 }
 
-size_t fft3d_block_lds_bytes(int BW, int BH, int T, int NB, int Sx) {
+size_t fft3d_block_lds_bytes(int BW, int BH, int T, int NB, int Sx, int storage) {
   const int ww = odd_stride((NB - 1) * Sx + BW);
-  return sizeof(float) * ((size_t)T * NB * spec_block_stride(BW) * 2 + (size_t)BH * ww + 2 * (size_t)NB);
+  const size_t sample = storage == kU8 ? 1 : storage == kU16 ? 2 : 4;
+  return sizeof(float) * ((size_t)T * NB * spec_block_stride(BW) * 2 + 2 * (size_t)NB) + sample * BH * ww;
 }
 
 hipError_t launch_fft3d_block(int BW, int BH, const Fft3dLaunch& L, int total_groups, size_t lds, hipStream_t stream) {

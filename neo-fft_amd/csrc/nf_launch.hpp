@@ -27,7 +27,7 @@ hipError_t launch_dfttest_adjoint(int B, const DftJob& J, int tiles, size_t lds,
                                   float* grad_table, unsigned* flags, hipStream_t stream);
 hipError_t dfttest_adjoint_set_lds_limit();
 
-size_t fft3d_block_lds_bytes(int BW, int BH, int T, int NB, int Sx);
+size_t fft3d_block_lds_bytes(int BW, int BH, int T, int NB, int Sx, int storage);
 hipError_t launch_fft3d_block(int BW, int BH, const Fft3dLaunch& L, int total_groups, size_t lds, hipStream_t stream);
 hipError_t launch_fft3d_synthesis(const Fft3dLaunch& L, hipStream_t stream);
 long long fft3d_synthesis_units(const Fft3dJob& J);
