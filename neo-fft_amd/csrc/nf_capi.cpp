@@ -67,6 +67,7 @@ void ensure_lds_limits() {
     if (g_lds_status == hipSuccess) g_lds_status = fft3d_set_lds_limit();
     if (g_lds_status == hipSuccess) g_lds_status = dfttest_mfma_set_lds_limit();
     if (g_lds_status == hipSuccess) g_lds_status = dfttest_generic_set_lds_limit();
+    if (g_lds_status == hipSuccess) g_lds_status = sampling_set_lds_limit();
   });
   hip_check(g_lds_status, "hipFuncSetAttribute");
 }
@@ -410,9 +411,9 @@ extern "C" nf_status nf_dfttest_plan_create(int width, int height, nf_sample_for
     p->generic = c.smode == 0 || !(B == 8 || B == 16 || B == 32);
     p->sampled = c.ftype < 2 && c.n_nlocation > 0;  // plan.cpp:194
     if (p->generic) {
-      if (p->sampled)
-        throw Unsupported("DFTTest nlocation needs sbsize 8, 16 or 32 with smode=1 on this device path");
       if (B > 256) throw Unsupported("DFTTest sbsize above 256 is not available on this device path");
+      if (p->sampled && dft_noise_lds_bytes(B, T) > 160 * 1024)
+        throw Unsupported("DFTTest sample volume does not fit the LDS at this sbsize/tbsize");
       p->generic_lds = dfttest_generic_lds_bytes(B, T);
       if (p->generic_lds > 160 * 1024)
         throw Unsupported("DFTTest block volume does not fit the LDS at this sbsize/tbsize");
@@ -556,8 +557,8 @@ extern "C" nf_status nf_fft3d_plan_create(int width, int height, nf_sample_forma
     }
     p->varying = sig[1] != sig[0] || sig[2] != sig[0] || sig[3] != sig[0];
     p->preview = c.pshow && (p->sampled || p->varying);  // plan.cpp:113-115
-    if (p->generic && (p->preview || (p->sampled && c.bt != -1)))
-      throw Unsupported("FFT3D pfactor/pshow need bw=bh in {8,16,32} on this device path");
+    if (p->generic && (p->preview || p->sampled) && fft3d_pattern_generic_lds_bytes(c.bw, c.bh) > 160 * 1024)
+      throw Unsupported("FFT3D pattern block does not fit the LDS at this bw/bh");
     p->denoise = c.bt != -1 && !p->preview;
     p->kalman = c.bt == 0 && !p->preview;
     p->tabled = p->varying || (p->sampled && p->denoise);
@@ -1265,6 +1266,8 @@ PatternArgs pattern_args(const nf_plan& p, const nf_plane& src, unsigned* flags)
   A.grid = p.d_grid;
   A.degrid = p.degrid;
   A.weights = p.d_weights;
+  A.twx = p.d_twx;
+  A.twy = p.d_twy;
   const bool automatic = p.px == 0 && p.py == 0;  // plan.cpp:248-250
   const int left = automatic ? 2 : p.px, right = automatic ? p.gx.count - 3 : p.px;
   const int top = automatic ? 2 : p.py, bottom = automatic ? p.gy.count - 3 : p.py;
@@ -2055,6 +2058,8 @@ extern "C" nf_status nf_dfttest_prepare_noise(nf_plan* plan, const nf_plane* sam
     A.h = plan->d_sample_h;
     A.grid = plan->d_sample_grid;
     A.ttw = plan->d_ttw;
+    A.B = B;
+    A.tw = plan->d_tw;
     nf_status st_out = NF_OK;
     try {
       hip_check(hipMemcpyAsync(mem, rects.data(), sizeof(void*) * n_samples, hipMemcpyHostToDevice, st),

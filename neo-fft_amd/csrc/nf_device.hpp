@@ -243,6 +243,8 @@ struct DftNoiseArgs {
   const float* ttw;          // temporal twiddles for depths outside 3..5
   float* power;              // [nloc][T*B*(B/2+1)] per-location mean-removed power
   unsigned* flags;
+  int B;                     // sbsize
+  const float* tw;           // (cos, sin) of 2*pi*e/B (direct DFT of other block sizes)
 };
 
 // FFT3D pattern search / model / preview.
@@ -265,6 +267,8 @@ struct PatternArgs {
   int kalman;
   float* inv;           // preview block [bh][bw]
   unsigned* flags;
+  const float* twx;     // (cos, sin) of 2*pi*e/bw and 2*pi*e/bh (other block shapes)
+  const float* twy;
 };
 
 }  // namespace nf

@@ -17,28 +17,11 @@
 #include "nf_fft.hpp"
 #include "nf_gain.hpp"
 #include "nf_launch.hpp"
+#include "nf_generic.hpp"
 
 namespace nf {
 
 namespace {
-
-// Output k of a direct DFT of n values x[j * st] (real when xi is null).
-__device__ __forceinline__ void direct_term(int n, const float* xr, const float* xi, int st, int k, const float* tw,
-                                            bool fwd, float& out_r, float& out_i) {
-  float sr = 0.0f, si = 0.0f;
-  for (int j = 0; j < n; ++j) {
-    const int e = (j * k) % n;
-    const float c = tw[2 * e];
-    const float s = fwd ? -tw[2 * e + 1] : tw[2 * e + 1];
-    const float r = xr[j * st], i = xi ? xi[j * st] : 0.0f;
-    const float pr = r * c - i * s;
-    const float pi = r * s + i * c;
-    sr = sr + pr;
-    si = si + pi;
-  }
-  out_r = sr;
-  out_i = si;
-}
 
 __device__ __forceinline__ void store_output(const DftJob& J, int oyy, int oxx, float a) {
   char* drow = static_cast<char*>(J.dst) + (long long)oyy * J.dst_stride;

@@ -53,6 +53,8 @@ size_t dft_noise_lds_bytes(int B, int T);
 hipError_t launch_dft_noise(int B, const DftNoiseArgs& A, int nloc, float calibration, float* out,
                             hipStream_t stream);
 hipError_t launch_fft3d_pattern(int BW, const PatternArgs& A, int mode, hipStream_t stream);
+size_t fft3d_pattern_generic_lds_bytes(int BW, int BH);
+hipError_t sampling_set_lds_limit();
 hipError_t launch_pattern_argmin(const float* scores, int n, int* selected, hipStream_t stream);
 hipError_t launch_fft3d_preview_store(const PatternArgs& A, void* dst, long long dst_stride, float peak, int bw,
                                       int bh, hipStream_t stream);

@@ -20,6 +20,7 @@
 #include "nf_device.hpp"
 #include "nf_fft.hpp"
 #include "nf_launch.hpp"
+#include "nf_generic.hpp"
 
 namespace nf {
 
@@ -128,7 +129,82 @@ __global__ __launch_bounds__(256) void dft_noise_reduce_kernel(const float* powe
   if (bad) atomicOr(flags, NF_FLAG_NONFINITE);
 }
 
-size_t dft_noise_lds_bytes(int B, int T) { return sizeof(float) * 2 * (size_t)T * spec_block_stride(B); }
+// Block sizes outside the codelets: the same steps with direct DFTs.
+__global__ __launch_bounds__(128) void dft_noise_generic_kernel(const DftNoiseArgs A) {
+  extern __shared__ float smem[];
+  const int B = A.B, K = B / 2 + 1, T = A.T, BINS2 = B * K;
+  const int l = blockIdx.x;
+  const int tid = threadIdx.x, nthreads = blockDim.x;
+  float* tw = smem;             // [B][2]
+  float* x = tw + 2 * B;        // [T][B][B]
+  float* rre = x + T * B * B;   // [T][B][K]
+  float* rim = rre + T * BINS2;
+  float* xre = rim + T * BINS2; // [T][B][K]
+  float* xim = xre + T * BINS2;
+  bool bad = false;
+  for (int i = tid; i < 2 * B; i += nthreads) tw[i] = A.tw[i];
+  for (int i = tid; i < T * B * B; i += nthreads) {
+    const int z = i / (B * B), e = i - z * (B * B);
+    const int y = e / B, xx = e - y * B;
+    const float v = (load_sample(A.rects[l * T + z], A.strides[l * T + z], A.storage, y, xx) - 0.0f) * A.dscale;
+    bad |= !isfinite(v);
+    x[i] = (v * 1.0f) * A.h[i];
+  }
+  __syncthreads();
+  for (int t = tid; t < T * BINS2; t += nthreads) {
+    const int zy = t / K, k = t - zy * K;
+    float r, i;
+    direct_term(B, x + zy * B, nullptr, 1, k, tw, true, r, i);
+    rre[t] = r;
+    rim[t] = (B % 2 == 0 && k == B / 2) ? 0.0f : i;
+  }
+  __syncthreads();
+  for (int t = tid; t < T * BINS2; t += nthreads) {
+    const int z = t / BINS2, e = t - z * BINS2;
+    const int ky = e / K, k = e - ky * K;
+    direct_term(B, rre + z * BINS2 + k, rim + z * BINS2 + k, K, ky, tw, true, xre[t], xim[t]);
+  }
+  __syncthreads();
+  if (T > 1) {
+    for (int bin = tid; bin < BINS2; bin += nthreads) {
+      Cpx v[15];
+      for (int z = 0; z < T; ++z) v[z] = Cpx{xre[z * BINS2 + bin], xim[z * BINS2 + bin]};
+      if (T >= 3 && T <= 5) {
+        temporal_butterfly(T, v, true);
+      } else {
+        Cpx w[15];
+        for (int m = 0; m < T; ++m) w[m] = dft_term(T, v, m, A.ttw, true);
+        for (int m = 0; m < T; ++m) v[m] = w[m];
+      }
+      for (int z = 0; z < T; ++z) {
+        xre[z * BINS2 + bin] = v[z].r;
+        xim[z * BINS2 + bin] = v[z].i;
+      }
+    }
+    __syncthreads();
+  }
+  const float ratio = A.grid ? xre[0] / A.grid[0].x : 0.0f;
+  bad |= !isfinite(ratio);
+  float* q = A.power + (long long)l * T * BINS2;
+  for (int k = tid; k < T * BINS2; k += nthreads) {
+    float re = xre[k], im = xim[k];
+    if (A.grid) {
+      const float2 gv = A.grid[k];
+      re = re - ratio * gv.x;
+      im = im - ratio * gv.y;
+    }
+    const float p = (re * re) + (im * im);
+    bad |= !isfinite(p);
+    q[k] = p;
+  }
+  if (bad) atomicOr(A.flags, NF_FLAG_NONFINITE);
+}
+
+size_t dft_noise_lds_bytes(int B, int T) {
+  if (B == 8 || B == 16 || B == 32) return sizeof(float) * 2 * (size_t)T * spec_block_stride(B);
+  const size_t bins2 = (size_t)B * (B / 2 + 1);
+  return sizeof(float) * (2 * (size_t)B + (size_t)T * B * B + 4 * (size_t)T * bins2);
+}
 
 hipError_t launch_dft_noise(int B, const DftNoiseArgs& A, int nloc, float calibration, float* out,
                             hipStream_t stream) {
@@ -144,7 +220,8 @@ hipError_t launch_dft_noise(int B, const DftNoiseArgs& A, int nloc, float calibr
       hipLaunchKernelGGL(dft_noise_block_kernel<32>, dim3(nloc), dim3(256), lds, stream, A);
       break;
     default:
-      return hipErrorInvalidValue;
+      hipLaunchKernelGGL(dft_noise_generic_kernel, dim3(nloc), dim3(128), lds, stream, A);
+      break;
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -330,8 +407,154 @@ __global__ __launch_bounds__(256) void fft3d_preview_store_kernel(const PatternA
   }
 }
 
+namespace {
+template <int N>
+__device__ __forceinline__ void row_inverse_plain(const float* zr, const float* zi, float fct, float* out, bool& bad) {
+  constexpr int K = N / 2 + 1;
+  float re[N], im[N];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    re[k] = zr[k];
+    im[k] = zi[k];
+  }
+#pragma unroll
+  for (int k = K; k < N; ++k) {
+    re[k] = re[N - k];
+    im[k] = 0.0f - im[N - k];
+  }
+  cfft<N, false>(re, im);
+#pragma unroll
+  for (int x = 0; x < N; ++x) {
+    const float v = re[x] * fct;
+    bad |= !isfinite(v);
+    out[x] = v;
+  }
+}
+}  // namespace
+
+// Block shapes outside the square codelets: the same steps, each axis in the
+// codelet order when its length is 8, 16 or 32, else as a direct DFT.
+__global__ __launch_bounds__(64) void fft3d_pattern_generic_kernel(const PatternArgs A, int mode) {
+  extern __shared__ float smem[];
+  const int BW = A.gx.block, BH = A.gy.block, K = BW / 2 + 1, BINS = BH * K;
+  float* twx = smem;             // [BW][2]
+  float* twy = twx + 2 * BW;     // [BH][2]
+  float* win = twy + 2 * BH;     // [BH][BW]
+  float* sre = win + BH * BW;    // [BINS]
+  float* sim = sre + BINS;
+  float* tre = sim + BINS;       // [BINS] scratch
+  float* tim = tre + BINS;
+  float* pw = tim + BINS;        // [BINS]
+  const int tid = threadIdx.x, nthreads = blockDim.x;
+  const int cand = mode == 0 ? (int)blockIdx.x : *A.selected;
+  const int bx = A.left + cand % A.ncx, by = A.top + cand / A.ncx;
+  const int ox = bx * A.gx.step, oy = by * A.gy.step;
+  const bool window = mode != 2;
+  bool bad = false;
+  for (int i = tid; i < 2 * BW; i += nthreads) twx[i] = A.twx[i];
+  for (int i = tid; i < 2 * BH; i += nthreads) twy[i] = A.twy[i];
+  for (int idx = tid; idx < BH * BW; idx += nthreads) {
+    const int y = idx / BW, x = idx - y * BW;
+    const int sy = reflect(oy + y, A.gy), sx = reflect(ox + x, A.gx);
+    float v = load_sample(A.src, A.stride, A.storage, sy, sx) - A.base;
+    bad |= !isfinite(v);
+    if (window) v = (v * A.wya[y]) * A.wxa[x];
+    win[idx] = v;
+  }
+  __syncthreads();
+  if (radix_length(BW)) {
+    for (int y = tid; y < BH; y += nthreads) {
+      if (BW == 8) row_forward_radix<8>(win + y * BW, sre + y * K, sim + y * K);
+      else if (BW == 16) row_forward_radix<16>(win + y * BW, sre + y * K, sim + y * K);
+      else row_forward_radix<32>(win + y * BW, sre + y * K, sim + y * K);
+    }
+  } else {
+    for (int t = tid; t < BINS; t += nthreads) {
+      const int y = t / K, k = t - y * K;
+      float r, i;
+      direct_term(BW, win + y * BW, nullptr, 1, k, twx, true, r, i);
+      sre[t] = r;
+      sim[t] = (BW % 2 == 0 && k == BW / 2) ? 0.0f : i;
+    }
+  }
+  __syncthreads();
+  columns(BH, K, sre, sim, tre, tim, twy, true, tid, nthreads);
+  if (mode == 2) {
+    // preview: the unfiltered spectrum goes straight back (fft.inverse)
+    columns(BH, K, sre, sim, tre, tim, twy, false, tid, nthreads);
+    const float fct = 1.0f / ((float)BW * (float)BH);
+    if (radix_length(BW)) {
+      for (int y = tid; y < BH; y += nthreads) {
+        if (BW == 8) row_inverse_plain<8>(sre + y * K, sim + y * K, fct, A.inv + y * BW, bad);
+        else if (BW == 16) row_inverse_plain<16>(sre + y * K, sim + y * K, fct, A.inv + y * BW, bad);
+        else row_inverse_plain<32>(sre + y * K, sim + y * K, fct, A.inv + y * BW, bad);
+      }
+    } else {
+      for (int t = tid; t < BH * BW; t += nthreads) {
+        const int y = t / BW, xx = t - y * BW;
+        float sr = 0.0f;
+        for (int k = 0; k < BW; ++k) {
+          const float br = k < K ? sre[y * K + k] : sre[y * K + BW - k];
+          const float bi = k < K ? sim[y * K + k] : 0.0f - sim[y * K + BW - k];
+          const int e = (k * xx) % BW;
+          const float pr = br * twx[2 * e] - bi * twx[2 * e + 1];
+          sr = sr + pr;
+        }
+        const float v = sr * fct;
+        bad |= !isfinite(v);
+        A.inv[t] = v;
+      }
+    }
+    if (bad) atomicOr(A.flags, NF_FLAG_NONFINITE);
+    return;
+  }
+  const float g = A.grid ? (A.degrid * sre[0]) / A.grid[0].x : 0.0f;
+  bad |= !isfinite(A.degrid * sre[0]) || !isfinite(g);
+  for (int k = tid; k < BINS; k += nthreads) {
+    float re = sre[k], im = sim[k];
+    if (A.grid) {
+      const float2 gv = A.grid[k];
+      re = re - g * gv.x;
+      im = im - g * gv.y;
+    }
+    const float p = (re * re) + (im * im);
+    bad |= !isfinite(p);
+    pw[k] = p;
+  }
+  __syncthreads();
+  if (mode == 0) {
+    if (tid == 0) {
+      float score = 0.0f;
+      for (int k = 0; k < BINS; ++k) {
+        score = score + pw[k] * A.weights[k];
+        bad |= !isfinite(score);
+      }
+      A.scores[cand] = score;
+    }
+  } else {
+    for (int k = tid; k < BINS; k += nthreads) {
+      float v = A.factor * pw[k];
+      v = v * A.weights[k];
+      bad |= !isfinite(A.maximum * v);
+      if (A.kalman) bad |= !isfinite((v > 1e-15f ? v : 1e-15f) * A.ratio2);
+      A.model[k] = v;
+    }
+  }
+  if (bad) atomicOr(A.flags, NF_FLAG_NONFINITE);
+}
+
+size_t fft3d_pattern_generic_lds_bytes(int BW, int BH) {
+  const size_t bins = (size_t)BH * (BW / 2 + 1);
+  return sizeof(float) * (2 * (size_t)BW + 2 * (size_t)BH + (size_t)BW * BH + 5 * bins);
+}
+
 hipError_t launch_fft3d_pattern(int BW, const PatternArgs& A, int mode, hipStream_t stream) {
   const int grid = mode == 0 ? A.ncx * A.ncy : 1;
+  if (A.gy.block != BW || !(BW == 8 || BW == 16 || BW == 32)) {
+    hipLaunchKernelGGL(fft3d_pattern_generic_kernel, dim3(grid), dim3(64),
+                       fft3d_pattern_generic_lds_bytes(A.gx.block, A.gy.block), stream, A, mode);
+    return hipGetLastError();
+  }
   switch (BW) {
     case 8:
       hipLaunchKernelGGL(fft3d_pattern_kernel<8>, dim3(grid), dim3(64), 0, stream, A, mode);
@@ -346,6 +569,15 @@ hipError_t launch_fft3d_pattern(int BW, const PatternArgs& A, int mode, hipStrea
       return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t sampling_set_lds_limit() {
+  const void* fns[] = {(const void*)dft_noise_generic_kernel, (const void*)fft3d_pattern_generic_kernel};
+  for (const void* f : fns) {
+    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_pattern_argmin(const float* scores, int n, int* selected, hipStream_t stream) {

@@ -131,7 +131,7 @@ def test_generic_rejections(nf):
     with pytest.raises(nf.InvalidArgument):
         nf.dfttest_plan(64, 48, bits=8, tbsize=1, sbsize=8, smode=0)  # centre mode needs an odd sbsize
     with pytest.raises(nf.Unsupported):
-        nf.dfttest_plan(64, 48, bits=8, tbsize=1, sbsize=12, sosize=6, nlocation=[0, 0, 4, 4])
+        nf.dfttest_plan(256, 200, bits=8, tbsize=15, sbsize=64, sosize=32)  # block volume beyond the LDS
 
 
 # ------------------------------------------------------------------ FFT3D
@@ -199,6 +199,50 @@ def test_fft3d_generic_spectrum_cache(nf, oracle):
     assert int(flags.item()) == 0
 
 
-def test_fft3d_generic_rejects_sampling(nf):
+def test_fft3d_generic_rejects_oversized_blocks(nf):
     with pytest.raises(nf.Unsupported):
-        nf.fft3d_plan(96, 72, bits=8, bt=1, bw=24, bh=24, pfactor=0.5)
+        nf.fft3d_plan(400, 300, bits=8, bt=5, bw=120, bh=120)
+
+
+# ---------------------------------------------------- sampled noise models
+@pytest.mark.parametrize("B,T,smode", [(12, 1, 1), (12, 3, 1), (20, 1, 1), (9, 1, 0), (9, 3, 0)])
+def test_generic_dft_noise_model(nf, oracle, B, T, smode):
+    rng = np.random.default_rng(B * 7 + T)
+    h, w = 70, 90
+    frames = [[rng.integers(0, 256, (h, w)).astype(np.uint8)] for _ in range(T + 3)]
+    locations = [(0, 0, 3, 5), (2, 0, h - B, w - B), (1, 0, 20, 30)]
+    params = dict(tbsize=T, tmode=0, sbsize=B, sosize=B * 3 // 4 if smode else 0, smode=smode, ftype=0, sigma=8.0,
+                  zmean=True)
+    plan = nf.dfttest_plan(w, h, bits=8, nlocation=[v for loc in locations for v in loc], **params)
+    dev = [[to_device(p) for p in fr] for fr in frames]
+    nf.dfttest_prepare_noise(plan, [dev[f + z][p] for (f, p, _, _) in locations for z in range(T)])
+    ref_table = oracle.dft_noise_model(frames, locations, bits=8, **params)
+    assert np.array_equal(nf.read_model(plan), ref_table)
+    slots = [frames[z][0] for z in range(T)]
+    dst = empty_like_np(slots[0])
+    plan.process([to_device(s) for s in slots], dst)
+    ref = oracle.dfttest_plane(slots, bits=8, noise_table=ref_table, **params)
+    assert np.array_equal(to_host(dst, np.uint8), ref)
+
+
+@pytest.mark.parametrize("bw,bh", [(24, 24), (32, 24), (20, 12)])
+@pytest.mark.parametrize("bt", [1, 3])
+def test_generic_fft3d_pattern(nf, oracle, bw, bh, bt):
+    rng = np.random.default_rng(bw + bh + bt)
+    h, w = 120, 150
+    frames = [rng.integers(0, 256, (h, w)).astype(np.uint8) for _ in range(bt)]
+    params = dict(bw=bw, bh=bh, bt=bt, pfactor=1.5, sharpen=0.3, sigma=3.0)
+    plan = nf.fft3d_plan(w, h, bits=8, **params)
+    sel = nf.fft3d_prepare_pattern(plan, to_device(frames[0]))
+    (rx, ry), _, ref_model = oracle.fft3d_pattern(frames[0], bits=8, **params)
+    assert sel == (rx, ry)
+    assert np.array_equal(nf.read_model(plan), ref_model)
+    dst = empty_like_np(frames[0])
+    plan.process([to_device(f) for f in frames], dst)
+    ref = oracle.fft3d_plane(frames, bits=8, pattern=ref_model, **params)
+    assert np.array_equal(to_host(dst, np.uint8), ref)
+    preview = nf.fft3d_plan(w, h, bits=8, pshow=True, **params)
+    dst = empty_like_np(frames[0])
+    preview.process([to_device(f) for f in frames], dst)
+    ref = oracle.fft3d_plane(frames, bits=8, pshow=True, **params)
+    assert np.array_equal(to_host(dst, np.uint8), ref)
