@@ -247,7 +247,8 @@ def measured_traffic(kernel):
 
 def mfma_error(nf, torch):
     """Max and RMS error of both forms against a binary64 evaluation on a
-    176x120 float32 plane (smooth texture + noise), tbsize 1 and 3."""
+    176x120 float32 plane (smooth texture + noise), sbsize 8 and 16 (sosize
+    3/4 of it), tbsize 1 and 3."""
     import numpy as np
     import dfttest_fp64 as F64
     from oracle import oracle as O
@@ -256,20 +257,24 @@ def mfma_error(nf, torch):
     h, w = 120, 176
     yy, xx = np.mgrid[0:h, 0:w]
     out = {}
-    for T in (1, 3):
-        slots = []
-        for _ in range(T):
-            img = 0.5 + 0.25 * np.sin(xx * 0.11 + yy * 0.05) + 0.1 * np.cos(yy * 0.3)
-            slots.append(np.clip(img + rng.normal(0, 0.03, (h, w)), 0, 1).astype(np.float32))
-        hwin, ws = O.dft_window_3d(T, 16, 12, 1, 0, 7)
-        ref = F64.dfttest_plane([s_.astype(np.float64) for s_ in slots], hwin, ws, 8.0)
-        for name, form in (("radix2", nf.TRANSFORM_RADIX2), ("mfma", nf.TRANSFORM_MFMA_F16X3)):
-            plan = nf.dfttest_plan(w, h, bits=32, floating=True, tbsize=T, sbsize=16, sosize=12, ftype=0, sigma=8.0)
-            nf.set_transform(plan, form)
-            dst = empty_like_np(slots[0])
-            plan.process([to_device(s_) for s_ in slots], dst)
-            d = np.abs(to_host(dst, np.float32).astype(np.float64) - ref)
-            out[f"{name}_t{T}"] = {"max": float(f"{d.max():.3g}"), "rms": float(f"{np.sqrt((d ** 2).mean()):.3g}")}
+    for B in (8, 16):
+        ov = 3 * B // 4
+        for T in (1, 3):
+            slots = []
+            for _ in range(T):
+                img = 0.5 + 0.25 * np.sin(xx * 0.11 + yy * 0.05) + 0.1 * np.cos(yy * 0.3)
+                slots.append(np.clip(img + rng.normal(0, 0.03, (h, w)), 0, 1).astype(np.float32))
+            hwin, ws = O.dft_window_3d(T, B, ov, 1, 0, 7)
+            ref = F64.dfttest_plane([s_.astype(np.float64) for s_ in slots], hwin, ws, 8.0, block=B, overlap=ov)
+            for name, form in (("radix2", nf.TRANSFORM_RADIX2), ("mfma", nf.TRANSFORM_MFMA_F16X3)):
+                plan = nf.dfttest_plan(w, h, bits=32, floating=True, tbsize=T, sbsize=B, sosize=ov, ftype=0,
+                                       sigma=8.0)
+                nf.set_transform(plan, form)
+                dst = empty_like_np(slots[0])
+                plan.process([to_device(s_) for s_ in slots], dst)
+                d = np.abs(to_host(dst, np.float32).astype(np.float64) - ref)
+                out[f"{name}_sbsize{B}_t{T}"] = {"max": float(f"{d.max():.3g}"),
+                                                 "rms": float(f"{np.sqrt((d ** 2).mean()):.3g}")}
     return out
 
 
@@ -382,8 +387,9 @@ def batched_clips(nf, torch, stream, frames=8, reps=2):
     6/12/24): per frame index ONE persistent launch over all 192 planes, the
     8 launches of the frame group captured in one hipGraph and replayed;
     frames resident in HBM. The schedule of the first launch is autotuned and
-    reused by the others. For sbsize 16 also the matrix-core form, and the
-    same frame index as 64 per-clip nf_process_batch calls (unbatched)."""
+    reused by the others. For sbsize 8 and 16 also the matrix-core form; for
+    sbsize 16 the same frame index as 64 per-clip nf_process_batch calls
+    (unbatched)."""
     from synthetic import yuv420_sinusoids_device
     dev = torch.device("cuda:0")
     clips, sizes = [], []
@@ -399,7 +405,8 @@ def batched_clips(nf, torch, stream, frames=8, reps=2):
                        "hipGraph per 8-frame group",
            "bytes_per_frame_index": bytes_per_index}
     torch.cuda.synchronize()
-    for B, O, form in ((8, 6, "radix2"), (16, 12, "radix2"), (16, 12, "mfma_f16x3"), (32, 24, "radix2")):
+    for B, O, form in ((8, 6, "radix2"), (8, 6, "mfma_f16x3"), (16, 12, "radix2"), (16, 12, "mfma_f16x3"),
+                       (32, 24, "radix2")):
         plans = {}
         for w, h in BATCH_SIZES:
             for ch in (False, True):
@@ -440,8 +447,9 @@ def batched_clips(nf, torch, stream, frames=8, reps=2):
             res[key]["unbatched_ms_per_frame_index"] = round(ums / frames, 3)
             res[key]["batched_speedup"] = round(ums / frames / (ms / frames), 3)
         del graph, batches, first
-    res["mfma_speedup_sbsize16"] = round(res["sbsize16_mfma_f16x3"]["clip_frames_per_s"]
-                                         / res["sbsize16_radix2"]["clip_frames_per_s"], 3)
+    for B in (8, 16):
+        res[f"mfma_speedup_sbsize{B}"] = round(res[f"sbsize{B}_mfma_f16x3"]["clip_frames_per_s"]
+                                              / res[f"sbsize{B}_radix2"]["clip_frames_per_s"], 3)
     if int(flags.item()) != 0:
         res["error"] = "non-finite flag raised"
     return res

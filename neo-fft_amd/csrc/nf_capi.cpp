@@ -1774,7 +1774,7 @@ extern "C" nf_status nf_batch_create(const nf_plan* const* plans, int n_jobs, co
     b->T = p0->temporal;
     b->kind = p0->transform == NF_TRANSFORM_MFMA_F16X3 ? 2 : (p0->temporal > 1 ? 1 : 0);
     if (b->kind == 2) {
-      require(dfttest_mfma_supported(b->B, b->T), "matrix-core form needs sbsize 16 and tbsize 1, 3 or 5");
+      require(dfttest_mfma_supported(b->B, b->T), "matrix-core form needs sbsize 8 or 16 and tbsize 1, 3 or 5");
       b->tw = p0->d_tw;
     }
     for (int i = 0; i < n_jobs; ++i) {
@@ -1994,7 +1994,7 @@ extern "C" nf_status nf_plan_set_transform(nf_plan* plan, int transform) {
     if (transform == NF_TRANSFORM_MFMA_F16X3) {
       if (plan->algorithm != 0) throw Unsupported("the matrix-core form covers DFTTest plans");
       if (!dfttest_mfma_supported(plan->B, plan->temporal) || plan->tmode != 0)
-        throw Unsupported("the matrix-core form covers sbsize 16 with tbsize 1, 3 or 5 (tmode 0)");
+        throw Unsupported("the matrix-core form covers sbsize 8 or 16 with tbsize 1, 3 or 5 (tmode 0)");
       if (plan->dither > 0 && plan->storage == kU8) throw Unsupported("the matrix-core form does not dither");
     }
     plan->transform = transform;

@@ -156,6 +156,9 @@ __host__ __device__ constexpr int t1_block_stride(int B) { return B == 8 ? 101 :
 // Source-window row stride of the matrix-core kernel: at least n, a multiple
 // of four with stride/4 odd (float4 loads of eight rows hit distinct banks).
 __host__ __device__ constexpr int mfma_window_stride(int n) { return ((n + 3) / 8) * 8 + 4; }
+// Block-output row stride of the matrix-core kernel (a multiple of four with
+// stride/4 odd, at least B).
+__host__ __device__ constexpr int mfma_out_stride(int B) { return B <= 12 ? 12 : B <= 20 ? 20 : 36; }
 // The same for the temporal kernel, whose consecutive blocks are tbsize
 // slices apart and whose temporal stages walk the bins of a block in order
 // (one layout chosen over depths 2..7, weighting every stage's accesses).

@@ -87,16 +87,24 @@ __device__ __forceinline__ Hl split_scaled(const float* v, float scale) {
   return split(t);
 }
 
-// One output tile of job J with temporal inputs Q. tw: cos/sin of 2*pi*k/16,
-// k = 0..15, as (cos, sin) pairs (binary64 -> float). ORs NF_FLAG_NONFINITE
+// One output tile of job J with temporal inputs Q. tw: cos/sin of 2*pi*k/B,
+// k = 0..B-1, as (cos, sin) pairs (binary64 -> float). ORs NF_FLAG_NONFINITE
 // into *flags on a non-finite sample or intermediate.
-template <int TT>
-__device__ __forceinline__ void mfma16_tile(const DftJob& J, const DftTemporal& Q, int tile, const float* tw,
-                                            unsigned* flags) {
+//
+// B = 8 packs four blocks into each 16x16 tile (tile rows 0-7 hold blocks
+// u, u+1 side by side, rows 8-15 blocks u+2, u+3) and uses block-diagonal
+// DFT matrices, so the same four products transform four blocks; the
+// transposed products swap the off-diagonal quadrants, which the gain and
+// output index maps follow.
+template <int B, int TT>
+__device__ __forceinline__ void mfma_tile(const DftJob& J, const DftTemporal& Q, int tile, const float* tw,
+                                          unsigned* flags) {
   // OS: row stride of the windows and block outputs, a multiple of four with
   // OS/4 odd, so the float4 accesses of eight lanes (rows r..r+7) fall on
   // distinct bank groups; the source window uses the same rule (ww).
-  constexpr int B = 16, K = B / 2 + 1, OS = 20, BINS2 = B * K;
+  constexpr int K = B / 2 + 1, OS = mfma_out_stride(B), BINS2 = B * K;
+  constexpr int BQ = 16 / B, PER_UNIT = BQ * BQ;  // blocks per tile side / per tile
+  constexpr float kInvB = 1.0f / (float)B;
   extern __shared__ __align__(16) float smem[];
   const int tx = tile % J.tiles_x, ty = tile / J.tiles_x;
   const int S = J.S;
@@ -129,44 +137,55 @@ __device__ __forceinline__ void mfma16_tile(const DftJob& J, const DftTemporal& 
   for (int i = tid; i < B * B; i += nthreads) hs[(i / B) * OS + i % B] = J.hsyn[target * B * B + i];
   for (int i = tid; i < B * J.tile_w; i += nthreads) ring[i] = 0.0f;
 
-  // Per-lane DFT-matrix fragments; entry (r, m = 4q+e) is cos/sin(2 pi r m / 16).
+  // Per-lane DFT-matrix fragments; entry (r, m = 4q+e) is cos/sin(2 pi r m / B)
+  // inside a diagonal B x B block of the tile, 0 outside.
+  const int rb = r % B;
   float cs[4], sn[4];
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
-    const int m = 4 * q + e, idx = (r * m) & (B - 1);
-    cs[e] = tw[2 * idx];
-    sn[e] = tw[2 * idx + 1];
+    const int m = 4 * q + e, mb = m % B, idx = (rb * mb) & (B - 1);
+    const bool diag = r / B == m / B;
+    cs[e] = diag ? tw[2 * idx] : 0.0f;
+    sn[e] = diag ? tw[2 * idx + 1] : 0.0f;
   }
   float t[4];
   // row transform (B operand, k = x = m, j = kx = r): cos, -sin, kx < K
 #pragma unroll
-  for (int e = 0; e < 4; ++e) t[e] = r < K ? cs[e] : 0.0f;
+  for (int e = 0; e < 4; ++e) t[e] = rb < K ? cs[e] : 0.0f;
   const Hl Cf = split_scaled(t, kConstScale);
 #pragma unroll
-  for (int e = 0; e < 4; ++e) t[e] = r < K ? -sn[e] : 0.0f;
+  for (int e = 0; e < 4; ++e) t[e] = rb < K ? -sn[e] : 0.0f;
   const Hl Sf = split_scaled(t, kConstScale);
   // column transform (A operand, i = ky = r, k = y = m): cos, sin, -sin
   const Hl C2 = split_scaled(cs, kConstScale);
   const Hl S2 = split_scaled(sn, kConstScale);
   const Hl S2n = split_scaled(sn, -kConstScale);
-  // inverse column transform (B operand, k = ky = m, j = yy = r), 1/16
-  const Hl C3 = split_scaled(cs, kConstScale * 0.0625f);
-  const Hl S3 = split_scaled(sn, kConstScale * 0.0625f);
-  const Hl S3n = split_scaled(sn, -kConstScale * 0.0625f);
+  // inverse column transform (B operand, k = ky = m, j = yy = r), 1/B
+  const Hl C3 = split_scaled(cs, kConstScale * kInvB);
+  const Hl S3 = split_scaled(sn, kConstScale * kInvB);
+  const Hl S3n = split_scaled(sn, -kConstScale * kInvB);
   // inverse row transform (A operand, i = xx = r, k = kx = m): Hermitian
-  // weights, 1/16, kx < K
+  // weights, 1/B, kx < K
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
-    const int m = 4 * q + e;
-    t[e] = m < K ? ((m == 0 || m == B / 2) ? 1.0f : 2.0f) * cs[e] * 0.0625f : 0.0f;
+    const int mb = (4 * q + e) % B;
+    t[e] = mb < K ? ((mb == 0 || mb == B / 2) ? 1.0f : 2.0f) * cs[e] * kInvB : 0.0f;
   }
   const Hl Mc = split_scaled(t, kConstScale);
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
-    const int m = 4 * q + e;
-    t[e] = m < K ? -((m == 0 || m == B / 2) ? 1.0f : 2.0f) * sn[e] * 0.0625f : 0.0f;
+    const int mb = (4 * q + e) % B;
+    t[e] = mb < K ? -((mb == 0 || mb == B / 2) ? 1.0f : 2.0f) * sn[e] * kInvB : 0.0f;
   }
   const Hl Ms = split_scaled(t, kConstScale);
+  // index maps of this lane's four elements: input rows/columns (A layout,
+  // row r, columns 4q+e), spectrum bins (D layout, rows 4q+e, column r) and
+  // outputs (transposed D layout, rows xx = 4q+e, columns yy = r)
+  const int in_blk = BQ * (r / B) + (4 * q) / B, in_y = rb, in_x0 = (4 * q) % B;
+  // spectrum block of this lane: BQ * ((4 * q) / B) + r / B
+  const int sp_ky0 = (4 * q) % B, sp_kx = rb;
+  const int dc_lane = 16 * ((4 * q) / B) * (B / 4) + B * (r / B);
+  const int out_blk = BQ * (r / B) + (4 * q) / B, out_yy = rb, out_xx0 = (4 * q) % B;
   const float tscale = TT > 1 ? 1.0f / (float)TT : 1.0f;
   const bool use_grid = J.grid && J.mean_scale != 0.0f && J.grid[0].x != 0.0f;
   __syncthreads();
@@ -185,10 +204,12 @@ __device__ __forceinline__ void mfma16_tile(const DftJob& J, const DftTemporal& 
       win[(z * B + rr) * ww + c] = v;
     }
     __syncthreads();
-    // M: each wave transforms two blocks at a time (independent product
-    // chains interleave): forward products, gain, inverse products
+    // M: each wave transforms NP tiles (of PER_UNIT blocks) at a time
+    // (independent product chains interleave): forward products, gain,
+    // inverse products
     constexpr int NP = TT >= 5 ? 1 : 2;
-    for (int b0 = wave * NP; b0 < nbx; b0 += nwaves * NP) {
+    const int nunits = (nbx + PER_UNIT - 1) / PER_UNIT;
+    for (int b0 = wave * NP; b0 < nunits; b0 += nwaves * NP) {
       f4 Xr[NP][TT], Xi[NP][TT];
       const f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -196,11 +217,11 @@ __device__ __forceinline__ void mfma16_tile(const DftJob& J, const DftTemporal& 
         Hl xs[NP];
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
-          const int bi = min(b0 + p, nbx - 1);
+          const int bi = min((b0 + p) * PER_UNIT + in_blk, nbx - 1);
           f4 xv;
           if (vec) {
-            const float4 a = *reinterpret_cast<const float4*>(win + (z * B + r) * ww + bi * S + 4 * q);
-            const float4 h = *reinterpret_cast<const float4*>(hw + (z * B + r) * OS + 4 * q);
+            const float4 a = *reinterpret_cast<const float4*>(win + (z * B + in_y) * ww + bi * S + in_x0);
+            const float4 h = *reinterpret_cast<const float4*>(hw + (z * B + in_y) * OS + in_x0);
             xv[0] = a.x * h.x;
             xv[1] = a.y * h.y;
             xv[2] = a.z * h.z;
@@ -208,8 +229,8 @@ __device__ __forceinline__ void mfma16_tile(const DftJob& J, const DftTemporal& 
           } else {
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-              const int x = 4 * q + e;
-              xv[e] = win[(z * B + r) * ww + bi * S + x] * hw[(z * B + r) * OS + x];
+              const int x = in_x0 + e;
+              xv[e] = win[(z * B + in_y) * ww + bi * S + x] * hw[(z * B + in_y) * OS + x];
             }
           }
           xs[p] = split(xv);
@@ -244,17 +265,18 @@ __device__ __forceinline__ void mfma16_tile(const DftJob& J, const DftTemporal& 
             }
           }
         }
-        // zmean scale from the (3D) DC: lane 0, element 0
-        const float dcr = __shfl(Xr[p][0][0], 0, 64);
+        // zmean scale from the (3D) DC of this lane's block: element 0 of
+        // the lane holding bin (0, 0) of the block
+        const float dcr = __shfl(Xr[p][0][0], dc_lane, 64);
         const float gs = use_grid ? (J.mean_scale * dcr) / J.grid[0].x : 0.0f;
         // gain over the (3D) spectrum
 #pragma unroll
         for (int z = 0; z < TT; ++z) {
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
-            const int ky = 4 * q + e;
-            if (r < K) {
-              const int k3 = z * BINS2 + ky * K + r;
+            const int ky = sp_ky0 + e;
+            if (sp_kx < K) {
+              const int k3 = z * BINS2 + ky * K + sp_kx;
               float re = Xr[p][z][e], im = Xi[p][z][e], mr = 0.0f, mi = 0.0f;
               if (J.grid) {
                 const float2 gv = J.grid[k3];
@@ -301,13 +323,13 @@ __device__ __forceinline__ void mfma16_tile(const DftJob& J, const DftTemporal& 
 #pragma unroll
       for (int p = 0; p < NP; ++p) {
         const f4 yT = unscale(mma(Ms, Zi[p], mma(Mc, Zr[p], zero)));
-        const int bi = b0 + p;
+        const int bi = (b0 + p) * PER_UNIT + out_blk;
         if (bi < nbx) {
-          // yT: column yy = r, rows xx = 4q+e
-          const float4 h = *reinterpret_cast<const float4*>(hs + r * OS + 4 * q);
+          // yT: column yy, rows xx..xx+3 of block bi
+          const float4 h = *reinterpret_cast<const float4*>(hs + out_yy * OS + out_xx0);
 #pragma unroll
           for (int e = 0; e < 4; ++e) bad |= !isfinite(yT[e]);
-          *reinterpret_cast<float4*>(out + (bi * B + r) * OS + 4 * q) =
+          *reinterpret_cast<float4*>(out + (bi * B + out_yy) * OS + out_xx0) =
               make_float4(yT[0] * h.x, yT[1] * h.y, yT[2] * h.z, yT[3] * h.w);
         }
       }
@@ -364,81 +386,72 @@ __device__ __forceinline__ void mfma16_tile(const DftJob& J, const DftTemporal& 
 // Register budget: the compiler's default allocation for depths 3 and 5
 // exceeds 256 VGPRs (one wave per SIMD); capped at two waves per SIMD (four
 // for depth 1) it fits without spills.
-template <int TT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TT == 1 ? 4 : 2))) void dfttest_mfma16_kernel(
+template <int B, int TT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TT == 1 ? 4 : 2))) void dfttest_mfma_kernel(
     const DftVolLaunch L, const float* tw) {
   int jid = 0;
   while (jid + 1 < L.njobs && (int)blockIdx.x >= L.tile_start[jid + 1]) ++jid;
-  mfma16_tile<TT>(L.jobs[jid], L.tq[jid], (int)blockIdx.x - L.tile_start[jid], tw, L.flags);
+  mfma_tile<B, TT>(L.jobs[jid], L.tq[jid], (int)blockIdx.x - L.tile_start[jid], tw, L.flags);
 }
 
-template <int TT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TT == 1 ? 4 : 2))) void dfttest_mfma16_list_kernel(
+template <int B, int TT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TT == 1 ? 4 : 2))) void dfttest_mfma_list_kernel(
     const DftWorkList W, const float* tw) {
   for_each_work_item(W, [&](int j, int tile) {
-    mfma16_tile<TT>(list_job(W, j), list_temporal(W, j), tile, tw, W.flags);
+    mfma_tile<B, TT>(list_job(W, j), list_temporal(W, j), tile, tw, W.flags);
   });
 }
 
 size_t dfttest_mfma_lds_bytes(int B, int T, int nbx_max, int ww, int tile_w) {
-  const size_t OS = 20;
+  const size_t OS = (size_t)mfma_out_stride(B);
   return sizeof(float) * ((size_t)T * B * mfma_window_stride(ww) + (size_t)nbx_max * B * OS + (size_t)B * tile_w +
                           (size_t)T * B * OS + (size_t)B * OS);
 }
 
-bool dfttest_mfma_supported(int B, int T) { return B == 16 && (T == 1 || T == 3 || T == 5); }
+bool dfttest_mfma_supported(int B, int T) { return (B == 8 || B == 16) && (T == 1 || T == 3 || T == 5); }
 
-hipError_t launch_dfttest_mfma(int B, int T, const DftVolLaunch& L, int total_tiles, size_t lds, const float* tw,
-                               hipStream_t stream) {
-  if (B != 16) return hipErrorInvalidValue;
-  dim3 grid(total_tiles), block(256);
+namespace {
+template <int B>
+const void* mfma_function(int T, bool list) {
   switch (T) {
-    case 1:
-      hipLaunchKernelGGL(dfttest_mfma16_kernel<1>, grid, block, lds, stream, L, tw);
-      break;
-    case 3:
-      hipLaunchKernelGGL(dfttest_mfma16_kernel<3>, grid, block, lds, stream, L, tw);
-      break;
-    case 5:
-      hipLaunchKernelGGL(dfttest_mfma16_kernel<5>, grid, block, lds, stream, L, tw);
-      break;
-    default:
-      return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
-
-const void* dfttest_mfma_list_function(int B, int T) {
-  if (B != 16) return nullptr;
-  switch (T) {
-    case 1: return (const void*)dfttest_mfma16_list_kernel<1>;
-    case 3: return (const void*)dfttest_mfma16_list_kernel<3>;
-    case 5: return (const void*)dfttest_mfma16_list_kernel<5>;
+    case 1: return list ? (const void*)dfttest_mfma_list_kernel<B, 1> : (const void*)dfttest_mfma_kernel<B, 1>;
+    case 3: return list ? (const void*)dfttest_mfma_list_kernel<B, 3> : (const void*)dfttest_mfma_kernel<B, 3>;
+    case 5: return list ? (const void*)dfttest_mfma_list_kernel<B, 5> : (const void*)dfttest_mfma_kernel<B, 5>;
     default: return nullptr;
   }
 }
+const void* mfma_function(int B, int T, bool list) {
+  return B == 16 ? mfma_function<16>(T, list) : B == 8 ? mfma_function<8>(T, list) : nullptr;
+}
+}  // namespace
+
+hipError_t launch_dfttest_mfma(int B, int T, const DftVolLaunch& L, int total_tiles, size_t lds, const float* tw,
+                               hipStream_t stream) {
+  const void* f = mfma_function(B, T, false);
+  if (!f) return hipErrorInvalidValue;
+  void* args[] = {const_cast<DftVolLaunch*>(&L), const_cast<const float**>(&tw)};
+  return hipLaunchKernel(f, dim3(total_tiles), dim3(256), args, lds, stream);
+}
+
+const void* dfttest_mfma_list_function(int B, int T) { return mfma_function(B, T, true); }
 
 hipError_t launch_dfttest_mfma_list(int B, int T, const DftWorkList& W, int workgroups, size_t lds, const float* tw,
                                     hipStream_t stream) {
   const void* f = dfttest_mfma_list_function(B, T);
   if (!f) return hipErrorInvalidValue;
-  dim3 grid(workgroups), block(256);
   void* args[] = {const_cast<DftWorkList*>(&W), const_cast<const float**>(&tw)};
-  return hipLaunchKernel(f, grid, block, args, lds, stream);
+  return hipLaunchKernel(f, dim3(workgroups), dim3(256), args, lds, stream);
 }
 
 hipError_t dfttest_mfma_set_lds_limit() {
-  const void* fns[] = {(const void*)dfttest_mfma16_kernel<1>, (const void*)dfttest_mfma16_kernel<3>,
-                       (const void*)dfttest_mfma16_kernel<5>};
-  for (const void* f : fns) {
-    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    if (e != hipSuccess) return e;
-  }
-  for (int T : {1, 3, 5}) {
-    const hipError_t e =
-        hipFuncSetAttribute(dfttest_mfma_list_function(16, T), hipFuncAttributeMaxDynamicSharedMemorySize, kListLdsLimit);
-    if (e != hipSuccess) return e;
-  }
+  for (int B : {8, 16})
+    for (int T : {1, 3, 5}) {
+      hipError_t e = hipFuncSetAttribute(mfma_function(B, T, false), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         160 * 1024);
+      if (e != hipSuccess) return e;
+      e = hipFuncSetAttribute(mfma_function(B, T, true), hipFuncAttributeMaxDynamicSharedMemorySize, kListLdsLimit);
+      if (e != hipSuccess) return e;
+    }
   return hipSuccess;
 }
 

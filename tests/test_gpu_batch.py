@@ -69,12 +69,13 @@ def test_batch_matches_oracle(nf, oracle, B, O, T):
 
 
 @pytest.mark.parametrize("T", [1, 3])
-def test_batch_matrix_core_matches_single(nf, T):
+@pytest.mark.parametrize("B", [8, 16])
+def test_batch_matrix_core_matches_single(nf, T, B):
     import torch
-    clips = _clips(16, 3, seed=90 + T)
+    clips = _clips(16, 3, seed=90 + T + B)
     cache = {}
-    jobs, _ = _jobs(nf, clips, 1, T, 16, 12, cache, transform=nf.TRANSFORM_MFMA_F16X3)
-    print(f"matrix-core batch T={T} jobs={len(jobs)}", flush=True)
+    jobs, _ = _jobs(nf, clips, 1, T, B, 3 * B // 4, cache, transform=nf.TRANSFORM_MFMA_F16X3)
+    print(f"matrix-core batch B={B} T={T} jobs={len(jobs)}", flush=True)
     batch = nf.Batch(jobs)
     for _, _, dst in jobs:
         dst.zero_()

@@ -1,6 +1,7 @@
 """Matrix-core DFTTest form (FP16 split hi/lo DFT-matrix products, FP32
 accumulation) against a binary64 evaluation of the same filter: float planes
-must stay within 1e-4 of the plane's range (max) and report their RMS error;
+must stay within 1e-4 of the plane's range (max) and report their RMS error
+(sbsize 8 and 16);
 the radix-2 FP32 form is measured the same way for comparison. Integer planes
 must agree with the bit-exact radix-2 form to one step everywhere (8-bit: on
 all but 0.1% of samples exactly)."""
@@ -26,13 +27,15 @@ def _run(nf, plan, slots, dtype):
 
 
 @pytest.mark.parametrize("T", [1, 3, 5])
-def test_mfma_float_error_against_binary64(nf, oracle, T):
-    rng = np.random.default_rng(40 + T)
+@pytest.mark.parametrize("B", [8, 16])
+def test_mfma_float_error_against_binary64(nf, oracle, T, B):
+    rng = np.random.default_rng(40 + T + B)
     h, w = 120, 176
     slots = [_smooth_noisy(rng, h, w) for _ in range(T)]
-    params = dict(tbsize=T, sbsize=16, sosize=12, ftype=0, sigma=8.0)
-    hwin, ws = oracle.dft_window_3d(T, 16, 12, 1, 0, 7)
-    ref = F64.dfttest_plane([s.astype(np.float64) for s in slots], hwin, ws, 8.0)
+    O = 3 * B // 4
+    params = dict(tbsize=T, sbsize=B, sosize=O, ftype=0, sigma=8.0)
+    hwin, ws = oracle.dft_window_3d(T, B, O, 1, 0, 7)
+    ref = F64.dfttest_plane([s.astype(np.float64) for s in slots], hwin, ws, 8.0, block=B, overlap=O)
     errs = {}
     for form in (nf.TRANSFORM_RADIX2, nf.TRANSFORM_MFMA_F16X3):
         plan = nf.dfttest_plan(w, h, bits=32, floating=True, **params)
@@ -40,7 +43,7 @@ def test_mfma_float_error_against_binary64(nf, oracle, T):
         got = _run(nf, plan, slots, np.float32).astype(np.float64)
         d = np.abs(got - ref)
         errs[form] = (d.max(), np.sqrt((d ** 2).mean()))
-    print(f"T={T} max/rms error vs binary64: radix2 {errs[0]}, mfma {errs[1]}")
+    print(f"B={B} T={T} max/rms error vs binary64: radix2 {errs[0]}, mfma {errs[1]}")
     assert errs[nf.TRANSFORM_RADIX2][0] <= 2e-6
     assert errs[nf.TRANSFORM_MFMA_F16X3][0] <= 1e-4
     assert errs[nf.TRANSFORM_MFMA_F16X3][1] <= 1e-5
@@ -48,14 +51,15 @@ def test_mfma_float_error_against_binary64(nf, oracle, T):
 
 @pytest.mark.parametrize("T", [1, 3])
 @pytest.mark.parametrize("bits", [8, 16])
-def test_mfma_integer_planes_match_radix2(nf, oracle, T, bits):
-    rng = np.random.default_rng(bits + T)
+@pytest.mark.parametrize("B", [8, 16])
+def test_mfma_integer_planes_match_radix2(nf, oracle, T, bits, B):
+    rng = np.random.default_rng(bits + T + B)
     h, w = 100, 150
     peak = (1 << bits) - 1
     base = rng.integers(0, peak + 1, (h, w))
     dt = np.uint8 if bits == 8 else np.uint16
     slots = [np.clip(base + rng.integers(-peak // 30, peak // 30 + 1, (h, w)), 0, peak).astype(dt) for _ in range(T)]
-    params = dict(tbsize=T, sbsize=16, sosize=12, ftype=0, sigma=6.0)
+    params = dict(tbsize=T, sbsize=B, sosize=3 * B // 4, ftype=0, sigma=6.0)
     ref = oracle.dfttest_plane(slots, bits=bits, **params)
     plan = nf.dfttest_plan(w, h, bits=bits, **params)
     nf.set_transform(plan, nf.TRANSFORM_MFMA_F16X3)
@@ -69,7 +73,10 @@ def test_mfma_integer_planes_match_radix2(nf, oracle, T, bits):
 
 
 def test_mfma_form_coverage(nf):
-    plan = nf.dfttest_plan(64, 64, bits=8, tbsize=1, sbsize=8, sosize=6)
+    plan = nf.dfttest_plan(96, 96, bits=8, tbsize=1, sbsize=32, sosize=24)
+    with pytest.raises(nf.Unsupported):
+        nf.set_transform(plan, nf.TRANSFORM_MFMA_F16X3)
+    plan = nf.dfttest_plan(64, 64, bits=8, tbsize=1, sbsize=12, sosize=9)
     with pytest.raises(nf.Unsupported):
         nf.set_transform(plan, nf.TRANSFORM_MFMA_F16X3)
     plan = nf.fft3d_plan(64, 64, bits=8)
