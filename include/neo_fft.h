@@ -42,8 +42,12 @@ typedef enum nf_status {
   NF_EDEVICE = 5       /* HIP runtime error */
 } nf_status;
 
-/* Error bits OR-ed by the kernels into a caller-supplied device word. */
+/* Error bits OR-ed by the kernels into a caller-supplied device word.
+ * NF_FLAG_RANGE: a block of the matrix-core form (NF_TRANSFORM_MFMA_F16X3)
+ * held magnitudes beyond its FP16 operands (float planes far outside the
+ * nominal range); the synchronous calls report it as NF_EUNSUPPORTED. */
 #define NF_FLAG_NONFINITE 1u
+#define NF_FLAG_RANGE 2u
 
 /* SampleFormat (reference src/algorithms/plan.hpp:20-23). bits 8..16 integer
  * (u8 storage for 8, u16 otherwise) or 32 with floating=1. chroma selects the

@@ -884,6 +884,8 @@ nf_status run_sync(hipStream_t stream, Enqueue&& enqueue) {
   hip_check(hipMemcpyAsync(&host, flag, sizeof(unsigned), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
   hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
   if (host & NF_FLAG_NONFINITE) return set_error(NF_ENONFINITE, "non-finite sample or intermediate");
+  if (host & NF_FLAG_RANGE)
+    return set_error(NF_EUNSUPPORTED, "sample range exceeds the matrix-core form's FP16 operands (use NF_TRANSFORM_RADIX2)");
   return NF_OK;
 }
 

@@ -11,7 +11,9 @@
 // (the lo*lo term, 2^-22 relative, is dropped), accumulated in FP32. The
 // DFT-matrix constants are held at 256x scale (|c| >= 0.38 for the non-zero
 // entries, so their low parts stay normal FP16 numbers) and every product is
-// brought back with an exact 1/256 in FP32. Results
+// brought back with an exact 1/256 in FP32. The magnitudes entering the FP16
+// splits are checked: a block whose samples or spectrum would leave the FP16
+// range raises NF_FLAG_RANGE instead of returning saturated values. Results
 // are not bit-identical to the radix-2 path; their error against a binary64
 // evaluation is measured by the tests.
 //
@@ -80,6 +82,13 @@ __device__ __forceinline__ f4 unscale(f4 v) {
   return v;
 }
 
+// Largest magnitude of a split operand: the FP16 maximum finite value.
+constexpr float kHalfMax = 65504.0f;
+
+__device__ __forceinline__ float abs_max(float m, f4 v) {
+  return fmaxf(fmaxf(m, fmaxf(fabsf(v[0]), fabsf(v[1]))), fmaxf(fabsf(v[2]), fabsf(v[3])));
+}
+
 __device__ __forceinline__ Hl split_scaled(const float* v, float scale) {
   f4 t;
 #pragma unroll
@@ -105,6 +114,9 @@ __device__ __forceinline__ void mfma_tile(const DftJob& J, const DftTemporal& Q,
   constexpr int K = B / 2 + 1, OS = mfma_out_stride(B), BINS2 = B * K;
   constexpr int BQ = 16 / B, PER_UNIT = BQ * BQ;  // blocks per tile side / per tile
   constexpr float kInvB = 1.0f / (float)B;
+  // split-operand bounds: |R| <= B max|x| and |Z| <= 2 max(|Yr|, |Yi|)
+  constexpr float kInLimit = kHalfMax / (float)B, kSpecLimit = 0.5f * kHalfMax;
+  float xmax = 0.0f, ymax = 0.0f;
   extern __shared__ __align__(16) float smem[];
   const int tx = tile % J.tiles_x, ty = tile / J.tiles_x;
   const int S = J.S;
@@ -233,6 +245,7 @@ __device__ __forceinline__ void mfma_tile(const DftJob& J, const DftTemporal& Q,
               xv[e] = win[(z * B + in_y) * ww + bi * S + x] * hw[(z * B + in_y) * OS + x];
             }
           }
+          xmax = abs_max(xmax, xv);
           xs[p] = split(xv);
         }
         Hl Rr[NP], Ri[NP];
@@ -311,6 +324,7 @@ __device__ __forceinline__ void mfma_tile(const DftJob& J, const DftTemporal& Q,
             Yi[e] = res.i * tscale;
           }
         }
+        ymax = abs_max(abs_max(ymax, Yr), Yi);
         Ar[p] = split(Yr);
         Ai[p] = split(Yi);
       }
@@ -379,6 +393,7 @@ __device__ __forceinline__ void mfma_tile(const DftJob& J, const DftTemporal& Q,
     }
   }
   if (bad) atomicOr(flags, NF_FLAG_NONFINITE);
+  if (xmax > kInLimit || ymax > kSpecLimit) atomicOr(flags, NF_FLAG_RANGE);
 }
 
 }  // namespace

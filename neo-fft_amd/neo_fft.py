@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libneo_fft_amd.so")
 
 NF_OK, NF_EINVAL, NF_EUNSUPPORTED, NF_ENONFINITE, NF_ENOMEM, NF_EDEVICE = range(6)
 NF_FLAG_NONFINITE = 1
+NF_FLAG_RANGE = 2
 
 c_float_p = ctypes.POINTER(ctypes.c_float)
 c_int_p = ctypes.POINTER(ctypes.c_int)

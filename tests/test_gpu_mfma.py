@@ -82,3 +82,21 @@ def test_mfma_form_coverage(nf):
     plan = nf.fft3d_plan(64, 64, bits=8)
     with pytest.raises(nf.Unsupported):
         nf.set_transform(plan, nf.TRANSFORM_MFMA_F16X3)
+
+
+@pytest.mark.parametrize("B", [8, 16])
+def test_mfma_range_flag(nf, B):
+    """A float plane far outside its nominal range would leave the FP16
+    operands: the form reports it instead of returning saturated values;
+    the radix-2 form filters the same plane."""
+    rng = np.random.default_rng(7)
+    h, w = 64, 96
+    plane = _smooth_noisy(rng, h, w) * 4000.0
+    params = dict(tbsize=1, sbsize=B, sosize=3 * B // 4, ftype=0, sigma=8.0)
+    plan = nf.dfttest_plan(w, h, bits=32, floating=True, **params)
+    nf.set_transform(plan, nf.TRANSFORM_MFMA_F16X3)
+    with pytest.raises(nf.Unsupported, match="FP16"):
+        _run(nf, plan, [plane], np.float32)
+    plan = nf.dfttest_plan(w, h, bits=32, floating=True, **params)
+    out = _run(nf, plan, [plane], np.float32)
+    assert np.isfinite(out).all()
