@@ -32,15 +32,25 @@ template <int T>
 __device__ __forceinline__ void temporal_core(const Fft3dJob& J, const float* xr, const float* xi, float Mr, float Mi,
                                               float noise, float& out_r, float& out_i, bool& bad) {
   const float gTr = Mr * (float)T, gTi = Mi * (float)T;
+  // Twiddles with m*j = 0 are exactly (1, -0): their products x*1 - y*(-0)
+  // and x*(-0) + y*1 equal x and y up to the sign of a zero, which the
+  // running sums (started at +0, never -0 after the first addition) do not
+  // see, so those terms add x and y directly.
   float Fr[T], Fi[T];
 #pragma unroll
   for (int m = 0; m < T; ++m) {
     float sr = 0.0f, si = 0.0f;
 #pragma unroll
     for (int j = 0; j < T; ++j) {
-      const float wr = J.ftw[2 * (m * 5 + j)], wi = J.ftw[2 * (m * 5 + j) + 1];
-      const float pr = xr[j] * wr - xi[j] * wi;
-      const float pi = xr[j] * wi + xi[j] * wr;
+      float pr, pi;
+      if (m == 0 || j == 0) {
+        pr = xr[j];
+        pi = xi[j];
+      } else {
+        const float wr = J.ftw[2 * (m * 5 + j)], wi = J.ftw[2 * (m * 5 + j) + 1];
+        pr = xr[j] * wr - xi[j] * wi;
+        pi = xr[j] * wi + xi[j] * wr;
+      }
       sr = sr + pr;
       si = si + pi;
     }
@@ -64,9 +74,12 @@ __device__ __forceinline__ void temporal_core(const Fft3dJob& J, const float* xr
   float yr = 0.0f, yi = 0.0f;
 #pragma unroll
   for (int m = 0; m < T; ++m) {
-    const float wr = J.itw[2 * m], wi = J.itw[2 * m + 1];
-    const float pr = Fr[m] * wr - Fi[m] * wi;
-    const float pi = Fr[m] * wi + Fi[m] * wr;
+    float pr = Fr[0], pi = Fi[0];  // m = 0: twiddle (1, 0)
+    if (m > 0) {
+      const float wr = J.itw[2 * m], wi = J.itw[2 * m + 1];
+      pr = Fr[m] * wr - Fi[m] * wi;
+      pi = Fr[m] * wi + Fi[m] * wr;
+    }
     yr = yr + pr;
     yi = yi + pi;
   }
@@ -203,6 +216,136 @@ __device__ __forceinline__ void kalman_bin(const Fft3dJob& J, long long s, int k
 
 }  // namespace
 
+// 32-point column and row transforms on lane pairs (h = lane parity, both
+// lanes of a pair in one wave). fft32 is fft16 of the even and of the odd
+// samples followed by one twiddled butterfly stage; pass A runs the two
+// fft16s on the two lanes, pass B the butterflies (lane h: outputs k and
+// k + 16 for k = 8h..8h+7), with fft32's operation order throughout. The
+// exchange goes through LDS; a wave's LDS accesses execute in program order,
+// so pass B's reads of rows that the partner lane rewrites complete before
+// its writes (the empty asm keeps the compiler from moving them).
+template <bool FWD, int K, int BS>
+__device__ __forceinline__ void columns32_a(float* sre, float* sim, int ncols, int tid, int nthreads) {
+  for (int t = tid; t < 2 * ncols; t += nthreads) {
+    const int h = t & 1, col = t >> 1, bi = col / K, k = col - bi * K;
+    float* cr = sre + bi * BS + k;
+    float* ci = sim + bi * BS + k;
+    float re[16], im[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      re[m] = cr[(2 * m + h) * K];
+      im[m] = ci[(2 * m + h) * K];
+    }
+    fft16<FWD>(re, im);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      cr[(2 * m + h) * K] = re[m];
+      ci[(2 * m + h) * K] = im[m];
+    }
+  }
+}
+
+// twl: w32r[0..15] then w32i[0..15] in LDS. out(bi, k, row, re, im) sees
+// every output bin once.
+template <bool FWD, int K, int BS, class Out>
+__device__ __forceinline__ void columns32_b(float* sre, float* sim, const float* twl, int ncols, int tid,
+                                            int nthreads, Out&& out) {
+  for (int t = tid; t < 2 * ncols; t += nthreads) {
+    const int h = t & 1, col = t >> 1, bi = col / K, k = col - bi * K;
+    float* cr = sre + bi * BS + k;
+    float* ci = sim + bi * BS + k;
+    float er[8], ei[8], orr[8], oi[8], wr[8], wi[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int kk = 8 * h + s;
+      er[s] = cr[2 * kk * K];
+      ei[s] = ci[2 * kk * K];
+      orr[s] = cr[(2 * kk + 1) * K];
+      oi[s] = ci[(2 * kk + 1) * K];
+      wr[s] = twl[kk];
+      wi[s] = twl[16 + kk];
+    }
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int kk = 8 * h + s;
+      float vr, vi;
+      cmul<FWD>(vr, vi, orr[s], oi[s], wr[s], wi[s]);
+      const float lr = er[s] + vr, li = ei[s] + vi;
+      const float hr = er[s] - vr, hi = ei[s] - vi;
+      cr[kk * K] = lr;
+      ci[kk * K] = li;
+      cr[(kk + 16) * K] = hr;
+      ci[(kk + 16) * K] = hi;
+      out(bi, k, kk, lr, li);
+      out(bi, k, kk + 16, hr, hi);
+    }
+  }
+}
+
+// Inverse rows of 32 points (Hermitian extension of the K = 17 bins, real
+// part kept), then the 1/(BW*BH) scale, the finite check and the horizontal
+// synthesis window into the contribution rows. Pass A leaves, per row, the
+// even half's real parts in the block's sre region and the odd half's
+// twiddled real parts in its sim region (32 x 16 floats each); every read of
+// a block's spectrum happens in the same wave iteration before those writes
+// (64 lanes = 32 rows x 2 per block, the kernel's workgroup size).
+template <int K, int BS>
+__device__ __forceinline__ void inverse_rows32(const Fft3dJob& J, float* sre, float* sim, int nb,
+                                               long long block_base, int tid, bool& bad) {
+  const int h = tid & 1, y = tid >> 1;
+  for (int bi = 0; bi < nb; ++bi) {
+    const float* rr = sre + bi * BS + y * K;
+    const float* ri = sim + bi * BS + y * K;
+    float re[16], im[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      if (m < 8) {
+        re[m] = rr[2 * m + h];
+        im[m] = ri[2 * m + h];
+      } else if (m == 8) {
+        const float a = rr[16 - h], b = ri[16 - h];
+        re[m] = a;
+        im[m] = h ? 0.0f - b : b;
+      } else {
+        re[m] = rr[32 - 2 * m - h];
+        im[m] = 0.0f - ri[32 - 2 * m - h];
+      }
+    }
+    fft16<false>(re, im);
+    float val[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const float p = im[m] * Twiddle::w32i[m];
+      const float vr = fmaf(re[m], Twiddle::w32r[m], -p);
+      val[m] = h ? vr : re[m];
+    }
+    asm volatile("" ::: "memory");
+    float* dst = (h ? sim : sre) + bi * BS + y * 16;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) dst[m] = val[m];
+    __syncthreads();
+    const float* E = sre + bi * BS + y * 16 + 8 * h;
+    const float* V = sim + bi * BS + y * 16 + 8 * h;
+    const float fct = 1.0f / (32.0f * 32.0f);
+    float o[16];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const float e = E[s], v = V[s];
+      const float lo = (e + v) * fct, hi = (e - v) * fct;
+      bad |= !isfinite(lo) || !isfinite(hi);
+      o[s] = lo * J.wxs[8 * h + s];
+      o[8 + s] = hi * J.wxs[16 + 8 * h + s];
+    }
+    float4* dst4 = reinterpret_cast<float4*>(J.contrib + ((block_base + bi) * 32 + y) * 32);
+    dst4[2 * h] = make_float4(o[0], o[1], o[2], o[3]);
+    dst4[2 * h + 1] = make_float4(o[4], o[5], o[6], o[7]);
+    dst4[4 + 2 * h] = make_float4(o[8], o[9], o[10], o[11]);
+    dst4[4 + 2 * h + 1] = make_float4(o[12], o[13], o[14], o[15]);
+    __syncthreads();
+  }
+}
+
 // Stages the reflected source rows of one temporal slot in LDS and runs the
 // windowed forward row transforms ((x - base) * wy) * wx (pad.hpp:33-38,
 // scalar/spatial.cpp:12-16). Integer samples are staged raw (two or one
@@ -283,8 +426,16 @@ __global__ __launch_bounds__(64) void fft3d_block_kernel(const Fft3dLaunch L) {
   float* sre = smem;                     // [Tst][NB][BS] real parts
   float* sim = sre + Tst * NB * BS;      // [Tst][NB][BS] imaginary parts
   float* dc = sim + Tst * NB * BS;       // [NB][2]
-  void* win = dc + 2 * NB;               // [BH][ww] source samples (raw integers, or decoded floats)
+  float* twl = dc + 2 * NB;              // [32] fft32 twiddles for the lane-pair column passes
+  void* win = twl + 32;                  // [BH][ww] source samples (raw integers, or decoded floats)
   bool bad = false;
+  if constexpr (BW == 32) {
+    // read after the first barrier below
+    if (tid < 16) {
+      twl[tid] = Twiddle::w32r[tid];
+      twl[16 + tid] = Twiddle::w32i[tid];
+    }
+  }
 
   if (J.mode != 2) {
     for (int j = 0; j < nslots; ++j) {
@@ -302,33 +453,51 @@ __global__ __launch_bounds__(64) void fft3d_block_kernel(const Fft3dLaunch L) {
         forward_rows<BW, BH, BS, float>(J, j, static_cast<float*>(win), ww, oy, ox0, WW, nb, jre, jim, tid, nthreads,
                                         bad);
       __syncthreads();
-      for (int t = tid; t < nb * K; t += nthreads) {
-        const int bi = t / K, k = t - bi * K;
-        float re[BH], im[BH];
-        float* cr = jre + bi * BS + k;
-        float* ci = jim + bi * BS + k;
-#pragma unroll
-        for (int y = 0; y < BH; ++y) {
-          re[y] = cr[y * K];
-          im[y] = ci[y * K];
-        }
-        cfft<BH, true>(re, im);
-#pragma unroll
-        for (int y = 0; y < BH; ++y) {
-          cr[y * K] = re[y];
-          ci[y * K] = im[y];
-        }
-        if (J.mode == 0 && J.cache_out[j]) {
-          float* go = J.cache_out[j] + (block_base + bi) * 2 * BINS + k;
+      if constexpr (BW == 32) {
+        columns32_a<true, K, BS>(jre, jim, nb * K, tid, nthreads);
+        __syncthreads();
+        float* cache = J.mode == 0 ? J.cache_out[j] : nullptr;
+        columns32_b<true, K, BS>(jre, jim, twl, nb * K, tid, nthreads,
+                                 [&](int bi, int k, int y, float re, float im) {
+                                   if (cache) {
+                                     float* go = cache + (block_base + bi) * 2 * BINS + k;
+                                     go[y * K] = re;
+                                     go[BINS + y * K] = im;
+                                   }
+                                   if (k == 0 && j == 0 && y == 0) {
+                                     dc[2 * bi] = re;
+                                     dc[2 * bi + 1] = im;
+                                   }
+                                 });
+      } else {
+        for (int t = tid; t < nb * K; t += nthreads) {
+          const int bi = t / K, k = t - bi * K;
+          float re[BH], im[BH];
+          float* cr = jre + bi * BS + k;
+          float* ci = jim + bi * BS + k;
 #pragma unroll
           for (int y = 0; y < BH; ++y) {
-            go[y * K] = re[y];
-            go[BINS + y * K] = im[y];
+            re[y] = cr[y * K];
+            im[y] = ci[y * K];
           }
-        }
-        if (k == 0 && j == 0) {
-          dc[2 * bi] = re[0];
-          dc[2 * bi + 1] = im[0];
+          cfft<BH, true>(re, im);
+#pragma unroll
+          for (int y = 0; y < BH; ++y) {
+            cr[y * K] = re[y];
+            ci[y * K] = im[y];
+          }
+          if (J.mode == 0 && J.cache_out[j]) {
+            float* go = J.cache_out[j] + (block_base + bi) * 2 * BINS + k;
+#pragma unroll
+            for (int y = 0; y < BH; ++y) {
+              go[y * K] = re[y];
+              go[BINS + y * K] = im[y];
+            }
+          }
+          if (k == 0 && j == 0) {
+            dc[2 * bi] = re[0];
+            dc[2 * bi + 1] = im[0];
+          }
         }
       }
       __syncthreads();
@@ -429,6 +598,15 @@ __global__ __launch_bounds__(64) void fft3d_block_kernel(const Fft3dLaunch L) {
       *ci = im;
     }
     __syncthreads();
+  }
+  if constexpr (BW == 32) {
+    columns32_a<false, K, BS>(sre, sim, nb * K, tid, nthreads);
+    __syncthreads();
+    columns32_b<false, K, BS>(sre, sim, twl, nb * K, tid, nthreads, [](int, int, int, float, float) {});
+    __syncthreads();
+    inverse_rows32<K, BS>(J, sre, sim, nb, block_base, tid, bad);
+    if (bad) atomicOr(L.flags, NF_FLAG_NONFINITE);
+    return;
   }
   for (int t = tid; t < nb * K; t += nthreads) {
     const int bi = t / K, k = t - bi * K;
@@ -790,7 +968,7 @@ long long fft3d_synthesis_units(const Fft3dJob& J) {
 size_t fft3d_block_lds_bytes(int BW, int BH, int T, int NB, int Sx, int storage) {
   const int ww = odd_stride((NB - 1) * Sx + BW);
   const size_t sample = storage == kU8 ? 1 : storage == kU16 ? 2 : 4;
-  return sizeof(float) * ((size_t)T * NB * spec_block_stride(BW) * 2 + 2 * (size_t)NB) + sample * BH * ww;
+  return sizeof(float) * ((size_t)T * NB * spec_block_stride(BW) * 2 + 2 * (size_t)NB + 32) + sample * BH * ww;
 }
 
 hipError_t launch_fft3d_block(int BW, int BH, const Fft3dLaunch& L, int total_groups, size_t lds, hipStream_t stream) {

@@ -124,11 +124,12 @@ def test_fft3d_end_of_clip_fallback(nf, oracle):
                                 dict(beta=1.5, degrid=0.0),
                                 dict(wintype=1), dict(wintype=2)])
 @pytest.mark.parametrize("bt", [1, 3])
-def test_fft3d_models(nf, oracle, kw, bt):
-    rng = np.random.default_rng(len(str(kw)) + bt)
+@pytest.mark.parametrize("bw", [16, 32])
+def test_fft3d_models(nf, oracle, kw, bt, bw):
+    rng = np.random.default_rng(len(str(kw)) + bt + bw)
     h, w = 80, 96
     frames = [_rand_plane(rng, h, w, 8, False) for _ in range(max(1, bt))]
-    params = dict(bw=16, bh=16, bt=bt, sigma=4.0, **kw)
+    params = dict(bw=bw, bh=bw, bt=bt, sigma=4.0, **kw)
     plan = nf.fft3d_plan(w, h, bits=8, **params)
     ref = oracle.fft3d_plane(frames, bits=8, **params)
     got = _run(plan, frames)
@@ -164,10 +165,11 @@ def test_fft3d_c2_shape_yuv420p16(nf, oracle):
 
 
 @pytest.mark.parametrize("kw", [dict(), dict(sigma2=1.0, sigma3=3.0, sigma4=5.0), dict(sharpen=0.3)])
-def test_kalman_sequence(nf, oracle, kw):
-    rng = np.random.default_rng(17)
+@pytest.mark.parametrize("bw", [16, 32])
+def test_kalman_sequence(nf, oracle, kw, bw):
+    rng = np.random.default_rng(17 + bw)
     h, w = 72, 100
-    params = dict(bw=16, bh=16, bt=0, sigma=3.0, kratio=2.0, **kw)
+    params = dict(bw=bw, bh=bw, bt=0, sigma=3.0, kratio=2.0, **kw)
     base = _rand_plane(rng, h, w, 8, False).astype(np.int32)
     frames = [np.clip(base + rng.integers(-6, 7, size=(h, w)), 0, 255).astype(np.uint8) for _ in range(6)]
     plan = nf.fft3d_plan(w, h, bits=8, **params)
