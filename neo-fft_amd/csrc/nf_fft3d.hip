@@ -346,14 +346,26 @@ __device__ __forceinline__ void inverse_rows32(const Fft3dJob& J, float* sre, fl
   }
 }
 
+// Value of the other lane of this lane's pair (lanes 2i, 2i+1).
+__device__ __forceinline__ float swap_pair(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, false));
+}
+
 // Stages the reflected source rows of one temporal slot in LDS and runs the
 // windowed forward row transforms ((x - base) * wy) * wx (pad.hpp:33-38,
 // scalar/spatial.cpp:12-16). Integer samples are staged raw (two or one
 // bytes) and decoded when read, which is exact and keeps the LDS footprint
 // small; float samples are staged decoded.
+//
+// 32-point rows run on lane pairs: lane h transforms the even (h = 0) or odd
+// (h = 1) samples with fft16, the pair swaps halves across lanes (DPP quad
+// permutation 1,0,3,2) and lane h forms bins 8h..8h+7 (lane 0 also bin 16)
+// with fft32's twiddled butterflies. twl holds w32r/w32i, wxl the
+// horizontal analysis window de-interleaved (even x, then odd x).
 template <int BW, int BH, int BS, class Sample>
 __device__ __forceinline__ void forward_rows(const Fft3dJob& J, int j, Sample* win, int ww, int oy, int ox0, int WW,
-                                             int nb, float* jre, float* jim, int tid, int nthreads, bool& bad) {
+                                             int nb, float* jre, float* jim, const float* twl, const float* wxl,
+                                             int tid, int nthreads, bool& bad) {
   constexpr int K = BW / 2 + 1;
   const int Sx = J.gx.step;
   for (int idx = tid; idx < BH * WW; idx += nthreads) {
@@ -370,6 +382,41 @@ __device__ __forceinline__ void forward_rows(const Fft3dJob& J, int j, Sample* w
     }
   }
   __syncthreads();
+  if constexpr (BW == 32) {
+    for (int t = tid; t < 2 * nb * BH; t += nthreads) {
+      const int h = t & 1, row = t >> 1, bi = row / BH, y = row - bi * BH;
+      const Sample* src = win + y * ww + bi * Sx + h;
+      const float wy = J.wya[y];
+      float re[16], im[16];
+#pragma unroll
+      for (int m = 0; m < 16; ++m) {
+        const float v = sizeof(Sample) == 4 ? (float)src[2 * m] : (float)src[2 * m] - J.base;
+        re[m] = (v * wy) * wxl[16 * h + m];
+        im[m] = 0.0f;
+      }
+      fft16<true>(re, im);
+      float* dr = jre + bi * BS + y * K;
+      float* di = jim + bi * BS + y * K;
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        // lane 0 keeps E[s] and receives O[s]; lane 1 keeps O[8+s] and receives E[8+s]
+        const float sr = h ? re[s] : re[8 + s], si = h ? im[s] : im[8 + s];
+        const float xr = swap_pair(sr), xi = swap_pair(si);
+        const float er = h ? xr : re[s], ei = h ? xi : im[s];
+        const float orr = h ? re[8 + s] : xr, oi = h ? im[8 + s] : xi;
+        const int kk = 8 * h + s;
+        float vr, vi;
+        cmul<true>(vr, vi, orr, oi, twl[kk], twl[16 + kk]);
+        dr[kk] = er + vr;
+        di[kk] = ei + vi;
+        if (s == 0 && h == 0) {
+          dr[16] = er - vr;
+          di[16] = 0.0f;
+        }
+      }
+    }
+    return;
+  }
   for (int t = tid; t < nb * BH; t += nthreads) {
     const int bi = t / BH, y = t - bi * BH;
     float re[BW], im[BW];
@@ -426,8 +473,9 @@ __global__ __launch_bounds__(64) void fft3d_block_kernel(const Fft3dLaunch L) {
   float* sre = smem;                     // [Tst][NB][BS] real parts
   float* sim = sre + Tst * NB * BS;      // [Tst][NB][BS] imaginary parts
   float* dc = sim + Tst * NB * BS;       // [NB][2]
-  float* twl = dc + 2 * NB;              // [32] fft32 twiddles for the lane-pair column passes
-  void* win = twl + 32;                  // [BH][ww] source samples (raw integers, or decoded floats)
+  float* twl = dc + 2 * NB;              // [32] fft32 twiddles for the lane-pair passes
+  float* wxl = twl + 32;                 // [32] analysis window x, even then odd positions
+  void* win = wxl + 32;                  // [BH][ww] source samples (raw integers, or decoded floats)
   bool bad = false;
   if constexpr (BW == 32) {
     // read after the first barrier below
@@ -435,6 +483,7 @@ __global__ __launch_bounds__(64) void fft3d_block_kernel(const Fft3dLaunch L) {
       twl[tid] = Twiddle::w32r[tid];
       twl[16 + tid] = Twiddle::w32i[tid];
     }
+    if (tid < 32) wxl[16 * (tid & 1) + (tid >> 1)] = J.wxa[tid];
   }
 
   if (J.mode != 2) {
@@ -444,13 +493,13 @@ __global__ __launch_bounds__(64) void fft3d_block_kernel(const Fft3dLaunch L) {
       float* jim = sim + lidx[j] * NB * BS;
       // reflected source rows, then windowed rows -> forward row transforms
       if (J.storage == kU16)
-        forward_rows<BW, BH, BS, uint16_t>(J, j, static_cast<uint16_t*>(win), ww, oy, ox0, WW, nb, jre, jim, tid,
+        forward_rows<BW, BH, BS, uint16_t>(J, j, static_cast<uint16_t*>(win), ww, oy, ox0, WW, nb, jre, jim, twl, wxl, tid,
                                            nthreads, bad);
       else if (J.storage == kU8)
-        forward_rows<BW, BH, BS, uint8_t>(J, j, static_cast<uint8_t*>(win), ww, oy, ox0, WW, nb, jre, jim, tid,
+        forward_rows<BW, BH, BS, uint8_t>(J, j, static_cast<uint8_t*>(win), ww, oy, ox0, WW, nb, jre, jim, twl, wxl, tid,
                                           nthreads, bad);
       else
-        forward_rows<BW, BH, BS, float>(J, j, static_cast<float*>(win), ww, oy, ox0, WW, nb, jre, jim, tid, nthreads,
+        forward_rows<BW, BH, BS, float>(J, j, static_cast<float*>(win), ww, oy, ox0, WW, nb, jre, jim, twl, wxl, tid, nthreads,
                                         bad);
       __syncthreads();
       if constexpr (BW == 32) {
@@ -968,7 +1017,7 @@ long long fft3d_synthesis_units(const Fft3dJob& J) {
 size_t fft3d_block_lds_bytes(int BW, int BH, int T, int NB, int Sx, int storage) {
   const int ww = odd_stride((NB - 1) * Sx + BW);
   const size_t sample = storage == kU8 ? 1 : storage == kU16 ? 2 : 4;
-  return sizeof(float) * ((size_t)T * NB * spec_block_stride(BW) * 2 + 2 * (size_t)NB + 32) + sample * BH * ww;
+  return sizeof(float) * ((size_t)T * NB * spec_block_stride(BW) * 2 + 2 * (size_t)NB + 64) + sample * BH * ww;
 }
 
 hipError_t launch_fft3d_block(int BW, int BH, const Fft3dLaunch& L, int total_groups, size_t lds, hipStream_t stream) {
