@@ -368,17 +368,24 @@ __device__ __forceinline__ void forward_rows(const Fft3dJob& J, int j, Sample* w
                                              int tid, int nthreads, bool& bad) {
   constexpr int K = BW / 2 + 1;
   const int Sx = J.gx.step;
-  for (int idx = tid; idx < BH * WW; idx += nthreads) {
-    const int r = idx / WW, c = idx - r * WW;
-    const int sy = reflect(oy + r, J.gy), sx = reflect(ox0 + c, J.gx);
-    const char* row = static_cast<const char*>(J.src[j]) + (long long)sy * J.src_stride[j];
-    const Sample raw = reinterpret_cast<const Sample*>(row)[sx];
-    if constexpr (sizeof(Sample) == 4) {
-      const float v = raw - J.base;
-      bad |= !isfinite(v);
-      win[r * ww + c] = v;
-    } else {
-      win[r * ww + c] = raw;
+  // lanes own window columns (the column reflection is computed once) and
+  // step over rows: R rows at a time when a row is narrower than the group
+  const int R = nthreads >= WW ? nthreads / WW : 1;
+  const int r0 = R > 1 ? tid / WW : 0;
+  const char* base = static_cast<const char*>(J.src[j]);
+  const long long pitch = J.src_stride[j];
+  for (int c = (R > 1 ? tid - r0 * WW : tid); c < WW && r0 < R; c += nthreads) {
+    const int sx = reflect(ox0 + c, J.gx);
+    for (int r = r0; r < BH; r += R) {
+      const int sy = reflect(oy + r, J.gy);
+      const Sample raw = reinterpret_cast<const Sample*>(base + (long long)sy * pitch)[sx];
+      if constexpr (sizeof(Sample) == 4) {
+        const float v = raw - J.base;
+        bad |= !isfinite(v);
+        win[r * ww + c] = v;
+      } else {
+        win[r * ww + c] = raw;
+      }
     }
   }
   __syncthreads();
