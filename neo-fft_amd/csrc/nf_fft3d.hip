@@ -974,17 +974,17 @@ __global__ __launch_bounds__(256) void fft3d_synthesis_kernel(const Fft3dLaunch 
   while (jid + 1 < L.njobs && gid >= unit_start[jid + 1]) ++jid;
   if (gid >= unit_start[L.njobs]) return;
   const Fft3dJob& J = L.jobs[jid];
-  const long long u = gid - unit_start[jid];
+  const int u = (int)(gid - unit_start[jid]);  // units per job < 2^31
   const int w = J.gx.length;
   if (!synthesis_vector(J)) {
-    const int oy = (int)(u / w), ox = (int)(u - (long long)oy * w);
+    const int oy = u / w, ox = u - oy * w;
     const float acc = synthesize_pixel(J, ox + J.gx.offset, oy + J.gy.offset);
     if (!isfinite(acc)) atomicOr(L.flags, NF_FLAG_NONFINITE);
     store_fft3d_output(J, oy, ox, acc);
     return;
   }
   const int gpr = synthesis_groups(J);
-  const int oy = (int)(u / gpr), gi = (int)(u - (long long)oy * gpr);
+  const int oy = u / gpr, gi = u - oy * gpr;
   const int x0 = (J.gx.offset / 4 + gi) * 4;
   const int y = oy + J.gy.offset;
   const int BW = J.gx.block, BH = J.gy.block, Sx = J.gx.step, Sy = J.gy.step;
@@ -1007,12 +1007,39 @@ __global__ __launch_bounds__(256) void fft3d_synthesis_kernel(const Fft3dLaunch 
     for (int e = 0; e < 4; ++e) acc[e] = acc[e] + r[e] * wy;
   }
   bool bad = false;
+  const int ox0 = x0 - J.gx.offset;
+  char* drow = static_cast<char*>(J.dst) + (long long)oy * J.dst_stride;
+  const size_t bytes = J.storage == kF32 ? 4 : J.storage == kU16 ? 2 : 1;
+  if (ox0 >= 0 && ox0 + 3 < w && (reinterpret_cast<size_t>(drow + ox0 * bytes) & (4 * bytes - 1)) == 0) {
+    // four pixels of one row, one aligned vector store (same conversions)
 #pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    const int ox = x0 + e - J.gx.offset;
-    if (ox < 0 || ox >= w) continue;
-    bad |= !isfinite(acc[e]);
-    store_fft3d_output(J, oy, ox, acc[e]);
+    for (int e = 0; e < 4; ++e) bad |= !isfinite(acc[e]);
+    if (J.storage == kF32) {
+      float o[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) o[e] = acc[e] < 0.0f ? 0.0f : (1.0f < acc[e] ? 1.0f : acc[e]);
+      *reinterpret_cast<float4*>(drow + ox0 * 4) = make_float4(o[0], o[1], o[2], o[3]);
+    } else {
+      unsigned q[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float v = (acc[e] + 0.5f) + J.base;
+        const float cl = v < 0.0f ? 0.0f : (J.peak < v ? J.peak : v);
+        q[e] = J.storage == kU8 ? (unsigned)(uint8_t)cl : (unsigned)(uint16_t)cl;
+      }
+      if (J.storage == kU16)
+        *reinterpret_cast<uint2*>(drow + ox0 * 2) = make_uint2(q[0] | (q[1] << 16), q[2] | (q[3] << 16));
+      else
+        *reinterpret_cast<unsigned*>(drow + ox0) = q[0] | (q[1] << 8) | (q[2] << 16) | (q[3] << 24);
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int ox = ox0 + e;
+      if (ox < 0 || ox >= w) continue;
+      bad |= !isfinite(acc[e]);
+      store_fft3d_output(J, oy, ox, acc[e]);
+    }
   }
   if (bad) atomicOr(L.flags, NF_FLAG_NONFINITE);
 }
