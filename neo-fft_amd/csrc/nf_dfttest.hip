@@ -71,13 +71,19 @@ __device__ __forceinline__ void t1_tile(const DftJob& J, int tile, unsigned* fla
 
   for (int by = by_lo; by <= by_hi; ++by) {
     const int oy = by * S;
-    // A: decoded, reflected source window [B][WW]
-    for (int idx = tid; idx < B * WW; idx += nthreads) {
-      const int r = idx / WW, c = idx - r * WW;
-      const int sy = reflect(oy + r, J.gy), sx = reflect(cx0 + c, J.gx);
-      const float v = load_sample(J.src, J.src_stride, J.storage, sy, sx) * J.dscale;
-      bad |= !isfinite(v);
-      win[r * J.ww_max + c] = v;
+    // A: decoded, reflected source window [B][WW]; lanes own columns
+    {
+      const int R = nthreads >= WW ? nthreads / WW : 1;
+      const int rl = R > 1 ? tid / WW : 0;
+      for (int c = (R > 1 ? tid - rl * WW : tid); c < WW && rl < R; c += nthreads) {
+        const int sx = reflect(cx0 + c, J.gx);
+        for (int r = rl; r < B; r += R) {
+          const int sy = reflect(oy + r, J.gy);
+          const float v = load_sample(J.src, J.src_stride, J.storage, sy, sx) * J.dscale;
+          bad |= !isfinite(v);
+          win[r * J.ww_max + c] = v;
+        }
+      }
     }
     __syncthreads();
     // B: windowed rows -> forward row transforms, bins 0..B/2
@@ -178,25 +184,30 @@ __device__ __forceinline__ void t1_tile(const DftJob& J, int tile, unsigned* fla
       }
     }
     __syncthreads();
-    // F: overlap-add into the ring, block columns in ascending order
+    // F: overlap-add into the ring, block columns in ascending order; lanes
+    // own output columns (their covering block range is computed once)
+    const int RT = nthreads >= TW ? nthreads / TW : 1;
+    const int rlt = RT > 1 ? tid / TW : 0;
+    const int c_first = RT > 1 ? tid - rlt * TW : tid;
     {
       const int r0 = max(oy, Y0), r1 = min(oy + B, Y1);
       const int nr = r1 - r0;
-      for (int idx = tid; idx < nr * TW; idx += nthreads) {
-        const int rr = idx / TW;
-        const int x = X0 + (idx - rr * TW);
-        const int y = r0 + rr;
-        float* cell = ring + (y % B) * J.tile_w + (x - X0);
-        float sum = *cell;
+      for (int c = c_first; c < TW && rlt < RT; c += nthreads) {
+        const int x = X0 + c;
         const int b_first = max(bx_lo, ceil_div_int(x - B + 1, S));
         const int b_last = min(bx_hi, x / S);
-        for (int bx = b_first; bx <= b_last; ++bx) {
-          const int bi = bx - bx_lo;
-          const int xx = x - bx * S;
-          const int row = bi * BS + (y - oy) * KS;
-          sum = sum + (xx < KS ? sre[row + xx] : sim[row + xx - KS]);
+        for (int rr = rlt; rr < nr; rr += RT) {
+          const int y = r0 + rr;
+          float* cell = ring + (y % B) * J.tile_w + c;
+          float sum = *cell;
+          for (int bx = b_first; bx <= b_last; ++bx) {
+            const int bi = bx - bx_lo;
+            const int xx = x - bx * S;
+            const int row = bi * BS + (y - oy) * KS;
+            sum = sum + (xx < KS ? sre[row + xx] : sim[row + xx - KS]);
+          }
+          *cell = sum;
         }
-        *cell = sum;
       }
     }
     __syncthreads();
@@ -204,26 +215,27 @@ __device__ __forceinline__ void t1_tile(const DftJob& J, int tile, unsigned* fla
     {
       const int r0 = max(oy, Y0), r1 = min(oy + S, Y1);
       const int nr = r1 - r0;
-      for (int idx = tid; idx < nr * TW; idx += nthreads) {
-        const int rr = idx / TW;
-        const int x = X0 + (idx - rr * TW);
-        const int y = r0 + rr;
-        float* cell = ring + (y % B) * J.tile_w + (x - X0);
-        const float a = *cell;
-        *cell = 0.0f;
-        bad |= !isfinite(a);
-        const int oyy = y - vy0, oxx = x - vx0;
-        if (J.acc_out) {
-          J.acc_out[(long long)oyy * J.acc_stride + oxx] = a;
-        } else {
-          char* drow = static_cast<char*>(J.dst) + (long long)oyy * J.dst_stride;
-          if (J.storage == kF32) {
-            reinterpret_cast<float*>(drow)[oxx] = a * J.oscale;
+      for (int c = c_first; c < TW && rlt < RT; c += nthreads) {
+        const int x = X0 + c, oxx = x - vx0;
+        for (int rr = rlt; rr < nr; rr += RT) {
+          const int y = r0 + rr;
+          float* cell = ring + (y % B) * J.tile_w + c;
+          const float a = *cell;
+          *cell = 0.0f;
+          bad |= !isfinite(a);
+          const int oyy = y - vy0;
+          if (J.acc_out) {
+            J.acc_out[(long long)oyy * J.acc_stride + oxx] = a;
           } else {
-            const float v = (a * J.oscale) + 0.5f;
-            const float cl = v < 0.0f ? 0.0f : (J.peak < v ? J.peak : v);
-            if (J.storage == kU8) reinterpret_cast<uint8_t*>(drow)[oxx] = (uint8_t)cl;
-            else reinterpret_cast<uint16_t*>(drow)[oxx] = (uint16_t)cl;
+            char* drow = static_cast<char*>(J.dst) + (long long)oyy * J.dst_stride;
+            if (J.storage == kF32) {
+              reinterpret_cast<float*>(drow)[oxx] = a * J.oscale;
+            } else {
+              const float v = (a * J.oscale) + 0.5f;
+              const float cl = v < 0.0f ? 0.0f : (J.peak < v ? J.peak : v);
+              if (J.storage == kU8) reinterpret_cast<uint8_t*>(drow)[oxx] = (uint8_t)cl;
+              else reinterpret_cast<uint16_t*>(drow)[oxx] = (uint16_t)cl;
+            }
           }
         }
       }
