@@ -202,15 +202,21 @@ __device__ __forceinline__ void vol_tile(const DftJob& J, const DftTemporal& Q, 
         const int WWc = (nb - 1) * S + B;
         __syncthreads();
         // A: all T slots' source rows of this chunk
-        for (int idx = tid; idx < T * B * WWc; idx += nthreads) {
-          const int z = idx / (B * WWc);
-          const int rem = idx - z * (B * WWc);
-          const int r = rem / WWc, c = rem - r * WWc;
-          const int f = Q.slot_frame[tb * T + z];
-          const int sy = reflect(oy + r, J.gy), sx = reflect(cx0 + c, J.gx);
-          const float v = load_sample(Q.frames[f], Q.frame_stride[f], J.storage, sy, sx) * J.dscale;
-          bad |= !isfinite(v);
-          win[(z * B + r) * ww_c + c] = v;
+        {
+          // lanes own window columns and step over (slot, row)
+          const int R = nthreads >= WWc ? nthreads / WWc : 1;
+          const int rl = R > 1 ? tid / WWc : 0;
+          for (int c = (R > 1 ? tid - rl * WWc : tid); c < WWc && rl < R; c += nthreads) {
+            const int sx = reflect(cx0 + c, J.gx);
+            for (int zr = rl; zr < T * B; zr += R) {
+              const int z = zr / B, r = zr - z * B;
+              const int f = Q.slot_frame[tb * T + z];
+              const int sy = reflect(oy + r, J.gy);
+              const float v = load_sample(Q.frames[f], Q.frame_stride[f], J.storage, sy, sx) * J.dscale;
+              bad |= !isfinite(v);
+              win[zr * ww_c + c] = v;
+            }
+          }
         }
         __syncthreads();
         // B: windowed rows of every slice
@@ -330,20 +336,23 @@ __device__ __forceinline__ void vol_tile(const DftJob& J, const DftTemporal& Q, 
             const int xa = max(X0, cx0), xb = min(X1, cx0 + WWc);
             const int nr = r1 - r0, ncol = xb - xa;
             float* A = acc + o * acc_rows * J.tile_w;
-            for (int idx = tid; idx < nr * ncol; idx += nthreads) {
-              const int rr = idx / ncol;
-              const int x = xa + (idx - rr * ncol);
-              const int y = r0 + rr;
-              float* cell = A + (ring_mode ? (y % B) : (y - Y0)) * J.tile_w + (x - X0);
-              float sum = *cell;
+            const int R = nthreads >= ncol ? nthreads / ncol : 1;
+            const int rl = R > 1 ? tid / ncol : 0;
+            for (int c = (R > 1 ? tid - rl * ncol : tid); c < ncol && rl < R; c += nthreads) {
+              const int x = xa + c;
               const int b_first = max(cb, ceil_div_int(x - B + 1, S));
               const int b_last = min(cb + nb - 1, x / S);
-              for (int bx = b_first; bx <= b_last; ++bx) {
-                const int xx = x - bx * S;
-                const int row = ((bx - cb) * T + zs) * BS + (y - oy) * KS;
-                sum = sum + (xx < KS ? sre[row + xx] : sim[row + xx - KS]);
+              for (int rr = rl; rr < nr; rr += R) {
+                const int y = r0 + rr;
+                float* cell = A + (ring_mode ? (y % B) : (y - Y0)) * J.tile_w + (x - X0);
+                float sum = *cell;
+                for (int bx = b_first; bx <= b_last; ++bx) {
+                  const int xx = x - bx * S;
+                  const int row = ((bx - cb) * T + zs) * BS + (y - oy) * KS;
+                  sum = sum + (xx < KS ? sre[row + xx] : sim[row + xx - KS]);
+                }
+                *cell = sum;
               }
-              *cell = sum;
             }
           }
         }
@@ -356,17 +365,20 @@ __device__ __forceinline__ void vol_tile(const DftJob& J, const DftTemporal& Q, 
         for (int o = 0; o < nout; ++o) {
           void* dst = Q.emit_all ? Q.dsts[o] : J.dst;
           float* A = acc + o * acc_rows * J.tile_w;
-          for (int idx = tid; idx < nr * TW; idx += nthreads) {
-            const int rr = idx / TW;
-            const int x = X0 + (idx - rr * TW);
-            const int y = r0 + rr;
-            float* cell = A + (y % B) * J.tile_w + (x - X0);
-            const float a = *cell;
-            *cell = 0.0f;
-            if (dst == nullptr) continue;
-            bad |= !isfinite(a);
-            if (J.acc_out) J.acc_out[(long long)(y - vy0) * J.acc_stride + (x - vx0)] = a;
-            else store_dft_output(J, dst, y - vy0, x - vx0, a);
+          const int R = nthreads >= TW ? nthreads / TW : 1;
+          const int rl = R > 1 ? tid / TW : 0;
+          for (int c = (R > 1 ? tid - rl * TW : tid); c < TW && rl < R; c += nthreads) {
+            const int x = X0 + c;
+            for (int rr = rl; rr < nr; rr += R) {
+              const int y = r0 + rr;
+              float* cell = A + (y % B) * J.tile_w + c;
+              const float a = *cell;
+              *cell = 0.0f;
+              if (dst == nullptr) continue;
+              bad |= !isfinite(a);
+              if (J.acc_out) J.acc_out[(long long)(y - vy0) * J.acc_stride + (x - vx0)] = a;
+              else store_dft_output(J, dst, y - vy0, x - vx0, a);
+            }
           }
         }
       }

@@ -204,16 +204,21 @@ __device__ __forceinline__ void mfma_tile(const DftJob& J, const DftTemporal& Q,
 
   for (int by = by_lo; by <= by_hi; ++by) {
     const int oy = by * S;
-    // A: decoded, reflected source rows of every slot
-    for (int idx = tid; idx < TT * B * WW; idx += nthreads) {
-      const int z = idx / (B * WW);
-      const int rem = idx - z * (B * WW);
-      const int rr = rem / WW, c = rem - rr * WW;
-      const int f = Q.slot_frame[z];
-      const int sy = reflect(oy + rr, J.gy), sx = reflect(cx0 + c, J.gx);
-      const float v = load_sample(Q.frames[f], Q.frame_stride[f], J.storage, sy, sx) * J.dscale;
-      bad |= !isfinite(v);
-      win[(z * B + rr) * ww + c] = v;
+    // A: decoded, reflected source rows of every slot; lanes own columns
+    {
+      const int R = nthreads >= WW ? nthreads / WW : 1;
+      const int rl = R > 1 ? tid / WW : 0;
+      for (int c = (R > 1 ? tid - rl * WW : tid); c < WW && rl < R; c += nthreads) {
+        const int sx = reflect(cx0 + c, J.gx);
+        for (int zr = rl; zr < TT * B; zr += R) {
+          const int z = zr / B, rr = zr - z * B;
+          const int f = Q.slot_frame[z];
+          const int sy = reflect(oy + rr, J.gy);
+          const float v = load_sample(Q.frames[f], Q.frame_stride[f], J.storage, sy, sx) * J.dscale;
+          bad |= !isfinite(v);
+          win[zr * ww + c] = v;
+        }
+      }
     }
     __syncthreads();
     // M: each wave transforms NP tiles (of PER_UNIT blocks) at a time
@@ -349,21 +354,26 @@ __device__ __forceinline__ void mfma_tile(const DftJob& J, const DftTemporal& Q,
       }
     }
     __syncthreads();
-    // F: overlap-add into the ring, block columns in ascending order
+    // F: overlap-add into the ring, block columns in ascending order; lanes
+    // own output columns
+    const int RT = nthreads >= TW ? nthreads / TW : 1;
+    const int rlt = RT > 1 ? tid / TW : 0;
+    const int c_first = RT > 1 ? tid - rlt * TW : tid;
     {
       const int r0 = max(oy, Y0), r1 = min(oy + B, Y1);
       const int nr = r1 - r0;
-      for (int idx = tid; idx < nr * TW; idx += nthreads) {
-        const int rr = idx / TW;
-        const int x = X0 + (idx - rr * TW);
-        const int y = r0 + rr;
-        float* cell = ring + (y % B) * J.tile_w + (x - X0);
-        float sum = *cell;
+      for (int c = c_first; c < TW && rlt < RT; c += nthreads) {
+        const int x = X0 + c;
         const int b_first = max(bx_lo, ceil_div_int(x - B + 1, S));
         const int b_last = min(bx_hi, x / S);
-        for (int bx = b_first; bx <= b_last; ++bx)
-          sum = sum + out[((bx - bx_lo) * B + (y - oy)) * OS + (x - bx * S)];
-        *cell = sum;
+        for (int rr = rlt; rr < nr; rr += RT) {
+          const int y = r0 + rr;
+          float* cell = ring + (y % B) * J.tile_w + c;
+          float sum = *cell;
+          for (int bx = b_first; bx <= b_last; ++bx)
+            sum = sum + out[((bx - bx_lo) * B + (y - oy)) * OS + (x - bx * S)];
+          *cell = sum;
+        }
       }
     }
     __syncthreads();
@@ -371,23 +381,24 @@ __device__ __forceinline__ void mfma_tile(const DftJob& J, const DftTemporal& Q,
     {
       const int r0 = max(oy, Y0), r1 = min(oy + S, Y1);
       const int nr = r1 - r0;
-      for (int idx = tid; idx < nr * TW; idx += nthreads) {
-        const int rr = idx / TW;
-        const int x = X0 + (idx - rr * TW);
-        const int y = r0 + rr;
-        float* cell = ring + (y % B) * J.tile_w + (x - X0);
-        const float a = *cell;
-        *cell = 0.0f;
-        bad |= !isfinite(a);
-        const int oyy = y - vy0, oxx = x - vx0;
-        char* drow = static_cast<char*>(J.dst) + (long long)oyy * J.dst_stride;
-        if (J.storage == kF32) {
-          reinterpret_cast<float*>(drow)[oxx] = a * J.oscale;
-        } else {
-          const float v = (a * J.oscale) + 0.5f;
-          const float cl = v < 0.0f ? 0.0f : (J.peak < v ? J.peak : v);
-          if (J.storage == kU8) reinterpret_cast<uint8_t*>(drow)[oxx] = (uint8_t)cl;
-          else reinterpret_cast<uint16_t*>(drow)[oxx] = (uint16_t)cl;
+      for (int c = c_first; c < TW && rlt < RT; c += nthreads) {
+        const int x = X0 + c, oxx = x - vx0;
+        for (int rr = rlt; rr < nr; rr += RT) {
+          const int y = r0 + rr;
+          float* cell = ring + (y % B) * J.tile_w + c;
+          const float a = *cell;
+          *cell = 0.0f;
+          bad |= !isfinite(a);
+          const int oyy = y - vy0;
+          char* drow = static_cast<char*>(J.dst) + (long long)oyy * J.dst_stride;
+          if (J.storage == kF32) {
+            reinterpret_cast<float*>(drow)[oxx] = a * J.oscale;
+          } else {
+            const float v = (a * J.oscale) + 0.5f;
+            const float cl = v < 0.0f ? 0.0f : (J.peak < v ? J.peak : v);
+            if (J.storage == kU8) reinterpret_cast<uint8_t*>(drow)[oxx] = (uint8_t)cl;
+            else reinterpret_cast<uint16_t*>(drow)[oxx] = (uint16_t)cl;
+          }
         }
       }
     }
