@@ -287,12 +287,15 @@ __device__ __forceinline__ void columns32_b(float* sre, float* sim, const float*
 // part kept), then the 1/(BW*BH) scale, the finite check and the horizontal
 // synthesis window into the contribution rows. Pass A leaves, per row, the
 // even half's real parts in the block's sre region and the odd half's
-// twiddled real parts in its sim region (32 x 16 floats each); every read of
-// a block's spectrum happens in the same wave iteration before those writes
-// (64 lanes = 32 rows x 2 per block, the kernel's workgroup size).
+// twiddled real parts in its sim region (16 x 32 floats each, element (m, y)
+// at m*33 + y, plus 8 for m >= 8, so that both passes hit distinct banks);
+// every read of a block's spectrum happens in the same wave iteration before
+// those writes (64 lanes = 32 rows x 2 per block, the kernel's workgroup
+// size).
 template <int K, int BS>
 __device__ __forceinline__ void inverse_rows32(const Fft3dJob& J, float* sre, float* sim, int nb,
                                                long long block_base, int tid, bool& bad) {
+  static_assert(15 * 33 + 8 + 31 < BS, "the exchange fits one block's spectrum region");
   const int h = tid & 1, y = tid >> 1;
   for (int bi = 0; bi < nb; ++bi) {
     const float* rr = sre + bi * BS + y * K;
@@ -321,17 +324,17 @@ __device__ __forceinline__ void inverse_rows32(const Fft3dJob& J, float* sre, fl
       val[m] = h ? vr : re[m];
     }
     asm volatile("" ::: "memory");
-    float* dst = (h ? sim : sre) + bi * BS + y * 16;
+    float* dst = (h ? sim : sre) + bi * BS + y;
 #pragma unroll
-    for (int m = 0; m < 16; ++m) dst[m] = val[m];
+    for (int m = 0; m < 16; ++m) dst[m * 33 + (m >= 8 ? 8 : 0)] = val[m];
     __syncthreads();
-    const float* E = sre + bi * BS + y * 16 + 8 * h;
-    const float* V = sim + bi * BS + y * 16 + 8 * h;
+    const float* E = sre + bi * BS + y + h * (8 * 33 + 8);
+    const float* V = sim + bi * BS + y + h * (8 * 33 + 8);
     const float fct = 1.0f / (32.0f * 32.0f);
     float o[16];
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-      const float e = E[s], v = V[s];
+      const float e = E[s * 33], v = V[s * 33];
       const float lo = (e + v) * fct, hi = (e - v) * fct;
       bad |= !isfinite(lo) || !isfinite(hi);
       o[s] = lo * J.wxs[8 * h + s];
