@@ -138,51 +138,51 @@ __device__ __forceinline__ void temporal_pass(const Fft3dJob& J, float* sre, flo
 }
 
 // The same pass in the sequential steady state of the spectrum cache: slots
-// 0..T-2 cached, slot T-1 (the newest frame) in LDS slot 0. The per-block
-// template ratio is computed once per block into gr (scratch LDS), and
+// 0..T-2 cached, slot T-1 (the newest frame) in LDS slot 0. Block by block:
+// the template ratio and the slot addresses are formed once per block, and
 // every load of a bin is unconditional and typed by address space, so the
 // loads of an iteration issue together. Where the job has no template grid
 // or no sampled model, the grid and model loads read the bin's own cached
 // value instead (in bounds, discarded).
 template <int T, int BINS, int BS>
-__device__ __forceinline__ void temporal_pass_steady(const Fft3dJob& J, float* sre, float* sim, float* dc,
-                                                     float* gr, int nb, long long block_base, int tid, int nthreads,
-                                                     bool& bad) {
+__device__ __forceinline__ void temporal_pass_steady(const Fft3dJob& J, float* sre, float* sim, float* dc, int nb,
+                                                     long long block_base, int tid, int nthreads, bool& bad) {
   const int c = J.c;
   const bool use_grid = J.grid && J.degrid != 0.0f && J.grid[0].x != 0.0f;
-  for (int bi = tid; bi < nb; bi += nthreads) {
-    const float dcc = c < T - 1 ? J.cache_in[c][(block_base + bi) * 2 * BINS] : sre[bi * BS];
+  const bool has_grid = J.grid != nullptr, has_model = J.g.primary != nullptr;
+  for (int bi = 0; bi < nb; ++bi) {
+    const long long off = (block_base + bi) * 2 * BINS;
+    const float* cache[T > 1 ? T - 1 : 1];
+#pragma unroll
+    for (int j = 0; j < T - 1; ++j) cache[j] = J.cache_in[j] + off;
+    const float dcc = c < T - 1 ? J.cache_in[c][off] : sre[bi * BS];
     const float g_ratio = use_grid ? (J.degrid * dcc) / J.grid[0].x : 0.0f;
     bad |= !isfinite(g_ratio);
-    gr[bi] = g_ratio;
-  }
-  __syncthreads();
-  const bool has_grid = J.grid != nullptr, has_model = J.g.primary != nullptr;
-  for (int t = tid; t < nb * BINS; t += nthreads) {
-    const int bi = t / BINS, k = t - bi * BINS;
-    const long long off = (block_base + bi) * 2 * BINS;
-    float xr[T], xi[T];
+    float* br = sre + bi * BS;
+    float* bm = sim + bi * BS;
+    for (int k = tid; k < BINS; k += nthreads) {
+      float xr[T], xi[T];
 #pragma unroll
-    for (int j = 0; j < T - 1; ++j) {
-      xr[j] = J.cache_in[j][off + k];
-      xi[j] = J.cache_in[j][off + BINS + k];
-    }
-    xr[T - 1] = sre[bi * BS + k];
-    xi[T - 1] = sim[bi * BS + k];
-    const float2* gp = has_grid ? J.grid + k : reinterpret_cast<const float2*>(J.cache_in[0] + off + (k & ~1));
-    const float* pp = has_model ? J.g.primary + k : J.cache_in[0] + off + k;
-    const float2 gv = *gp;
-    const float pv = *pp;
-    const float g_ratio = gr[bi];
-    const float Mr = has_grid ? g_ratio * gv.x : 0.0f, Mi = has_grid ? g_ratio * gv.y : 0.0f;
-    const float noise = has_model ? pv * J.noise_mult : J.noise;
-    float orr, oi;
-    temporal_core<T>(J, xr, xi, Mr, Mi, noise, orr, oi, bad);
-    sre[bi * BS + k] = orr;
-    sim[bi * BS + k] = oi;
-    if (k == 0) {
-      dc[2 * bi] = orr;
-      dc[2 * bi + 1] = oi;
+      for (int j = 0; j < T - 1; ++j) {
+        xr[j] = cache[j][k];
+        xi[j] = cache[j][BINS + k];
+      }
+      xr[T - 1] = br[k];
+      xi[T - 1] = bm[k];
+      const float2* gp = has_grid ? J.grid + k : reinterpret_cast<const float2*>(cache[0] + (k & ~1));
+      const float* pp = has_model ? J.g.primary + k : cache[0] + k;
+      const float2 gv = *gp;
+      const float pv = *pp;
+      const float Mr = has_grid ? g_ratio * gv.x : 0.0f, Mi = has_grid ? g_ratio * gv.y : 0.0f;
+      const float noise = has_model ? pv * J.noise_mult : J.noise;
+      float orr, oi;
+      temporal_core<T>(J, xr, xi, Mr, Mi, noise, orr, oi, bad);
+      br[k] = orr;
+      bm[k] = oi;
+      if (k == 0) {
+        dc[2 * bi] = orr;
+        dc[2 * bi + 1] = oi;
+      }
     }
   }
 }
@@ -589,12 +589,10 @@ __global__ __launch_bounds__(64) void fft3d_block_kernel(const Fft3dLaunch L) {
     __syncthreads();
   } else if (J.T > 1) {
     // temporal Wiener per bin into slot 0 (spectral.cpp:162-232)
-    // the staging window is free here and holds the per-block template ratios
     bool steady = lidx[J.T - 1] == 0;
     for (int j = 0; j + 1 < J.T; ++j) steady = steady && lidx[j] < 0;
 #define NF_TEMPORAL(TT)                                                                                            \
-  (steady ? temporal_pass_steady<TT, BINS, BS>(J, sre, sim, dc, static_cast<float*>(win), nb, block_base, tid,    \
-                                               nthreads, bad)                                                      \
+  (steady ? temporal_pass_steady<TT, BINS, BS>(J, sre, sim, dc, nb, block_base, tid, nthreads, bad)             \
           : temporal_pass<TT, BINS, BS>(J, sre, sim, dc, nb, NB, lidx, block_base, tid, nthreads, bad))
     switch (J.T) {
       case 2: NF_TEMPORAL(2); break;
