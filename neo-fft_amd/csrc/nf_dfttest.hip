@@ -10,6 +10,8 @@
 // docs/specs/phase-5/dfttest/kernel-temporal-ola.md). Rows that no later
 // block row touches are converted and stored. No global atomics, no global
 // intermediate.
+#include <type_traits>
+
 #include "nf_device.hpp"
 #include "nf_fft.hpp"
 #include "nf_gain.hpp"
@@ -130,32 +132,36 @@ __device__ __forceinline__ void t1_tile(const DftJob& J, int tile, unsigned* fla
     }
     __syncthreads();
     // D: gain (zmean residual) and inverse column transforms
-    for (int t = tid; t < nbx * K; t += nthreads) {
-      const int bi = t / K, k = t - bi * K;
-      const float gs = J.grid ? (J.mean_scale * dc[2 * bi]) / J.grid[0].x : 0.0f;
-      float re[B], im[B];
-      float* cr = sre + bi * BS + k;
-      float* ci = sim + bi * BS + k;
+    auto stage_d = [&](auto wiener) {
+      for (int t = tid; t < nbx * K; t += nthreads) {
+        const int bi = t / K, k = t - bi * K;
+        const float gs = J.grid ? (J.mean_scale * dc[2 * bi]) / J.grid[0].x : 0.0f;
+        float re[B], im[B];
+        float* cr = sre + bi * BS + k;
+        float* ci = sim + bi * BS + k;
 #pragma unroll
-      for (int y = 0; y < B; ++y) {
-        re[y] = cr[y * KS];
-        im[y] = ci[y * KS];
-        const int bin = y * K + k;
-        float mr = 0.0f, mi = 0.0f;
-        if (J.grid) {
-          const float2 gv = J.grid[bin];
-          mr = gs * gv.x;
-          mi = gs * gv.y;
+        for (int y = 0; y < B; ++y) {
+          re[y] = cr[y * KS];
+          im[y] = ci[y * KS];
+          const int bin = y * K + k;
+          float mr = 0.0f, mi = 0.0f;
+          if (J.grid) {
+            const float2 gv = J.grid[bin];
+            mr = gs * gv.x;
+            mi = gs * gv.y;
+          }
+          apply_gain<decltype(wiener)::value>(J.g, bin, re[y], im[y], mr, mi, bad);
         }
-        apply_gain(J.g, bin, re[y], im[y], mr, mi, bad);
-      }
-      cfft<B, false>(re, im);
+        cfft<B, false>(re, im);
 #pragma unroll
-      for (int y = 0; y < B; ++y) {
-        cr[y * KS] = re[y];
-        ci[y * KS] = im[y];
+        for (int y = 0; y < B; ++y) {
+          cr[y * KS] = re[y];
+          ci[y * KS] = im[y];
+        }
       }
-    }
+    };
+    if (gain_is_wiener(J.g)) stage_d(std::true_type{});
+    else stage_d(std::false_type{});
     __syncthreads();
     // E: inverse row transforms with Hermitian extension, synthesis window
     for (int t = tid; t < nbx * B; t += nthreads) {

@@ -22,14 +22,29 @@ __device__ __forceinline__ float enhancement_gain(float q, int k, const GainPara
   return s * d;
 }
 
+// True when p is the plain Wiener gain (type 0, exponent 1, no enhancement):
+// apply_gain<true> then evaluates exactly the operations apply_gain<false>
+// would, without the per-bin dispatch.
+__device__ __forceinline__ bool gain_is_wiener(const GainParams& p) {
+  return p.type == 0 && fabsf(p.exponent - 1.0f) < 0.00005f && p.sharpen == 0.0f && p.dehalo == 0.0f;
+}
+
 // spectral.cpp:5-53 (spectral_prevalidated). x is modified in place; (mr, mi)
 // is the bin of scale*grid. Sets bad on any non-finite intermediate that can
-// reach the output.
+// reach the output. WIENER: the caller has checked gain_is_wiener(p).
+template <bool WIENER = false>
 __device__ __forceinline__ void apply_gain(const GainParams& p, int k, float& xr, float& xi, float mr, float mi,
                                            bool& bad) {
   const float a = p.primary ? p.primary[k] : p.a;
   const float re = xr - mr, im = xi - mi;
   const float power = re * re + im * im;
+  if constexpr (WIENER) {
+    const float gain = max_ref((power - a) / (power + 1e-15f), 0.0f);
+    xr = gain * re + mr;
+    xi = gain * im + mi;
+    bad |= !isfinite(power) || !isfinite(gain) || !isfinite(xr) || !isfinite(xi) || !isfinite(mr) || !isfinite(mi);
+    return;
+  }
   float gain = 0.0f;
   switch (p.type) {
     case -2:
