@@ -60,7 +60,9 @@ __device__ __forceinline__ Cpx inverse_slice(int T, const Cpx* v, int s, const f
 
 // Stages D and E of one chunk. TT > 0 fixes the temporal depth at compile
 // time (T == TT); TT == 0 handles any depth up to 15.
-template <int B, int TT>
+// WIENER: the plain Wiener gain (gain_is_wiener), evaluated without the
+// per-bin dispatch.
+template <int B, int TT, bool WIENER = false>
 __device__ __forceinline__ void temporal_stages(const DftJob& J, const DftTemporal& Q, float* sre, float* sim, float* dc, int nb,
                                                 int Tdyn, int target, int tid, int nthreads, bool& bad) {
   constexpr int K = B / 2 + 1;
@@ -122,7 +124,7 @@ __device__ __forceinline__ void temporal_stages(const DftJob& J, const DftTempor
         mr = gs * gv.x;
         mi = gs * gv.y;
       }
-      apply_gain(J.g, k3, re, im, mr, mi, bad);
+      apply_gain<WIENER>(J.g, k3, re, im, mr, mi, bad);
       v[z] = Cpx{re, im};
     }
     if (Q.emit_all) {
@@ -267,14 +269,21 @@ __device__ __forceinline__ void vol_tile(const DftJob& J, const DftTemporal& Q, 
         // inverse temporal transform. Depths 2..7 are compile-time so the
         // per-bin vectors stay in registers; the kAnyDepth instance takes
         // depths 8..15 (the host picks the instance from the depth).
+        const bool wiener = gain_is_wiener(J.g);
         if (kAnyDepth) {
           temporal_stages<B, 0>(J, Q, sre, sim, dc, nb, T, target, tid, nthreads, bad);
         } else {
           switch (T) {
             case 2: temporal_stages<B, 2>(J, Q, sre, sim, dc, nb, T, target, tid, nthreads, bad); break;
-            case 3: temporal_stages<B, 3>(J, Q, sre, sim, dc, nb, T, target, tid, nthreads, bad); break;
+            case 3:
+              if (wiener) temporal_stages<B, 3, true>(J, Q, sre, sim, dc, nb, T, target, tid, nthreads, bad);
+              else temporal_stages<B, 3>(J, Q, sre, sim, dc, nb, T, target, tid, nthreads, bad);
+              break;
             case 4: temporal_stages<B, 4>(J, Q, sre, sim, dc, nb, T, target, tid, nthreads, bad); break;
-            case 5: temporal_stages<B, 5>(J, Q, sre, sim, dc, nb, T, target, tid, nthreads, bad); break;
+            case 5:
+              if (wiener) temporal_stages<B, 5, true>(J, Q, sre, sim, dc, nb, T, target, tid, nthreads, bad);
+              else temporal_stages<B, 5>(J, Q, sre, sim, dc, nb, T, target, tid, nthreads, bad);
+              break;
             case 6: temporal_stages<B, 6>(J, Q, sre, sim, dc, nb, T, target, tid, nthreads, bad); break;
             default: temporal_stages<B, 7>(J, Q, sre, sim, dc, nb, T, target, tid, nthreads, bad); break;
           }

@@ -105,7 +105,7 @@ __device__ __forceinline__ Hl split_scaled(const float* v, float scale) {
 // DFT matrices, so the same four products transform four blocks; the
 // transposed products swap the off-diagonal quadrants, which the gain and
 // output index maps follow.
-template <int B, int TT>
+template <int B, int TT, bool WIENER>
 __device__ __forceinline__ void mfma_tile(const DftJob& J, const DftTemporal& Q, int tile, const float* tw,
                                           unsigned* flags) {
   // OS: row stride of the windows and block outputs, a multiple of four with
@@ -301,7 +301,7 @@ __device__ __forceinline__ void mfma_tile(const DftJob& J, const DftTemporal& Q,
                 mr = gs * gv.x;
                 mi = gs * gv.y;
               }
-              apply_gain(J.g, k3, re, im, mr, mi, bad);
+              apply_gain<WIENER>(J.g, k3, re, im, mr, mi, bad);
               Xr[p][z][e] = re;
               Xi[p][z][e] = im;
             } else {
@@ -407,24 +407,34 @@ __device__ __forceinline__ void mfma_tile(const DftJob& J, const DftTemporal& Q,
   if (xmax > kInLimit || ymax > kSpecLimit) atomicOr(flags, NF_FLAG_RANGE);
 }
 
+// The plain Wiener gain is selected once per tile (gain_is_wiener).
+template <int B, int TT>
+__device__ __forceinline__ void mfma_tile_any(const DftJob& J, const DftTemporal& Q, int tile, const float* tw,
+                                              unsigned* flags) {
+  // depth 5 keeps one path: two copies exceed its register budget
+  if (TT < 5 && gain_is_wiener(J.g)) mfma_tile<B, TT, TT < 5>(J, Q, tile, tw, flags);
+  else mfma_tile<B, TT, false>(J, Q, tile, tw, flags);
+}
+
 }  // namespace
 
 // Register budget: the compiler's default allocation for depths 3 and 5
-// exceeds 256 VGPRs (one wave per SIMD); capped at two waves per SIMD (four
-// for depth 1) it fits without spills.
+// exceeds 256 VGPRs (one wave per SIMD); capped at two waves per SIMD (three
+// for depth 1, where LDS allows no more than three workgroups per CU) it
+// fits without spills.
 template <int B, int TT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TT == 1 ? 4 : 2))) void dfttest_mfma_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TT == 1 ? 3 : 2))) void dfttest_mfma_kernel(
     const DftVolLaunch L, const float* tw) {
   int jid = 0;
   while (jid + 1 < L.njobs && (int)blockIdx.x >= L.tile_start[jid + 1]) ++jid;
-  mfma_tile<B, TT>(L.jobs[jid], L.tq[jid], (int)blockIdx.x - L.tile_start[jid], tw, L.flags);
+  mfma_tile_any<B, TT>(L.jobs[jid], L.tq[jid], (int)blockIdx.x - L.tile_start[jid], tw, L.flags);
 }
 
 template <int B, int TT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TT == 1 ? 4 : 2))) void dfttest_mfma_list_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TT == 1 ? 3 : 2))) void dfttest_mfma_list_kernel(
     const DftWorkList W, const float* tw) {
   for_each_work_item(W, [&](int j, int tile) {
-    mfma_tile<B, TT>(list_job(W, j), list_temporal(W, j), tile, tw, W.flags);
+    mfma_tile_any<B, TT>(list_job(W, j), list_temporal(W, j), tile, tw, W.flags);
   });
 }
 
