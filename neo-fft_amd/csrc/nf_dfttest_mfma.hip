@@ -419,11 +419,11 @@ __device__ __forceinline__ void mfma_tile_any(const DftJob& J, const DftTemporal
 }  // namespace
 
 // Register budget: the compiler's default allocation for depths 3 and 5
-// exceeds 256 VGPRs (one wave per SIMD); capped at two waves per SIMD (three
-// for depth 1, where LDS allows no more than three workgroups per CU) it
-// fits without spills.
+// exceeds 256 VGPRs (one wave per SIMD); capped at two waves per SIMD (depth
+// 1: four for the per-tile kernel, three for the work-list kernel, whose
+// job lookup needs more registers) it fits without spills.
 template <int B, int TT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TT == 1 ? 3 : 2))) void dfttest_mfma_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TT == 1 ? 4 : 2))) void dfttest_mfma_kernel(
     const DftVolLaunch L, const float* tw) {
   int jid = 0;
   while (jid + 1 < L.njobs && (int)blockIdx.x >= L.tile_start[jid + 1]) ++jid;
