@@ -330,15 +330,16 @@ def test_fft3d_launch_shape_invariance(nf, oracle, monkeypatch, nb):
     assert np.array_equal(got, ref), compare(got, ref)
 
 
-@pytest.mark.parametrize("bt,bw", [(3, 32), (5, 16), (2, 8)])
-def test_fft3d_spectrum_cache_sequences(nf, oracle, bt, bw):
+@pytest.mark.parametrize("bt,bw,kw", [(3, 32, {}), (5, 16, {}), (2, 8, {}), (3, 32, dict(degrid=0.0)),
+                                      (4, 32, dict(sharpen=0.4, dehalo=0.2)), (5, 32, dict(sigma2=1.0, sigma4=6.0))])
+def test_fft3d_spectrum_cache_sequences(nf, oracle, bt, bw, kw):
     """Reusing cached source spectra across outputs gives the same bits as
     computing every slot: sequential order, seeks, and a frame number whose
     plane moved to another buffer."""
     N, h, w = 10, 96, 130
     frames = _clip_frames(N, h, w, 8, 60 + bt)
     dev = [to_device(f) for f in frames]
-    params = dict(bw=bw, bh=bw, bt=bt, sigma=3.0)
+    params = dict(bw=bw, bh=bw, bt=bt, sigma=3.0, **kw)
     plan = nf.fft3d_plan(w, h, bits=8, **params)
     nf.set_spectrum_cache(plan, bt + 1)
     c = bt // 2

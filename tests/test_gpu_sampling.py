@@ -201,3 +201,24 @@ def test_filter_fft3d_pfactor_with_roi(nf, oracle):
             ref[t:t + rh, l:l + rw] = oracle.fft3d_plane([frames[s][p][t:t + rh, l:l + rw] for s in slots], bits=8,
                                                          chroma=p > 0, pattern=models[p], **oparams)
             assert np.array_equal(to_host(got[p], np.uint8), ref), (n, p)
+
+
+@pytest.mark.parametrize("bw", [16, 32])
+def test_fft3d_pattern_model_with_spectrum_cache(nf, oracle, bw):
+    """A sampled noise model in the spectrum cache's sequential steady state
+    (cached slots, newest frame computed): same bits as the oracle."""
+    rng = np.random.default_rng(30 + bw)
+    h, w, N = 96, 128, 7
+    frames = [_plane(rng, h, w, 8, False) for _ in range(N)]
+    params = dict(bw=bw, bh=bw, bt=3, pfactor=1.5, sigma=3.0)
+    plan = nf.fft3d_plan(w, h, bits=8, **params)
+    nf.set_spectrum_cache(plan, 4)
+    nf.fft3d_prepare_pattern(plan, to_device(frames[0]))
+    _, _, ref_model = oracle.fft3d_pattern(frames[0], bits=8, **params)
+    dev = [to_device(f) for f in frames]
+    for n in range(1, N - 1):
+        dst = empty_like_np(frames[0])
+        plan.process([dev[n - 1], dev[n], dev[n + 1]], dst, frame=n)
+        ref = oracle.fft3d_plane(frames[n - 1:n + 2], bits=8, pattern=ref_model, **params)
+        got = to_host(dst, np.uint8)
+        assert np.array_equal(got, ref), (n, compare(got, ref))
