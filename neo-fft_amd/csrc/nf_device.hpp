@@ -25,6 +25,17 @@ __host__ __device__ __forceinline__ int floor_div(int a, int b) {
 }
 __host__ __device__ __forceinline__ int ceil_div_int(int a, int b) { return -floor_div(-a, b); }
 
+// floor(a / b) for |a| < 2^23 and 0 < b < 2^15 through a float estimate
+// (within one of the quotient: a and b are exact in float, inv_b = 1/b to a
+// few ulp) and one integer correction; inv_b is computed once per divisor.
+__device__ __forceinline__ int floor_div_est(int a, int b, float inv_b) {
+  int q = (int)floorf((float)a * inv_b);
+  const int r = a - q * b;
+  q += (r >= b) - (r < 0);
+  return q;
+}
+__device__ __forceinline__ int ceil_div_est(int a, int b, float inv_b) { return -floor_div_est(-a, b, inv_b); }
+
 // Sample storage of a plane: 0 = u8, 1 = u16, 2 = f32.
 enum Storage : int { kU8 = 0, kU16 = 1, kF32 = 2 };
 

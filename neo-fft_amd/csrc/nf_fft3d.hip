@@ -985,12 +985,15 @@ __global__ __launch_bounds__(256) void fft3d_synthesis_kernel(const Fft3dLaunch 
     return;
   }
   const int gpr = synthesis_groups(J);
-  const int oy = u / gpr, gi = u - oy * gpr;
+  const int BW = J.gx.block, BH = J.gy.block, Sx = J.gx.step, Sy = J.gy.step;
+  // integer divisions by job constants through float estimates (exact after
+  // one correction; all operands are far below 2^23)
+  const float inv_gpr = 1.0f / (float)gpr, inv_sx = 1.0f / (float)Sx, inv_sy = 1.0f / (float)Sy;
+  const int oy = floor_div_est(u, gpr, inv_gpr), gi = u - oy * gpr;
   const int x0 = (J.gx.offset / 4 + gi) * 4;
   const int y = oy + J.gy.offset;
-  const int BW = J.gx.block, BH = J.gy.block, Sx = J.gx.step, Sy = J.gy.step;
-  const int by_first = max(0, ceil_div_int(y - BH + 1, Sy)), by_last = min(J.gy.count - 1, y / Sy);
-  const int bx_first = max(0, ceil_div_int(x0 - BW + 1, Sx)), bx_last = min(J.gx.count - 1, x0 / Sx);
+  const int by_first = max(0, ceil_div_est(y - BH + 1, Sy, inv_sy)), by_last = min(J.gy.count - 1, floor_div_est(y, Sy, inv_sy));
+  const int bx_first = max(0, ceil_div_est(x0 - BW + 1, Sx, inv_sx)), bx_last = min(J.gx.count - 1, floor_div_est(x0, Sx, inv_sx));
   float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   for (int by = by_first; by <= by_last; ++by) {
     const int yy = y - by * Sy;
