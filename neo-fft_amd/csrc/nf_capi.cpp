@@ -19,6 +19,7 @@
 
 #include "neo_fft.h"
 #include "nf_device.hpp"
+#include "nf_gain.hpp"
 #include "nf_launch.hpp"
 #include "nf_tables.hpp"
 
@@ -1329,9 +1330,11 @@ void enqueue_dft_mfma_jobs(const std::vector<const nf_plan*>& plans, const std::
     const int B = plans[first]->B, T = plans[first]->temporal;
     const std::vector<const nf_plan*> group(plans.begin() + first, plans.begin() + first + n);
     size_t ncand = group[0]->dft_tiles.size();
+    bool general = false;
     for (auto* p : group) {
       require(p->B == B && p->temporal == T, "batched matrix-core DFTTest jobs must share sbsize and tbsize");
       ncand = std::min(ncand, p->dft_tiles.size());
+      general = general || !gain_is_wiener(p->g);
     }
     auto launch = [&](auto tile_of) {
       DftVolLaunch L{};
@@ -1348,7 +1351,8 @@ void enqueue_dft_mfma_jobs(const std::vector<const nf_plan*>& plans, const std::
       L.tile_start[n] = tiles;
       L.njobs = (int)n;
       L.flags = flags;
-      hip_check(launch_dfttest_mfma(B, T, L, tiles, lds, group[0]->d_tw, stream), "dfttest mfma kernel launch");
+      hip_check(launch_dfttest_mfma(B, T, L, tiles, lds, group[0]->d_tw, stream, general),
+                "dfttest mfma kernel launch");
     };
     // candidates whose matrix-core LDS fits
     int fit = 0;
@@ -1678,9 +1682,17 @@ void batch_shapes(nf_batch& b) {
   if (b.shapes.empty()) throw Unsupported("DFTTest batch: no tile schedule fits the LDS");
 }
 
+// Some job of the batch has a gain other than the plain Wiener one.
+bool batch_general_gain(const nf_batch& b) {
+  for (auto* p : b.plans)
+    if (!gain_is_wiener(p->g)) return true;
+  return false;
+}
+
 const void* batch_function(const nf_batch& b) {
-  return b.kind == 0 ? dfttest_t1_list_function(b.B)
-                     : b.kind == 1 ? dfttest_vol_list_function(b.B, b.max_depth) : dfttest_mfma_list_function(b.B, b.T);
+  return b.kind == 0   ? dfttest_t1_list_function(b.B)
+         : b.kind == 1 ? dfttest_vol_list_function(b.B, b.max_depth)
+                       : dfttest_mfma_list_function(b.B, b.T, batch_general_gain(b));
 }
 
 // Builds and uploads the work list of schedule `s`: jobs tiled by the shape,
@@ -1758,7 +1770,7 @@ void launch_batch(const nf_batch& b, const BatchTables& t, unsigned* flags, hipS
   hipError_t e;
   if (b.kind == 0) e = launch_dfttest_t1_list(b.B, W, t.workgroups, t.lds, stream);
   else if (b.kind == 1) e = launch_dfttest_vol_list(b.B, b.max_depth, W, t.workgroups, t.lds, stream);
-  else e = launch_dfttest_mfma_list(b.B, b.T, W, t.workgroups, t.lds, b.tw, stream);
+  else e = launch_dfttest_mfma_list(b.B, b.T, W, t.workgroups, t.lds, b.tw, stream, batch_general_gain(b));
   hip_check(e, "dfttest batch launch");
 }
 }  // namespace
@@ -1779,7 +1791,7 @@ extern "C" nf_status nf_batch_create(const nf_plan* const* plans, int n_jobs, co
     b->T = p0->temporal;
     b->kind = p0->transform == NF_TRANSFORM_MFMA_F16X3 ? 2 : (p0->temporal > 1 ? 1 : 0);
     if (b->kind == 2) {
-      require(dfttest_mfma_supported(b->B, b->T), "matrix-core form needs sbsize 8 or 16 and tbsize 1, 3 or 5");
+      require(dfttest_mfma_supported(b->B, b->T), "matrix-core form needs sbsize 8 or 16 with tbsize 1, 3 or 5, or sbsize 32 with tbsize 1");
       b->tw = p0->d_tw;
     }
     for (int i = 0; i < n_jobs; ++i) {
@@ -1999,7 +2011,7 @@ extern "C" nf_status nf_plan_set_transform(nf_plan* plan, int transform) {
     if (transform == NF_TRANSFORM_MFMA_F16X3) {
       if (plan->algorithm != 0) throw Unsupported("the matrix-core form covers DFTTest plans");
       if (!dfttest_mfma_supported(plan->B, plan->temporal) || plan->tmode != 0)
-        throw Unsupported("the matrix-core form covers sbsize 8 or 16 with tbsize 1, 3 or 5 (tmode 0)");
+        throw Unsupported("the matrix-core form covers sbsize 8 or 16 with tbsize 1, 3 or 5, and sbsize 32 with tbsize 1 (tmode 0)");
       if (plan->dither > 0 && plan->storage == kU8) throw Unsupported("the matrix-core form does not dither");
     }
     plan->transform = transform;

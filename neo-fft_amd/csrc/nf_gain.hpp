@@ -25,7 +25,7 @@ __device__ __forceinline__ float enhancement_gain(float q, int k, const GainPara
 // True when p is the plain Wiener gain (type 0, exponent 1, no enhancement):
 // apply_gain<true> then evaluates exactly the operations apply_gain<false>
 // would, without the per-bin dispatch.
-__device__ __forceinline__ bool gain_is_wiener(const GainParams& p) {
+__host__ __device__ inline bool gain_is_wiener(const GainParams& p) {
   return p.type == 0 && fabsf(p.exponent - 1.0f) < 0.00005f && p.sharpen == 0.0f && p.dehalo == 0.0f;
 }
 
@@ -83,5 +83,6 @@ __device__ __forceinline__ void apply_gain(const GainParams& p, int k, float& xr
   xi = gain * im + mi;
   bad |= !isfinite(power) || !isfinite(gain) || !isfinite(xr) || !isfinite(xi) || !isfinite(mr) || !isfinite(mi);
 }
+
 
 }  // namespace nf

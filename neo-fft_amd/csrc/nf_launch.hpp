@@ -37,12 +37,14 @@ hipError_t launch_fft3d_generic_block(const GenericFft3dArgs& A, size_t lds, hip
 
 size_t dfttest_mfma_lds_bytes(int B, int T, int nbx_max, int ww, int tile_w);
 bool dfttest_mfma_supported(int B, int T);
+// general: some job has a gain other than the plain Wiener one (selects the
+// general-gain kernel where the form has separate kernels, sbsize 32)
 hipError_t launch_dfttest_mfma(int B, int T, const DftVolLaunch& L, int total_tiles, size_t lds, const float* tw,
-                               hipStream_t stream);
+                               hipStream_t stream, bool general);
 hipError_t dfttest_mfma_set_lds_limit();
 hipError_t launch_dfttest_mfma_list(int B, int T, const DftWorkList& W, int workgroups, size_t lds, const float* tw,
-                                    hipStream_t stream);
-const void* dfttest_mfma_list_function(int B, int T);
+                                    hipStream_t stream, bool general);
+const void* dfttest_mfma_list_function(int B, int T, bool general);
 
 size_t dfttest_generic_lds_bytes(int B, int T);
 size_t dfttest_generic_contrib_floats(const GenericDftArgs& A);

@@ -26,11 +26,10 @@ def _run(nf, plan, slots, dtype):
     return to_host(dst, dtype)
 
 
-@pytest.mark.parametrize("T", [1, 3, 5])
-@pytest.mark.parametrize("B", [8, 16])
+@pytest.mark.parametrize("T,B", [(1, 8), (3, 8), (5, 8), (1, 16), (3, 16), (5, 16), (1, 32)])
 def test_mfma_float_error_against_binary64(nf, oracle, T, B):
     rng = np.random.default_rng(40 + T + B)
-    h, w = 120, 176
+    h, w = (120, 176) if B < 32 else (160, 208)
     slots = [_smooth_noisy(rng, h, w) for _ in range(T)]
     O = 3 * B // 4
     params = dict(tbsize=T, sbsize=B, sosize=O, ftype=0, sigma=8.0)
@@ -49,12 +48,11 @@ def test_mfma_float_error_against_binary64(nf, oracle, T, B):
     assert errs[nf.TRANSFORM_MFMA_F16X3][1] <= 1e-5
 
 
-@pytest.mark.parametrize("T", [1, 3])
+@pytest.mark.parametrize("T,B", [(1, 8), (3, 8), (1, 16), (3, 16), (1, 32)])
 @pytest.mark.parametrize("bits", [8, 16])
-@pytest.mark.parametrize("B", [8, 16])
 def test_mfma_integer_planes_match_radix2(nf, oracle, T, bits, B):
     rng = np.random.default_rng(bits + T + B)
-    h, w = 100, 150
+    h, w = (100, 150) if B < 32 else (150, 200)
     peak = (1 << bits) - 1
     base = rng.integers(0, peak + 1, (h, w))
     dt = np.uint8 if bits == 8 else np.uint16
@@ -73,7 +71,7 @@ def test_mfma_integer_planes_match_radix2(nf, oracle, T, bits, B):
 
 
 def test_mfma_form_coverage(nf):
-    plan = nf.dfttest_plan(96, 96, bits=8, tbsize=1, sbsize=32, sosize=24)
+    plan = nf.dfttest_plan(96, 96, bits=8, tbsize=3, sbsize=32, sosize=24)
     with pytest.raises(nf.Unsupported):
         nf.set_transform(plan, nf.TRANSFORM_MFMA_F16X3)
     plan = nf.dfttest_plan(64, 64, bits=8, tbsize=1, sbsize=12, sosize=9)
@@ -84,13 +82,13 @@ def test_mfma_form_coverage(nf):
         nf.set_transform(plan, nf.TRANSFORM_MFMA_F16X3)
 
 
-@pytest.mark.parametrize("B", [8, 16])
+@pytest.mark.parametrize("B", [8, 16, 32])
 def test_mfma_range_flag(nf, B):
     """A float plane far outside its nominal range would leave the FP16
     operands: the form reports it instead of returning saturated values;
     the radix-2 form filters the same plane."""
     rng = np.random.default_rng(7)
-    h, w = 64, 96
+    h, w = (64, 96) if B < 32 else (128, 160)
     plane = _smooth_noisy(rng, h, w) * 4000.0
     params = dict(tbsize=1, sbsize=B, sosize=3 * B // 4, ftype=0, sigma=8.0)
     plan = nf.dfttest_plan(w, h, bits=32, floating=True, **params)
@@ -100,3 +98,20 @@ def test_mfma_range_flag(nf, B):
     plan = nf.dfttest_plan(w, h, bits=32, floating=True, **params)
     out = _run(nf, plan, [plane], np.float32)
     assert np.isfinite(out).all()
+
+
+@pytest.mark.parametrize("kw", [dict(ftype=1, sigma=20.0), dict(ftype=0, f0beta=0.5, sigma=8.0),
+                                dict(ftype=2, sigma=0.7)])
+def test_mfma_sbsize32_general_gains(nf, oracle, kw):
+    """sbsize 32 runs a separate kernel for gains other than the plain
+    Wiener one: integer outputs within one step of the bit-exact form."""
+    rng = np.random.default_rng(3)
+    h, w = 150, 200
+    slots = [np.clip(rng.normal(128, 40, (h, w)), 0, 255).astype(np.uint8)]
+    params = dict(tbsize=1, sbsize=32, sosize=24, **kw)
+    ref = oracle.dfttest_plane(slots, bits=8, **params)
+    plan = nf.dfttest_plan(w, h, bits=8, **params)
+    nf.set_transform(plan, nf.TRANSFORM_MFMA_F16X3)
+    got = _run(nf, plan, slots, np.uint8)
+    mx, frac = compare(got, ref)
+    assert mx <= 1 and frac <= 2e-3, (mx, frac)
