@@ -221,6 +221,28 @@ def main():
             "speedup_vs_radix2": round(dms / mms, 3),
             "error_vs_binary64": mfma_error(nf, torch),
         }
+        # sbsize 8 and 32 (sosize 3/4 of it), both forms, same frames
+        forms = {}
+        for B in (8, 32):
+            bplans = [nf.dfttest_plan(W if p == 0 else W // 2, H if p == 0 else H // 2, bits=16, chroma=p > 0,
+                                      **dict(DFT_PARAMS, sbsize=B, sosize=3 * B // 4)) for p in range(3)]
+
+            def bstep(n):
+                c = n % R
+                nf.process_batch([(bplans[p], [frames[c][p]], dsts[p]) for p in range(3)], flags, stream=stream)
+
+            for i in range(3):
+                bstep(i)
+            r_ms = _timed(torch, stream, bstep, dsteps)
+            for p in bplans:
+                nf.set_transform(p, nf.TRANSFORM_MFMA_F16X3)
+            for i in range(3):
+                bstep(i)
+            m_ms = _timed(torch, stream, bstep, dsteps)
+            forms[f"sbsize{B}"] = {"radix2_frames_per_s": round(dsteps / (r_ms / 1e3), 2),
+                                   "mfma_f16x3_frames_per_s": round(dsteps / (m_ms / 1e3), 2),
+                                   "mfma_speedup_vs_radix2": round(r_ms / m_ms, 3)}
+        result["dfttest_tbsize1_sbsize8_32"] = forms
 
     if not args.no_configs:
         result["configs"] = other_configs(nf, torch, clip, stream, flags, args)
@@ -247,19 +269,20 @@ def measured_traffic(kernel):
 
 def mfma_error(nf, torch):
     """Max and RMS error of both forms against a binary64 evaluation on a
-    176x120 float32 plane (smooth texture + noise), sbsize 8 and 16 (sosize
-    3/4 of it), tbsize 1 and 3."""
+    176x120 float32 plane (208x160 for sbsize 32; smooth texture + noise),
+    sbsize 8 and 16 with tbsize 1 and 3, sbsize 32 with tbsize 1 (sosize 3/4
+    of sbsize)."""
     import numpy as np
     import dfttest_fp64 as F64
     from oracle import oracle as O
     from gpu_util import empty_like_np, to_device, to_host
     rng = np.random.default_rng(40)
-    h, w = 120, 176
-    yy, xx = np.mgrid[0:h, 0:w]
     out = {}
-    for B in (8, 16):
+    for B in (8, 16, 32):
         ov = 3 * B // 4
-        for T in (1, 3):
+        h, w = (120, 176) if B < 32 else (160, 208)
+        yy, xx = np.mgrid[0:h, 0:w]
+        for T in ((1, 3) if B < 32 else (1,)):
             slots = []
             for _ in range(T):
                 img = 0.5 + 0.25 * np.sin(xx * 0.11 + yy * 0.05) + 0.1 * np.cos(yy * 0.3)
