@@ -25,6 +25,12 @@ __host__ __device__ __forceinline__ int floor_div(int a, int b) {
 }
 __host__ __device__ __forceinline__ int ceil_div_int(int a, int b) { return -floor_div(-a, b); }
 
+// Value of the other lane of this lane's pair (lanes 2i, 2i+1): one DPP
+// quad permutation (1,0,3,2). Both lanes of the pair must be active.
+__device__ __forceinline__ float swap_pair(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, false));
+}
+
 // floor(a / b) for |a| < 2^23 and 0 < b < 2^15 through a float estimate
 // (within one of the quotient: a and b are exact in float, inv_b = 1/b to a
 // few ulp) and one integer correction; inv_b is computed once per divisor.

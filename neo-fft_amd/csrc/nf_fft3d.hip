@@ -349,11 +349,6 @@ __device__ __forceinline__ void inverse_rows32(const Fft3dJob& J, float* sre, fl
   }
 }
 
-// Value of the other lane of this lane's pair (lanes 2i, 2i+1).
-__device__ __forceinline__ float swap_pair(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, false));
-}
-
 // Stages the reflected source rows of one temporal slot in LDS and runs the
 // windowed forward row transforms ((x - base) * wy) * wx (pad.hpp:33-38,
 // scalar/spatial.cpp:12-16). Integer samples are staged raw (two or one
