@@ -285,6 +285,17 @@ nf_status nf_kalman_enqueue_advance(const nf_plan* plan, nf_plane source, nf_kal
                                     unsigned* device_flags, void* stream);
 nf_status nf_kalman_enqueue_render(const nf_plan* plan, nf_plane source, nf_mut_plane dst,
                                    const nf_kalman_state* state, unsigned* device_flags, void* stream);
+/* Batched asynchronous forms: n planes (each with its own plan and state,
+ * e.g. the three planes of a frame) in one block-kernel launch (and, for
+ * render, one synthesis launch). Same results as n single calls; a state
+ * may appear once per advance batch. No counterpart in the reference (its
+ * planes run one after another). */
+nf_status nf_kalman_enqueue_advance_batch(const nf_plan* const* plans, const nf_plane* sources,
+                                          nf_kalman_state* const* states, int n, unsigned* device_flags,
+                                          void* stream);
+nf_status nf_kalman_enqueue_render_batch(const nf_plan* const* plans, const nf_plane* sources,
+                                         const nf_mut_plane* dsts, const nf_kalman_state* const* states, int n,
+                                         unsigned* device_flags, void* stream);
 /* Host access to the state (tests): last as interleaved complex, cov, proc. */
 nf_status nf_kalman_state_download(const nf_kalman_state* state, float* last, float* cov,
                                    float* proc);

@@ -2202,12 +2202,10 @@ extern "C" nf_status nf_kalman_state_download(const nf_kalman_state* s, float* l
 }
 
 namespace {
-void enqueue_kalman_advance(const nf_plan* plan, nf_plane source, nf_kalman_state* state, unsigned* flags,
-                            hipStream_t stream) {
+Fft3dJob kalman_advance_job(const nf_plan* plan, nf_plane source, const nf_kalman_state* state) {
   require(plan && plan->algorithm == 1 && plan->kalman, "not a Kalman plan");
   require(state && state->plan && state->bins == (size_t)plan->gx.count * plan->gy.count * plan->bh * (plan->bw / 2 + 1),
           "Kalman state shape differs");
-  require(flags != nullptr, "device flag word required");
   check_plane(*plan, source.width, source.height, source.stride_bytes, source.data);
   Fft3dJob j = make_fft3d_job(*plan, 1, 1);
   j.src[0] = source.data;
@@ -2215,15 +2213,13 @@ void enqueue_kalman_advance(const nf_plan* plan, nf_plane source, nf_kalman_stat
   j.k_last = state->last;
   j.k_cov = state->cov;
   j.k_proc = state->proc;
-  enqueue_fft3d_jobs({plan}, {j}, flags, stream, false);
+  return j;
 }
 
-void enqueue_kalman_render(const nf_plan* plan, nf_plane source, nf_mut_plane dst, const nf_kalman_state* state,
-                           unsigned* flags, hipStream_t stream) {
+Fft3dJob kalman_render_job(const nf_plan* plan, nf_plane source, nf_mut_plane dst, const nf_kalman_state* state) {
   require(plan && plan->algorithm == 1 && plan->kalman, "not a Kalman plan");
   require(state && state->bins == (size_t)plan->gx.count * plan->gy.count * plan->bh * (plan->bw / 2 + 1),
           "Kalman checkpoint shape differs");
-  require(flags != nullptr, "device flag word required");
   check_plane(*plan, source.width, source.height, source.stride_bytes, source.data);
   check_plane(*plan, dst.width, dst.height, dst.stride_bytes, dst.data);
   Fft3dJob j = make_fft3d_job(*plan, 1, 2);
@@ -2232,7 +2228,19 @@ void enqueue_kalman_render(const nf_plan* plan, nf_plane source, nf_mut_plane ds
   j.k_proc = state->proc;
   j.dst = dst.data;
   j.dst_stride = dst.stride_bytes;
-  enqueue_fft3d_jobs({plan}, {j}, flags, stream, true);
+  return j;
+}
+
+void enqueue_kalman_advance(const nf_plan* plan, nf_plane source, nf_kalman_state* state, unsigned* flags,
+                            hipStream_t stream) {
+  require(flags != nullptr, "device flag word required");
+  enqueue_fft3d_jobs({plan}, {kalman_advance_job(plan, source, state)}, flags, stream, false);
+}
+
+void enqueue_kalman_render(const nf_plan* plan, nf_plane source, nf_mut_plane dst, const nf_kalman_state* state,
+                           unsigned* flags, hipStream_t stream) {
+  require(flags != nullptr, "device flag word required");
+  enqueue_fft3d_jobs({plan}, {kalman_render_job(plan, source, dst, state)}, flags, stream, true);
 }
 }  // namespace
 
@@ -2264,6 +2272,42 @@ extern "C" nf_status nf_kalman_enqueue_render(const nf_plan* plan, nf_plane sour
                                               const nf_kalman_state* state, unsigned* device_flags, void* stream) {
   return guarded([&]() -> nf_status {
     enqueue_kalman_render(plan, source, dst, state, device_flags, (hipStream_t)stream);
+    return NF_OK;
+  });
+}
+
+extern "C" nf_status nf_kalman_enqueue_advance_batch(const nf_plan* const* plans, const nf_plane* sources,
+                                                     nf_kalman_state* const* states, int n, unsigned* device_flags,
+                                                     void* stream) {
+  return guarded([&]() -> nf_status {
+    require(plans && sources && states && n > 0, "null argument");
+    require(device_flags != nullptr, "device flag word required");
+    std::vector<const nf_plan*> pv;
+    std::vector<Fft3dJob> jobs;
+    for (int i = 0; i < n; ++i) {
+      require(states[i] != nullptr, "null argument");
+      for (int k = 0; k < i; ++k) require(states[k] != states[i], "a Kalman state appears twice in one batch");
+      pv.push_back(plans[i]);
+      jobs.push_back(kalman_advance_job(plans[i], sources[i], states[i]));
+    }
+    enqueue_fft3d_jobs(pv, jobs, device_flags, (hipStream_t)stream, false);
+    return NF_OK;
+  });
+}
+
+extern "C" nf_status nf_kalman_enqueue_render_batch(const nf_plan* const* plans, const nf_plane* sources,
+                                                    const nf_mut_plane* dsts, const nf_kalman_state* const* states,
+                                                    int n, unsigned* device_flags, void* stream) {
+  return guarded([&]() -> nf_status {
+    require(plans && sources && dsts && states && n > 0, "null argument");
+    require(device_flags != nullptr, "device flag word required");
+    std::vector<const nf_plan*> pv;
+    std::vector<Fft3dJob> jobs;
+    for (int i = 0; i < n; ++i) {
+      pv.push_back(plans[i]);
+      jobs.push_back(kalman_render_job(plans[i], sources[i], dsts[i], states[i]));
+    }
+    enqueue_fft3d_jobs(pv, jobs, device_flags, (hipStream_t)stream, true);
     return NF_OK;
   });
 }

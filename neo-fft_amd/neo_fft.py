@@ -142,6 +142,12 @@ _SIGNATURES = {
     "nf_kalman_advance": (ctypes.c_int, [ctypes.c_void_p, Plane, ctypes.c_void_p, ctypes.c_void_p]),
     "nf_kalman_render": (ctypes.c_int, [ctypes.c_void_p, Plane, MutPlane, ctypes.c_void_p, ctypes.c_void_p]),
     "nf_kalman_state_download": (ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p, c_float_p]),
+    "nf_kalman_enqueue_advance_batch": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(Plane),
+                                                       ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_void_p,
+                                                       ctypes.c_void_p]),
+    "nf_kalman_enqueue_render_batch": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(Plane),
+                                                      ctypes.POINTER(MutPlane), ctypes.POINTER(ctypes.c_void_p),
+                                                      ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     "nf_kalman_enqueue_advance": (ctypes.c_int, [ctypes.c_void_p, Plane, ctypes.c_void_p, ctypes.c_void_p,
                                                  ctypes.c_void_p]),
     "nf_kalman_enqueue_render": (ctypes.c_int, [ctypes.c_void_p, Plane, MutPlane, ctypes.c_void_p, ctypes.c_void_p,
@@ -534,6 +540,27 @@ class KalmanState:
     def enqueue_render(self, source, dst, flags, stream=None):
         _check(lib().nf_kalman_enqueue_render(self.plan.handle, _plane(source), _plane(dst, MutPlane), self._h,
                                               ctypes.c_void_p(flags.data_ptr()), _stream_handle(stream)))
+
+    @staticmethod
+    def enqueue_advance_batch(states, sources, flags, stream=None):
+        """nf_kalman_enqueue_advance_batch: one launch for every (state, source)."""
+        n = len(states)
+        plans = (ctypes.c_void_p * n)(*[st.plan.handle for st in states])
+        srcs = (Plane * n)(*[_plane(s_) for s_ in sources])
+        hs = (ctypes.c_void_p * n)(*[st._h for st in states])
+        _check(lib().nf_kalman_enqueue_advance_batch(plans, srcs, hs, n, ctypes.c_void_p(flags.data_ptr()),
+                                                     _stream_handle(stream)))
+
+    @staticmethod
+    def enqueue_render_batch(states, sources, dsts, flags, stream=None):
+        """nf_kalman_enqueue_render_batch: one block and one synthesis launch."""
+        n = len(states)
+        plans = (ctypes.c_void_p * n)(*[st.plan.handle for st in states])
+        srcs = (Plane * n)(*[_plane(s_) for s_ in sources])
+        outs = (MutPlane * n)(*[_plane(d, MutPlane) for d in dsts])
+        hs = (ctypes.c_void_p * n)(*[st._h for st in states])
+        _check(lib().nf_kalman_enqueue_render_batch(plans, srcs, outs, hs, n, ctypes.c_void_p(flags.data_ptr()),
+                                                    _stream_handle(stream)))
 
     def download(self):
         import numpy as np

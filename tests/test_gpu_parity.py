@@ -382,3 +382,39 @@ def test_fft3d_spectrum_cache_batch(nf, oracle):
             ref = oracle.fft3d_plane([frames[n - 1][p], frames[n][p], frames[n + 1][p]], bits=16, chroma=p > 0,
                                      **params)
             assert np.array_equal(to_host(outs[p], np.uint16), ref), (n, p)
+
+
+def test_kalman_batched_planes_match_single(nf, oracle):
+    """The batched Kalman calls (all planes of a frame in one launch) give
+    the same states and outputs as one call per plane, and both match the
+    oracle frame by frame."""
+    import torch
+    frames = yuv420_sinusoids(96, 64, frames=5, bits=10, seed=4)
+    params = dict(bw=32, bh=32, bt=0, sigma=2.0, kratio=2.0)
+    dims = [(96, 64), (48, 32), (48, 32)]
+    plans = [nf.fft3d_plan(w, h, bits=10, chroma=p > 0, **params) for p, (w, h) in enumerate(dims)]
+    batched = [nf.KalmanState(pl) for pl in plans]
+    single = [nf.KalmanState(pl) for pl in plans]
+    refs = [oracle.KalmanState(params, w, h, bits=10) for (w, h) in dims]
+    flags = torch.zeros(1, dtype=torch.int32, device="cuda")
+    for fr in frames:
+        dev = [to_device(pl) for pl in fr]
+        nf.KalmanState.enqueue_advance_batch(batched, dev, flags)
+        for p in range(3):
+            single[p].enqueue_advance(dev[p], flags)
+        outs_b = [empty_like_np(pl) for pl in fr]
+        outs_s = [empty_like_np(pl) for pl in fr]
+        nf.KalmanState.enqueue_render_batch(batched, dev, outs_b, flags)
+        for p in range(3):
+            single[p].enqueue_render(dev[p], outs_s[p], flags)
+        torch.cuda.synchronize()
+        assert int(flags.item()) == 0
+        for p in range(3):
+            refs[p].advance(fr[p], chroma=p > 0)
+            ref = refs[p].render(chroma=p > 0)
+            got_b, got_s = to_host(outs_b[p], np.uint16), to_host(outs_s[p], np.uint16)
+            assert np.array_equal(got_b, got_s), p
+            assert np.array_equal(got_b, ref), (p, compare(got_b, ref))
+            lb, cb, qb = batched[p].download()
+            ls, cs, qs = single[p].download()
+            assert np.array_equal(lb, ls) and np.array_equal(cb, cs) and np.array_equal(qb, qs)
