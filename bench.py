@@ -376,15 +376,23 @@ def other_configs(nf, torch, clip, stream, flags, args):
     vplans = [nf.dfttest_plan(w, h, bits=10, chroma=p > 0, **vparams) for p, (w, h) in enumerate(dims)]
     vouts = [[torch.empty_like(p10[0][p]) for _ in range(5)] for p in range(3)]
 
-    def c3b(i):
+    def c3v(i):
         for p in range(3):
             nf.dfttest_volume(vplans[p], [p10[(i + z) % len(p10)][p] for z in range(5)], vouts[p], flags,
                               stream=stream)
+
+    def c3b(i):
+        nf.dfttest_volume_batch(vplans, [[p10[(i + z) % len(p10)][p] for z in range(5)] for p in range(3)], vouts,
+                                flags, stream=stream)
+    c3v(0)
     c3b(0)
     vsteps = max(4, args.steps // 20)
+    ms_single = _timed(torch, stream, c3v, vsteps)
     ms = _timed(torch, stream, c3b, vsteps)
     out["dfttest_t5_tmode1_1080p_p10"] = {"frames_per_s": round(5 * vsteps / (ms / 1e3), 1),
-                                          "ms_per_frame": round(ms / (5 * vsteps), 4)}
+                                          "ms_per_frame": round(ms / (5 * vsteps), 4),
+                                          "launch": "planes batched: one volume launch per 5 frames",
+                                          "per_plane_calls_frames_per_s": round(5 * vsteps / (ms_single / 1e3), 1)}
 
     # 16 float32 256x256 patches: Wiener forward + adjoint per patch
     wplan = nf.dfttest_plan(256, 256, bits=32, floating=True, tbsize=1, sbsize=16, sosize=12, ftype=0, sigma=8.0)

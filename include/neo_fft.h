@@ -200,6 +200,13 @@ void nf_graph_destroy(nf_graph* graph);
 nf_status nf_dfttest_enqueue_volume(const nf_plan* plan, const nf_plane* sources, int n_sources,
                                     const nf_mut_plane* dsts, int first_frame, int plane,
                                     unsigned* device_flags, void* stream);
+/* Volume passes of n plans (the planes of one frame group, same tbsize and
+ * sbsize) in one launch: plan i reads sources[i*n_sources + z] and writes
+ * slice z to dsts[i*tbsize + z]. Same results as n nf_dfttest_enqueue_volume
+ * calls. No counterpart in the reference (see nf_dfttest_enqueue_volume). */
+nf_status nf_dfttest_enqueue_volume_batch(const nf_plan* const* plans, int n, const nf_plane* sources,
+                                          int n_sources, const nf_mut_plane* dsts, unsigned* device_flags,
+                                          void* stream);
 
 /* Differentiable DFTTest Wiener path (tbsize=1, ftype=0, f0beta=1, float32
  * planes). No counterpart in the reference: for tuning the per-frequency

@@ -361,11 +361,15 @@ void choose_dft_tiling(nf_plan& p) {
     // one block fits the LDS, then grows the chunk within two workgroups
     // per CU. A shape that cannot fit is left unavailable (vol_lds = 0).
     const int T = p.temporal;
-    const size_t budget = 76 * 1024;
+    size_t budget = 76 * 1024;
+    if (const char* e = std::getenv("NF_VOL_TILE_H")) p.vol_tile_h[0] = p.vol_tile_h[2] = std::max(8, std::atoi(e));
+    if (const char* e = std::getenv("NF_VOL_LDS_KIB")) budget = (size_t)std::max(16, std::atoi(e)) * 1024;
     for (int m = 0; m < 3; ++m) {
       const int nout = m == 2 ? T : 1;
       p.vol_lds[m] = 0;
+      const int tw_env = std::getenv("NF_VOL_TILE_W") ? std::atoi(std::getenv("NF_VOL_TILE_W")) : 0;
       for (int tw : {64, 48, 32, 16}) {
+        if (tw_env && tw > tw_env) continue;
         const int acc_rows = m == 1 ? p.vol_tile_h[m] : p.B;
         if (dfttest_vol_lds_bytes(p.B, T, 1, p.S, nout, acc_rows, tw) > 160 * 1024) continue;
         p.vol_tile_w[m] = tw;
@@ -1547,34 +1551,66 @@ extern "C" nf_status nf_process_batch(const nf_plan* const* plans, int n_jobs, c
   });
 }
 
+namespace {
+// Job and temporal inputs of one volume pass (nf_dfttest_enqueue_volume).
+std::pair<DftJob, DftTemporal> volume_job(const nf_plan* plan, const nf_plane* sources, int n_sources,
+                                          const nf_mut_plane* dsts) {
+  require(plan && plan->algorithm == 0 && sources && dsts, "null argument");
+  require(plan->tmode == 1 && plan->tosize == 0 && plan->temporal > 1,
+          "volume processing needs a tmode=1, tosize=0, tbsize>1 DFTTest plan");
+  if (plan->generic) throw Unsupported("volume processing covers sbsize 8, 16 or 32 with smode=1");
+  require(!(plan->dither > 0 && plan->storage == kU8), "dithered DFTTest planes use nf_plan_enqueue");
+  for (int i = 0; i < n_sources; ++i)
+    check_plane(*plan, sources[i].width, sources[i].height, sources[i].stride_bytes, sources[i].data);
+  DftTemporal q = make_temporal(*plan, sources, n_sources, nullptr, 0, 2);
+  bool any = false;
+  for (int z = 0; z < plan->temporal; ++z) {
+    q.dsts[z] = dsts[z].data;
+    if (dsts[z].data) {
+      check_plane(*plan, dsts[z].width, dsts[z].height, dsts[z].stride_bytes, dsts[z].data);
+      any = true;
+    }
+  }
+  require(any, "no output slice requested");
+  nf_mut_plane first{};
+  for (int z = 0; z < plan->temporal && !first.data; ++z) first = dsts[z];
+  for (int z = 0; z < plan->temporal; ++z)
+    if (dsts[z].data) require(dsts[z].stride_bytes == first.stride_bytes, "volume outputs must share a stride");
+  return {make_dft_job(*plan, sources[0], first), q};
+}
+}  // namespace
+
 extern "C" nf_status nf_dfttest_enqueue_volume(const nf_plan* plan, const nf_plane* sources, int n_sources,
                                                const nf_mut_plane* dsts, int first_frame, int plane,
                                                unsigned* device_flags, void* stream) {
   return guarded([&]() -> nf_status {
-    require(plan && plan->algorithm == 0 && sources && dsts && device_flags, "null argument");
-    require(plan->tmode == 1 && plan->tosize == 0 && plan->temporal > 1,
-            "volume processing needs a tmode=1, tosize=0, tbsize>1 DFTTest plan");
-    if (plan->generic) throw Unsupported("volume processing covers sbsize 8, 16 or 32 with smode=1");
-    require(!(plan->dither > 0 && plan->storage == kU8), "dithered DFTTest planes use nf_plan_enqueue");
+    require(device_flags, "null argument");
     (void)first_frame;
     (void)plane;
-    for (int i = 0; i < n_sources; ++i)
-      check_plane(*plan, sources[i].width, sources[i].height, sources[i].stride_bytes, sources[i].data);
-    DftTemporal q = make_temporal(*plan, sources, n_sources, nullptr, 0, 2);
-    bool any = false;
-    for (int z = 0; z < plan->temporal; ++z) {
-      q.dsts[z] = dsts[z].data;
-      if (dsts[z].data) {
-        check_plane(*plan, dsts[z].width, dsts[z].height, dsts[z].stride_bytes, dsts[z].data);
-        any = true;
-      }
+    const auto jq = volume_job(plan, sources, n_sources, dsts);
+    enqueue_dft_vol_jobs({plan}, {jq.first}, {jq.second}, device_flags, (hipStream_t)stream);
+    return NF_OK;
+  });
+}
+
+extern "C" nf_status nf_dfttest_enqueue_volume_batch(const nf_plan* const* plans, int n, const nf_plane* sources,
+                                                     int n_sources, const nf_mut_plane* dsts, unsigned* device_flags,
+                                                     void* stream) {
+  return guarded([&]() -> nf_status {
+    require(plans && sources && dsts && device_flags && n > 0 && plans[0], "null argument");
+    const int T = plans[0]->temporal;
+    std::vector<const nf_plan*> ps;
+    std::vector<DftJob> jobs;
+    std::vector<DftTemporal> tq;
+    for (int i = 0; i < n; ++i) {
+      require(plans[i] && plans[i]->temporal == T, "volume batch plans must share tbsize");
+      require(plans[i]->B == plans[0]->B, "volume batch plans must share sbsize");
+      const auto jq = volume_job(plans[i], sources + (size_t)i * n_sources, n_sources, dsts + (size_t)i * T);
+      ps.push_back(plans[i]);
+      jobs.push_back(jq.first);
+      tq.push_back(jq.second);
     }
-    require(any, "no output slice requested");
-    nf_mut_plane first{};
-    for (int z = 0; z < plan->temporal && !first.data; ++z) first = dsts[z];
-    for (int z = 0; z < plan->temporal; ++z)
-      if (dsts[z].data) require(dsts[z].stride_bytes == first.stride_bytes, "volume outputs must share a stride");
-    enqueue_dft_vol_jobs({plan}, {make_dft_job(*plan, sources[0], first)}, {q}, device_flags, (hipStream_t)stream);
+    enqueue_dft_vol_jobs(ps, jobs, tq, device_flags, (hipStream_t)stream);
     return NF_OK;
   });
 }

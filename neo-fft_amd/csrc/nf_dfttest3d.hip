@@ -288,80 +288,89 @@ __device__ __forceinline__ void vol_tile(const DftJob& J, const DftTemporal& Q, 
             default: temporal_stages<B, 7>(J, Q, sre, sim, dc, nb, T, target, tid, nthreads, bad); break;
           }
         }
+        // F, G, H for every slice this pass writes. Slice o's spectrum (zs)
+        // and accumulator are its own, so the slices' transforms run side
+        // by side; each element sees the same operations as a one-slice pass.
+        __syncthreads();
+        // F: inverse column transforms
+        for (int t = tid; t < nout * nb * K; t += nthreads) {
+          const int o = t / (nb * K);
+          if (Q.emit_all && Q.dsts[o] == nullptr) continue;
+          const int zs = Q.emit_all ? o : 0;
+          const int rem = t - o * (nb * K);
+          const int bi = rem / K, k = rem - bi * K;
+          float re[B], im[B];
+          float* cr = sre + (bi * T + zs) * BS + k;
+          float* ci = sim + (bi * T + zs) * BS + k;
+#pragma unroll
+          for (int y = 0; y < B; ++y) {
+            re[y] = cr[y * KS];
+            im[y] = ci[y * KS];
+          }
+          cfft<B, false>(re, im);
+#pragma unroll
+          for (int y = 0; y < B; ++y) {
+            cr[y * KS] = re[y];
+            ci[y * KS] = im[y];
+          }
+        }
+        __syncthreads();
+        // G: inverse rows, normalisation, synthesis window
+        for (int t = tid; t < nout * nb * B; t += nthreads) {
+          const int o = t / (nb * B);
+          if (Q.emit_all && Q.dsts[o] == nullptr) continue;
+          const int zs = Q.emit_all ? o : 0;
+          const int rem = t - o * (nb * B);
+          const int bi = rem / B, y = rem - bi * B;
+          float re[B], im[B];
+          float* rr = sre + (bi * T + zs) * BS + y * KS;
+          float* ri = sim + (bi * T + zs) * BS + y * KS;
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            re[k] = rr[k];
+            im[k] = ri[k];
+          }
+#pragma unroll
+          for (int k = K; k < B; ++k) {
+            re[k] = re[B - k];
+            im[k] = 0.0f - im[B - k];
+          }
+          cfft<B, false>(re, im);
+          const float* hsrow = hs + (o * B + y) * OS;
+#pragma unroll
+          for (int x = 0; x < B; ++x) {
+            float v = re[x] * fct;
+            bad |= !isfinite(v);
+            if (Q.tmode1) v = volume * v;
+            if (x < KS) rr[x] = v * hsrow[x];
+            else ri[x - KS] = v * hsrow[x];
+          }
+        }
+        __syncthreads();
+        // H: overlap-add of this chunk, block columns ascending
         for (int o = 0; o < nout; ++o) {
           if (Q.emit_all && Q.dsts[o] == nullptr) continue;
           const int zs = Q.emit_all ? o : 0;
-          __syncthreads();
-          // F: inverse column transforms of the slice
-          for (int t = tid; t < nb * K; t += nthreads) {
-            const int bi = t / K, k = t - bi * K;
-            float re[B], im[B];
-            float* cr = sre + (bi * T + zs) * BS + k;
-            float* ci = sim + (bi * T + zs) * BS + k;
-#pragma unroll
-            for (int y = 0; y < B; ++y) {
-              re[y] = cr[y * KS];
-              im[y] = ci[y * KS];
-            }
-            cfft<B, false>(re, im);
-#pragma unroll
-            for (int y = 0; y < B; ++y) {
-              cr[y * KS] = re[y];
-              ci[y * KS] = im[y];
-            }
-          }
-          __syncthreads();
-          // G: inverse rows, normalisation, synthesis window
-          for (int t = tid; t < nb * B; t += nthreads) {
-            const int bi = t / B, y = t - bi * B;
-            float re[B], im[B];
-            float* rr = sre + (bi * T + zs) * BS + y * KS;
-            float* ri = sim + (bi * T + zs) * BS + y * KS;
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-              re[k] = rr[k];
-              im[k] = ri[k];
-            }
-#pragma unroll
-            for (int k = K; k < B; ++k) {
-              re[k] = re[B - k];
-              im[k] = 0.0f - im[B - k];
-            }
-            cfft<B, false>(re, im);
-            const float* hsrow = hs + (o * B + y) * OS;
-#pragma unroll
-            for (int x = 0; x < B; ++x) {
-              float v = re[x] * fct;
-              bad |= !isfinite(v);
-              if (Q.tmode1) v = volume * v;
-              if (x < KS) rr[x] = v * hsrow[x];
-              else ri[x - KS] = v * hsrow[x];
-            }
-          }
-          __syncthreads();
-          // H: overlap-add of this chunk, block columns ascending
-          {
-            const int r0 = max(oy, Y0), r1 = min(oy + B, Y1);
-            const int xa = max(X0, cx0), xb = min(X1, cx0 + WWc);
-            const int nr = r1 - r0, ncol = xb - xa;
-            float* A = acc + o * acc_rows * J.tile_w;
-            const int R = nthreads >= ncol ? nthreads / ncol : 1;
-            const int rl = R > 1 ? tid / ncol : 0;
-            for (int c = (R > 1 ? tid - rl * ncol : tid); c < ncol && rl < R; c += nthreads) {
-              const int x = xa + c;
-              const int b_first = max(cb, ceil_div_int(x - B + 1, S));
-              const int b_last = min(cb + nb - 1, x / S);
-              for (int rr = rl; rr < nr; rr += R) {
-                const int y = r0 + rr;
-                float* cell = A + (ring_mode ? (y % B) : (y - Y0)) * J.tile_w + (x - X0);
-                float sum = *cell;
-                for (int bx = b_first; bx <= b_last; ++bx) {
-                  const int xx = x - bx * S;
-                  const int row = ((bx - cb) * T + zs) * BS + (y - oy) * KS;
-                  sum = sum + (xx < KS ? sre[row + xx] : sim[row + xx - KS]);
-                }
-                *cell = sum;
+          const int r0 = max(oy, Y0), r1 = min(oy + B, Y1);
+          const int xa = max(X0, cx0), xb = min(X1, cx0 + WWc);
+          const int nr = r1 - r0, ncol = xb - xa;
+          float* A = acc + o * acc_rows * J.tile_w;
+          const int R = nthreads >= ncol ? nthreads / ncol : 1;
+          const int rl = R > 1 ? tid / ncol : 0;
+          for (int c = (R > 1 ? tid - rl * ncol : tid); c < ncol && rl < R; c += nthreads) {
+            const int x = xa + c;
+            const int b_first = max(cb, ceil_div_int(x - B + 1, S));
+            const int b_last = min(cb + nb - 1, x / S);
+            for (int rr = rl; rr < nr; rr += R) {
+              const int y = r0 + rr;
+              float* cell = A + (ring_mode ? (y % B) : (y - Y0)) * J.tile_w + (x - X0);
+              float sum = *cell;
+              for (int bx = b_first; bx <= b_last; ++bx) {
+                const int xx = x - bx * S;
+                const int row = ((bx - cb) * T + zs) * BS + (y - oy) * KS;
+                sum = sum + (xx < KS ? sre[row + xx] : sim[row + xx - KS]);
               }
+              *cell = sum;
             }
           }
         }

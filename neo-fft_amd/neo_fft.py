@@ -123,6 +123,9 @@ _SIGNATURES = {
     "nf_dfttest_enqueue_volume": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Plane), ctypes.c_int,
                                                  ctypes.POINTER(MutPlane), ctypes.c_int, ctypes.c_int,
                                                  ctypes.c_void_p, ctypes.c_void_p]),
+    "nf_dfttest_enqueue_volume_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Plane),
+                                                       ctypes.c_int, ctypes.POINTER(MutPlane), ctypes.c_void_p,
+                                                       ctypes.c_void_p]),
     "nf_dfttest_wiener_forward": (ctypes.c_int, [ctypes.c_void_p, Plane, MutPlane, ctypes.c_void_p, ctypes.c_void_p,
                                                  ctypes.c_void_p]),
     "nf_dfttest_wiener_backward": (ctypes.c_int, [ctypes.c_void_p, Plane, Plane, MutPlane, ctypes.c_void_p,
@@ -332,6 +335,20 @@ def dfttest_volume(plan, sources, dsts, flags, first_frame=0, plane=0, stream=No
     outs = (MutPlane * len(dsts))(*[MutPlane(None, 0, 0, 0) if d is None else _plane(d, MutPlane) for d in dsts])
     _check(lib().nf_dfttest_enqueue_volume(plan.handle, srcs, len(sources), outs, first_frame, plane,
                                            ctypes.c_void_p(flags.data_ptr()), _stream_handle(stream)))
+
+
+def dfttest_volume_batch(plans, sources, dsts, flags, stream=None):
+    """Volume passes of several plans in one launch: sources[i] and dsts[i]
+    are plan i's lists as in dfttest_volume."""
+    n = len(plans)
+    ns = len(sources[0])
+    T = len(dsts[0])
+    srcs = (Plane * (n * ns))(*[_plane(s_) for row in sources for s_ in row])
+    outs = (MutPlane * (n * T))(*[MutPlane(None, 0, 0, 0) if d is None else _plane(d, MutPlane)
+                                  for row in dsts for d in row])
+    hs = (ctypes.c_void_p * n)(*[p.handle for p in plans])
+    _check(lib().nf_dfttest_enqueue_volume_batch(hs, n, srcs, ns, outs, ctypes.c_void_p(flags.data_ptr()),
+                                                 _stream_handle(stream)))
 
 
 def _ptr(t):

@@ -283,6 +283,34 @@ def test_dfttest_volume_matches_per_frame(nf, oracle):
             assert np.array_equal(to_host(outs[z], np.uint16), ref), (start, z)
 
 
+def test_dfttest_volume_batch_matches_single(nf, oracle):
+    """Three planes of different sizes in one volume launch give the outputs
+    of three single-plane passes, and those match the oracle."""
+    import torch
+    T, N = 5, 5
+    dims = [(64, 96), (32, 48), (40, 72)]
+    params = dict(tbsize=T, tmode=1, tosize=0, sbsize=16, sosize=12, ftype=0, sigma=8.0)
+    clips = [_clip_frames(N, h, w, 16, 5 + p) for p, (h, w) in enumerate(dims)]
+    plans = [nf.dfttest_plan(w, h, bits=16, **params) for (h, w) in dims]
+    flags = torch.zeros(1, dtype=torch.int32, device="cuda")
+    dev = [[to_device(f) for f in c] for c in clips]
+    outs_b = [[empty_like_np(c[0]) if z != 3 else None for z in range(T)] for c in clips]
+    outs_s = [[empty_like_np(c[0]) if z != 3 else None for z in range(T)] for c in clips]
+    nf.dfttest_volume_batch(plans, dev, outs_b, flags)
+    for p in range(3):
+        nf.dfttest_volume(plans[p], dev[p], outs_s[p], flags)
+    torch.cuda.synchronize()
+    assert int(flags.item()) == 0
+    for p in range(3):
+        for z in range(T):
+            if z == 3:
+                continue
+            got = to_host(outs_b[p][z], np.uint16)
+            assert np.array_equal(got, to_host(outs_s[p][z], np.uint16)), (p, z)
+            ref = oracle.dfttest_plane(clips[p], bits=16, targets=[z], **params)
+            assert np.array_equal(got, ref), (p, z, compare(got, ref))
+
+
 @pytest.mark.parametrize("mode", [1, 2, 4, 2147483647])
 @pytest.mark.parametrize("h,w", [(45, 70), (1100, 40)])
 def test_dfttest_dither(nf, oracle, mode, h, w):
