@@ -162,7 +162,7 @@ struct nf_plan {
   const float* d_ttw = nullptr;
   // temporal kernel tiling per mode: 0 per-frame ring, 1 per-frame tile (several
   // covering temporal blocks), 2 all slices of one volume
-  int vol_tile_w[3] = {64, 64, 64}, vol_tile_h[3] = {128, 32, 128}, vol_chunk[3] = {1, 1, 1};
+  int vol_tile_w[3] = {128, 64, 64}, vol_tile_h[3] = {64, 32, 128}, vol_chunk[3] = {1, 1, 1};
   size_t vol_lds[3] = {0, 0, 0};
   // FFT3D
   int bw = 0, bh = 0, bt = 1, kalman = 0, denoise = 1, varying = 0, enh_active = 0;
@@ -356,10 +356,11 @@ void choose_dft_tiling(nf_plan& p) {
   p.lds = p.dft_tiles[0].lds;
   if (p.temporal > 1) {
     // Three launch shapes: 0 = one covering temporal block (ring
-    // accumulator), 1 = several covering blocks (tile accumulator), 2 = all
-    // slices of a lattice block. Each takes the widest tile whose chunk of
-    // one block fits the LDS, then grows the chunk within two workgroups
-    // per CU. A shape that cannot fit is left unavailable (vol_lds = 0).
+    // accumulator, 128x64 tiles), 1 = several covering blocks (tile
+    // accumulator, 64x32), 2 = all slices of a lattice block (64x128). Each
+    // takes the widest tile up to that width whose chunk of one block fits
+    // the LDS, then grows the chunk within two workgroups per CU. A shape
+    // that cannot fit is left unavailable (vol_lds = 0).
     const int T = p.temporal;
     size_t budget = 76 * 1024;
     if (const char* e = std::getenv("NF_VOL_TILE_H")) p.vol_tile_h[0] = p.vol_tile_h[2] = std::max(8, std::atoi(e));
@@ -368,8 +369,8 @@ void choose_dft_tiling(nf_plan& p) {
       const int nout = m == 2 ? T : 1;
       p.vol_lds[m] = 0;
       const int tw_env = std::getenv("NF_VOL_TILE_W") ? std::atoi(std::getenv("NF_VOL_TILE_W")) : 0;
-      for (int tw : {64, 48, 32, 16}) {
-        if (tw_env && tw > tw_env) continue;
+      for (int tw : {128, 96, 64, 48, 32, 16}) {
+        if (tw > (tw_env ? tw_env : p.vol_tile_w[m])) continue;
         const int acc_rows = m == 1 ? p.vol_tile_h[m] : p.B;
         if (dfttest_vol_lds_bytes(p.B, T, 1, p.S, nout, acc_rows, tw) > 160 * 1024) continue;
         p.vol_tile_w[m] = tw;

@@ -58,20 +58,42 @@ __device__ __forceinline__ Cpx inverse_slice(int T, const Cpx* v, int s, const f
   return dft_term(T, v, s, ttw, false);
 }
 
-// Stages D and E of one chunk. TT > 0 fixes the temporal depth at compile
-// time (T == TT); TT == 0 handles any depth up to 15.
-// WIENER: the plain Wiener gain (gain_is_wiener), evaluated without the
-// per-bin dispatch.
+// Forward temporal transform of one bin's T values (fft_backend.cpp:397-420
+// for depths 3..5, a direct DFT otherwise).
+template <int NT>
+__device__ __forceinline__ void temporal_forward(int T, Cpx* v, const float* ttw) {
+  if (T >= 3 && T <= 5) {
+    temporal_butterfly(T, v, true);
+  } else {
+    Cpx w[NT];
+#pragma unroll
+    for (int m = 0; m < NT; ++m)
+      if (m < T) w[m] = dft_term(T, v, m, ttw, true);
+#pragma unroll
+    for (int m = 0; m < NT; ++m)
+      if (m < T) v[m] = w[m];
+  }
+}
+
+// Stages D and E of one chunk in one pass per bin: forward temporal
+// transform, gain with 3D mean removal, inverse temporal transform. The mean
+// term needs the block's temporal DC (slot 0 of bin 0 after the forward
+// transform); every thread recomputes it from the slices' spatial DC terms
+// that stage C left in sdc[block][slice], with the same operations as the
+// bin-0 thread. TT > 0 fixes the temporal depth at compile time (T == TT);
+// TT == 0 handles any depth up to 15. WIENER: the plain Wiener gain
+// (gain_is_wiener), evaluated without the per-bin dispatch.
 template <int B, int TT, bool WIENER = false>
-__device__ __forceinline__ void temporal_stages(const DftJob& J, const DftTemporal& Q, float* sre, float* sim, float* dc, int nb,
-                                                int Tdyn, int target, int tid, int nthreads, bool& bad) {
+__device__ __forceinline__ void temporal_stages(const DftJob& J, const DftTemporal& Q, float* sre, float* sim,
+                                                const float* sdc, int nb, int Tdyn, int target, int tid,
+                                                int nthreads, bool& bad) {
   constexpr int K = B / 2 + 1;
   constexpr int BINS2 = B * K;
   constexpr int KS = vol_row_stride(B);
   constexpr int BS = vol_block_stride(B);
   constexpr int NT = TT > 0 ? TT : 15;
   const int T = TT > 0 ? TT : Tdyn;
-  // D: forward temporal transform per bin
+  const bool use_grid = J.grid && J.mean_scale != 0.0f && J.grid[0].x != 0.0f;
   for (int t = tid; t < nb * BINS2; t += nthreads) {
     const int bi = t / BINS2, bin = t - bi * BINS2;
     const int at = (bin / K) * KS + (bin - (bin / K) * K);
@@ -81,42 +103,20 @@ __device__ __forceinline__ void temporal_stages(const DftJob& J, const DftTempor
 #pragma unroll
     for (int z = 0; z < NT; ++z)
       if (z < T) v[z] = Cpx{br[z * BS], bim[z * BS]};
-    if (T >= 3 && T <= 5) {
-      temporal_butterfly(T, v, true);
-    } else {
-      Cpx w[NT];
+    temporal_forward<NT>(T, v, Q.ttw);
+    float gs = 0.0f;
+    if (use_grid) {
+      Cpx u[NT];
 #pragma unroll
-      for (int m = 0; m < NT; ++m)
-        if (m < T) w[m] = dft_term(T, v, m, Q.ttw, true);
-#pragma unroll
-      for (int m = 0; m < NT; ++m)
-        if (m < T) v[m] = w[m];
+      for (int z = 0; z < NT; ++z)
+        if (z < T) u[z] = Cpx{sdc[2 * (bi * T + z)], sdc[2 * (bi * T + z) + 1]};
+      temporal_forward<NT>(T, u, Q.ttw);
+      gs = (J.mean_scale * u[0].r) / J.grid[0].x;
     }
-#pragma unroll
-    for (int z = 0; z < NT; ++z)
-      if (z < T) {
-        br[z * BS] = v[z].r;
-        bim[z * BS] = v[z].i;
-      }
-    if (bin == 0) {
-      dc[2 * bi] = v[0].r;
-      dc[2 * bi + 1] = v[0].i;
-    }
-  }
-  __syncthreads();
-  // E: gain with 3D mean removal, inverse temporal transform
-  const bool use_grid = J.grid && J.mean_scale != 0.0f && J.grid[0].x != 0.0f;
-  for (int t = tid; t < nb * BINS2; t += nthreads) {
-    const int bi = t / BINS2, bin = t - bi * BINS2;
-    const float gs = use_grid ? (J.mean_scale * dc[2 * bi]) / J.grid[0].x : 0.0f;
-    const int at = (bin / K) * KS + (bin - (bin / K) * K);
-    Cpx v[NT];
-    float* br = sre + (bi * T) * BS + at;
-    float* bim = sim + (bi * T) * BS + at;
 #pragma unroll
     for (int z = 0; z < NT; ++z) {
       if (z >= T) continue;
-      float re = br[z * BS], im = bim[z * BS];
+      float re = v[z].r, im = v[z].i;
       const int k3 = z * BINS2 + bin;
       float mr = 0.0f, mi = 0.0f;
       if (J.grid) {
@@ -178,8 +178,8 @@ __device__ __forceinline__ void vol_tile(const DftJob& J, const DftTemporal& Q, 
   float* sim = sre + NBc * T * BS;                    // [NBc][T][BS] imaginary parts
   float* win = sim + NBc * T * BS;                    // [T][B][ww_c]
   float* acc = win + T * B * ww_c;                    // [nout][acc_rows][tile_w]
-  float* dc = acc + nout * acc_rows * J.tile_w;       // [NBc][2]
-  float* hw = dc + 2 * NBc;                           // [T][B][OS]
+  float* dc = acc + nout * acc_rows * J.tile_w;       // [NBc][T][2] spatial DC terms
+  float* hw = dc + 2 * NBc * T;                       // [T][B][OS]
   float* hs = hw + T * B * OS;                        // [nout][B][OS]
 
   bool bad = false;
@@ -263,9 +263,13 @@ __device__ __forceinline__ void vol_tile(const DftJob& J, const DftTemporal& Q, 
             cr[y * KS] = re[y];
             ci[y * KS] = im[y];
           }
+          if (k == 0) {
+            dc[2 * (bi * T + z)] = re[0];
+            dc[2 * (bi * T + z) + 1] = im[0];
+          }
         }
         __syncthreads();
-        // D: temporal transform per bin; E: gain over the 3D spectrum and
+        // D+E: temporal transform per bin, gain over the 3D spectrum and
         // inverse temporal transform. Depths 2..7 are compile-time so the
         // per-bin vectors stay in registers; the kAnyDepth instance takes
         // depths 8..15 (the host picks the instance from the depth).
@@ -437,7 +441,7 @@ size_t dfttest_vol_lds_bytes(int B, int T, int chunk, int S, int nout, int acc_r
   const size_t ww_c = (size_t)odd_stride((chunk - 1) * S + B);
   const size_t OS = (size_t)B + 1;
   return sizeof(float) * ((size_t)chunk * T * vol_block_stride(B) * 2 + (size_t)T * B * ww_c +
-                          (size_t)nout * acc_rows * tile_w + 2 * (size_t)chunk + (size_t)T * B * OS +
+                          (size_t)nout * acc_rows * tile_w + 2 * (size_t)chunk * T + (size_t)T * B * OS +
                           (size_t)nout * B * OS);
 }
 
