@@ -291,8 +291,8 @@ class Plan:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
-            lib().nf_plan_destroy(h)
+        if h and _lib is not None:  # module globals may be gone at interpreter exit
+            _lib.nf_plan_destroy(h)
             self._h = None
 
     @property
@@ -532,8 +532,8 @@ class KalmanState:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
-            lib().nf_kalman_state_destroy(h)
+        if h and _lib is not None:  # module globals may be gone at interpreter exit
+            _lib.nf_kalman_state_destroy(h)
             self._h = None
 
     @property
@@ -661,8 +661,8 @@ class FilteredClip:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
-            lib().nf_filter_destroy(h)
+        if h and _lib is not None:  # module globals may be gone at interpreter exit
+            _lib.nf_filter_destroy(h)
             self._h = None
 
     def requests(self, n):
