@@ -343,6 +343,17 @@ def other_configs(nf, torch, clip, stream, flags, args):
     out["dfttest_gray8_640x360"] = {"frames_per_s": round(96 / (ms / 1e3), 1), "ms_per_frame": round(ms / 96, 4),
                                     "bytes_per_frame": 460800,
                                     "achieved_gbs": round(460800 / (ms / 96 / 1e3) / 1e9, 2)}
+    # the same frames eight to a launch (a host that requests frames in
+    # parallel): one 640x360 plane alone fills about one workgroup per CU
+    g8 = [torch.empty_like(g[0]) for _ in range(8)]
+
+    def c1b(i):
+        nf.process_batch([(plan, [g[(8 * i + k) % 24]], g8[k]) for k in range(8)], flags, stream=stream)
+    for i in range(3):
+        c1b(i)
+    ms8 = _timed(torch, stream, c1b, 12)
+    out["dfttest_gray8_640x360"]["eight_frames_per_launch"] = {"frames_per_s": round(96 / (ms8 / 1e3), 1),
+                                                               "ms_per_frame": round(ms8 / 96, 4)}
 
     # 1080p YUV420P10 views of the resident clip (samples >> 6)
     p10 = [[(f.to(torch.int32) & 0xFFFF).div(64, rounding_mode="floor").to(torch.int16) for f in fr]

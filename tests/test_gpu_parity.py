@@ -218,6 +218,22 @@ def test_batch_matches_single(nf, oracle):
         assert np.array_equal(to_host(dst, np.uint16), ref)
 
 
+def test_batch_one_plan_several_frames(nf, oracle):
+    """One plan used by several jobs of one batch (frames of a clip in one
+    launch) gives each frame's own result."""
+    import torch
+    frames = _clip_frames(6, 45, 70, 8, 12)
+    params = dict(tbsize=1, sbsize=16, sosize=12, sigma=8.0)
+    plan = nf.dfttest_plan(70, 45, bits=8, **params)
+    dsts = [empty_like_np(f) for f in frames]
+    flags = torch.zeros(1, dtype=torch.int32, device="cuda")
+    nf.process_batch([(plan, [to_device(f)], d) for f, d in zip(frames, dsts)], flags)
+    torch.cuda.synchronize()
+    assert int(flags.item()) == 0
+    for f, d in zip(frames, dsts):
+        assert np.array_equal(to_host(d, np.uint8), oracle.dfttest_plane([f], bits=8, **params))
+
+
 # ------------------------------------------------------- DFTTest temporal
 def _clip_frames(n, h, w, bits, seed):
     rng = np.random.default_rng(seed)
