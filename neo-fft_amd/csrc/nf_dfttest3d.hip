@@ -221,9 +221,13 @@ __device__ __forceinline__ void vol_tile(const DftJob& J, const DftTemporal& Q, 
           }
         }
         __syncthreads();
+        // task index divisions by the chunk's row and column counts through
+        // float estimates (task indices < 2^23, divisors < 2^15: exact)
+        const int nbB = nb * B, nbK = nb * K;
+        const float inv_nbB = 1.0f / (float)nbB, inv_nbK = 1.0f / (float)nbK;
         // B: windowed rows of every slice
         for (int t = tid; t < T * nb * B; t += nthreads) {
-          const int z = t / (nb * B);
+          const int z = floor_div_est(t, nbB, inv_nbB);
           const int rem = t - z * (nb * B);
           const int bi = rem / B, y = rem - bi * B;
           float re[B], im[B];
@@ -246,7 +250,7 @@ __device__ __forceinline__ void vol_tile(const DftJob& J, const DftTemporal& Q, 
         __syncthreads();
         // C: column transforms of every slice
         for (int t = tid; t < T * nb * K; t += nthreads) {
-          const int z = t / (nb * K);
+          const int z = floor_div_est(t, nbK, inv_nbK);
           const int rem = t - z * (nb * K);
           const int bi = rem / K, k = rem - bi * K;
           float re[B], im[B];
@@ -298,7 +302,7 @@ __device__ __forceinline__ void vol_tile(const DftJob& J, const DftTemporal& Q, 
         __syncthreads();
         // F: inverse column transforms
         for (int t = tid; t < nout * nb * K; t += nthreads) {
-          const int o = t / (nb * K);
+          const int o = floor_div_est(t, nbK, inv_nbK);
           if (Q.emit_all && Q.dsts[o] == nullptr) continue;
           const int zs = Q.emit_all ? o : 0;
           const int rem = t - o * (nb * K);
@@ -321,7 +325,7 @@ __device__ __forceinline__ void vol_tile(const DftJob& J, const DftTemporal& Q, 
         __syncthreads();
         // G: inverse rows, normalisation, synthesis window
         for (int t = tid; t < nout * nb * B; t += nthreads) {
-          const int o = t / (nb * B);
+          const int o = floor_div_est(t, nbB, inv_nbB);
           if (Q.emit_all && Q.dsts[o] == nullptr) continue;
           const int zs = Q.emit_all ? o : 0;
           const int rem = t - o * (nb * B);
