@@ -372,15 +372,20 @@ def other_configs(nf, torch, clip, stream, flags, args):
         for p in range(3):
             states[p].enqueue_render(fr[p], outs[p], flags, stream=stream)
 
-    def c3a(i):
+    def c3b(i):
         fr = p10[i % len(p10)]
         nf.KalmanState.enqueue_advance_batch(states, fr, flags, stream=stream)
         nf.KalmanState.enqueue_render_batch(states, fr, outs, flags, stream=stream)
+
+    def c3a(i):
+        nf.KalmanState.enqueue_advance_render_batch(states, p10[i % len(p10)], outs, flags, stream=stream)
     for i in range(3):
         c3s(i)
+        c3b(i)
         c3a(i)
     steps = max(20, args.steps // 4)
     ms_single = _timed(torch, stream, c3s, steps)
+    ms_two = _timed(torch, stream, c3b, steps)
     ms = _timed(torch, stream, c3a, steps)
     state_bytes = sum(st.nbytes for st in states)
     kbytes = 2 * (W * H + 2 * (W // 2) * (H // 2)) * 2 + 2 * state_bytes
@@ -388,7 +393,8 @@ def other_configs(nf, torch, clip, stream, flags, args):
                                      "bytes_per_frame": kbytes,
                                      "achieved_gbs": round(kbytes / (ms / steps / 1e3) / 1e9, 1),
                                      "hbm_frac": round(kbytes / (ms / steps / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-                                     "launch": "planes batched: 3 launches per frame",
+                                     "launch": "planes batched, advance and render fused: 2 launches per frame",
+                                     "advance_then_render_batches_frames_per_s": round(steps / (ms_two / 1e3), 1),
                                      "per_plane_calls_frames_per_s": round(steps / (ms_single / 1e3), 1)}
 
     # DFTTest tbsize=5 tmode=1 tosize=0, one pass per lattice block writes 5 frames

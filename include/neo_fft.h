@@ -303,6 +303,15 @@ nf_status nf_kalman_enqueue_advance_batch(const nf_plan* const* plans, const nf_
 nf_status nf_kalman_enqueue_render_batch(const nf_plan* const* plans, const nf_plane* sources,
                                          const nf_mut_plane* dsts, const nf_kalman_state* const* states, int n,
                                          unsigned* device_flags, void* stream);
+/* Advance each state with its source, then render that source from the new
+ * state into dsts[i], in one block pass per plane (the new L feeds the render
+ * without a round trip through memory). Same states and outputs as the
+ * advance batch followed by the render batch. No counterpart in the
+ * reference, whose replay calls advance_kalman and render_kalman in turn
+ * (filter.hpp:196-230). */
+nf_status nf_kalman_enqueue_advance_render_batch(const nf_plan* const* plans, const nf_plane* sources,
+                                                 const nf_mut_plane* dsts, nf_kalman_state* const* states, int n,
+                                                 unsigned* device_flags, void* stream);
 /* Host access to the state (tests): last as interleaved complex, cov, proc. */
 nf_status nf_kalman_state_download(const nf_kalman_state* state, float* last, float* cov,
                                    float* proc);

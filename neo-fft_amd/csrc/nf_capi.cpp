@@ -2268,6 +2268,16 @@ Fft3dJob kalman_render_job(const nf_plan* plan, nf_plane source, nf_mut_plane ds
   return j;
 }
 
+// Advance with `source`, then render it from the new state, in one block pass.
+Fft3dJob kalman_advance_render_job(const nf_plan* plan, nf_plane source, nf_mut_plane dst, nf_kalman_state* state) {
+  Fft3dJob j = kalman_render_job(plan, source, dst, state);
+  require(state->plan, "Kalman state shape differs");
+  j.mode = 3;
+  j.src[0] = source.data;
+  j.src_stride[0] = source.stride_bytes;
+  return j;
+}
+
 void enqueue_kalman_advance(const nf_plan* plan, nf_plane source, nf_kalman_state* state, unsigned* flags,
                             hipStream_t stream) {
   require(flags != nullptr, "device flag word required");
@@ -2343,6 +2353,34 @@ extern "C" nf_status nf_kalman_enqueue_render_batch(const nf_plan* const* plans,
     for (int i = 0; i < n; ++i) {
       pv.push_back(plans[i]);
       jobs.push_back(kalman_render_job(plans[i], sources[i], dsts[i], states[i]));
+    }
+    enqueue_fft3d_jobs(pv, jobs, device_flags, (hipStream_t)stream, true);
+    return NF_OK;
+  });
+}
+
+extern "C" nf_status nf_kalman_enqueue_advance_render_batch(const nf_plan* const* plans, const nf_plane* sources,
+                                                            const nf_mut_plane* dsts, nf_kalman_state* const* states,
+                                                            int n, unsigned* device_flags, void* stream) {
+  return guarded([&]() -> nf_status {
+    require(plans && sources && dsts && states && n > 0, "null argument");
+    require(device_flags != nullptr, "device flag word required");
+    bool generic = false;
+    for (int i = 0; i < n; ++i) {
+      require(plans[i] && states[i] != nullptr, "null argument");
+      for (int k = 0; k < i; ++k) require(states[k] != states[i], "a Kalman state appears twice in one batch");
+      generic = generic || plans[i]->generic;
+    }
+    std::vector<const nf_plan*> pv(plans, plans + n);
+    std::vector<Fft3dJob> jobs;
+    if (generic) {
+      // blocks outside the codelet sizes: the two passes, same results
+      for (int i = 0; i < n; ++i) jobs.push_back(kalman_advance_job(plans[i], sources[i], states[i]));
+      enqueue_fft3d_jobs(pv, jobs, device_flags, (hipStream_t)stream, false);
+      jobs.clear();
+      for (int i = 0; i < n; ++i) jobs.push_back(kalman_render_job(plans[i], sources[i], dsts[i], states[i]));
+    } else {
+      for (int i = 0; i < n; ++i) jobs.push_back(kalman_advance_render_job(plans[i], sources[i], dsts[i], states[i]));
     }
     enqueue_fft3d_jobs(pv, jobs, device_flags, (hipStream_t)stream, true);
     return NF_OK;

@@ -187,7 +187,7 @@ struct Fft3dJob {
   int storage;
   int T;            // temporal slots used (1 = spatial / end-of-clip fallback)
   int c;            // centre slot
-  int mode;         // 0 stateless, 1 Kalman advance, 2 Kalman render
+  int mode;         // 0 stateless, 1 Kalman advance, 2 Kalman render, 3 advance then render
   float base;       // integer chroma midpoint
   float peak;
   float degrid;     // mean_scale

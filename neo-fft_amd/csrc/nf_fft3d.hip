@@ -187,8 +187,8 @@ __device__ __forceinline__ void temporal_pass_steady(const Fft3dJob& J, float* s
   }
 }
 
-// kalman.cpp:3-24 for one bin.
-__device__ __forceinline__ void kalman_bin(const Fft3dJob& J, long long s, int k, float xr, float xi, bool& bad) {
+// kalman.cpp:3-24 for one bin; returns the new L it stores.
+__device__ __forceinline__ float2 kalman_bin(const Fft3dJob& J, long long s, int k, float xr, float xi, bool& bad) {
   const float r = J.k_pattern ? max_ref(J.k_pattern[k], 1e-15f) : J.k_r0;
   const float threshold = r * J.k_ratio2;
   const float2 l = J.k_last[s];
@@ -200,7 +200,7 @@ __device__ __forceinline__ void kalman_bin(const Fft3dJob& J, long long s, int k
     J.k_last[s] = make_float2(xr, xi);
     J.k_cov[s] = r;
     J.k_proc[s] = r;
-    return;
+    return make_float2(xr, xi);
   }
   const float sum = c + q;
   const float gain = sum / (sum + r);
@@ -212,6 +212,7 @@ __device__ __forceinline__ void kalman_bin(const Fft3dJob& J, long long s, int k
   J.k_last[s] = make_float2(nr, ni);
   J.k_cov[s] = nc;
   J.k_proc[s] = nq;
+  return make_float2(nr, ni);
 }
 
 }  // namespace
@@ -558,27 +559,42 @@ __global__ __launch_bounds__(64) void fft3d_block_kernel(const Fft3dLaunch L) {
     }
   }
 
-  if (J.mode == 1) {
-    // Kalman advance: update the recurrence with this frame's spectra.
+  if (J.mode == 1 || J.mode == 3) {
+    // Kalman advance: update the recurrence with this frame's spectra. Mode
+    // 3 renders from the new L at once: each lane puts the value it stored
+    // in place of the spectrum bin it read (the render path's private copy
+    // of L, plan.cpp:471-474).
     for (int t = tid; t < nb * BINS; t += nthreads) {
       const int bi = t / BINS, k = t - bi * BINS;
       const long long s = (block_base + bi) * BINS + k;
-      kalman_bin(J, s, k, sre[bi * BS + k], sim[bi * BS + k], bad);
+      const float2 l = kalman_bin(J, s, k, sre[bi * BS + k], sim[bi * BS + k], bad);
+      if (J.mode == 3) {
+        sre[bi * BS + k] = l.x;
+        sim[bi * BS + k] = l.y;
+        if (k == 0) {
+          dc[2 * bi] = l.x;
+          dc[2 * bi + 1] = l.y;
+        }
+      }
     }
-    if (bad) atomicOr(L.flags, NF_FLAG_NONFINITE);
-    return;
+    if (J.mode == 1) {
+      if (bad) atomicOr(L.flags, NF_FLAG_NONFINITE);
+      return;
+    }
   }
 
-  if (J.mode == 2) {
+  if (J.mode >= 2) {
     // Kalman render: private copy of L (plan.cpp:471-474)
-    for (int t = tid; t < nb * BINS; t += nthreads) {
-      const int bi = t / BINS, k = t - bi * BINS;
-      const float2 l = J.k_last[(block_base + bi) * BINS + k];
-      sre[bi * BS + k] = l.x;
-      sim[bi * BS + k] = l.y;
-      if (k == 0) {
-        dc[2 * bi] = l.x;
-        dc[2 * bi + 1] = l.y;
+    if (J.mode == 2) {
+      for (int t = tid; t < nb * BINS; t += nthreads) {
+        const int bi = t / BINS, k = t - bi * BINS;
+        const float2 l = J.k_last[(block_base + bi) * BINS + k];
+        sre[bi * BS + k] = l.x;
+        sim[bi * BS + k] = l.y;
+        if (k == 0) {
+          dc[2 * bi] = l.x;
+          dc[2 * bi + 1] = l.y;
+        }
       }
     }
     __syncthreads();
@@ -629,7 +645,7 @@ __global__ __launch_bounds__(64) void fft3d_block_kernel(const Fft3dLaunch L) {
 
   // enhancement-only pass (plan.cpp:325-329), a separate sweep over the bins
   // so the inverse column transforms keep a small register footprint
-  const bool enhance = J.mode == 2 || J.T > 1 || !J.denoise ? (J.enh.sharpen != 0.0f || J.enh.dehalo != 0.0f)
+  const bool enhance = J.mode >= 2 || J.T > 1 || !J.denoise ? (J.enh.sharpen != 0.0f || J.enh.dehalo != 0.0f)
                                                             : J.enh_separate != 0;
   if (enhance) {
     for (int t = tid; t < nb * BINS; t += nthreads) {

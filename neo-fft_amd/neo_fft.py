@@ -151,6 +151,9 @@ _SIGNATURES = {
     "nf_kalman_enqueue_render_batch": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(Plane),
                                                       ctypes.POINTER(MutPlane), ctypes.POINTER(ctypes.c_void_p),
                                                       ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "nf_kalman_enqueue_advance_render_batch": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(Plane),
+                                                              ctypes.POINTER(MutPlane), ctypes.POINTER(ctypes.c_void_p),
+                                                              ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     "nf_kalman_enqueue_advance": (ctypes.c_int, [ctypes.c_void_p, Plane, ctypes.c_void_p, ctypes.c_void_p,
                                                  ctypes.c_void_p]),
     "nf_kalman_enqueue_render": (ctypes.c_int, [ctypes.c_void_p, Plane, MutPlane, ctypes.c_void_p, ctypes.c_void_p,
@@ -578,6 +581,19 @@ class KalmanState:
         hs = (ctypes.c_void_p * n)(*[st._h for st in states])
         _check(lib().nf_kalman_enqueue_render_batch(plans, srcs, outs, hs, n, ctypes.c_void_p(flags.data_ptr()),
                                                     _stream_handle(stream)))
+
+    @staticmethod
+    def enqueue_advance_render_batch(states, sources, dsts, flags, stream=None):
+        """nf_kalman_enqueue_advance_render_batch: advance, then render from
+        the new state, one block pass per plane."""
+        n = len(states)
+        plans = (ctypes.c_void_p * n)(*[st.plan.handle for st in states])
+        srcs = (Plane * n)(*[_plane(s_) for s_ in sources])
+        outs = (MutPlane * n)(*[_plane(d, MutPlane) for d in dsts])
+        hs = (ctypes.c_void_p * n)(*[st._h for st in states])
+        _check(lib().nf_kalman_enqueue_advance_render_batch(plans, srcs, outs, hs, n,
+                                                            ctypes.c_void_p(flags.data_ptr()),
+                                                            _stream_handle(stream)))
 
     def download(self):
         import numpy as np

@@ -429,9 +429,9 @@ def test_fft3d_spectrum_cache_batch(nf, oracle):
 
 
 def test_kalman_batched_planes_match_single(nf, oracle):
-    """The batched Kalman calls (all planes of a frame in one launch) give
-    the same states and outputs as one call per plane, and both match the
-    oracle frame by frame."""
+    """The batched Kalman calls (all planes of a frame in one launch) and the
+    fused advance+render batch give the same states and outputs as one call
+    per plane, and all match the oracle frame by frame."""
     import torch
     frames = yuv420_sinusoids(96, 64, frames=5, bits=10, seed=4)
     params = dict(bw=32, bh=32, bt=0, sigma=2.0, kratio=2.0)
@@ -439,6 +439,7 @@ def test_kalman_batched_planes_match_single(nf, oracle):
     plans = [nf.fft3d_plan(w, h, bits=10, chroma=p > 0, **params) for p, (w, h) in enumerate(dims)]
     batched = [nf.KalmanState(pl) for pl in plans]
     single = [nf.KalmanState(pl) for pl in plans]
+    fused = [nf.KalmanState(pl) for pl in plans]
     refs = [oracle.KalmanState(params, w, h, bits=10) for (w, h) in dims]
     flags = torch.zeros(1, dtype=torch.int32, device="cuda")
     for fr in frames:
@@ -451,6 +452,8 @@ def test_kalman_batched_planes_match_single(nf, oracle):
         nf.KalmanState.enqueue_render_batch(batched, dev, outs_b, flags)
         for p in range(3):
             single[p].enqueue_render(dev[p], outs_s[p], flags)
+        outs_f = [empty_like_np(pl) for pl in fr]
+        nf.KalmanState.enqueue_advance_render_batch(fused, dev, outs_f, flags)
         torch.cuda.synchronize()
         assert int(flags.item()) == 0
         for p in range(3):
@@ -458,7 +461,10 @@ def test_kalman_batched_planes_match_single(nf, oracle):
             ref = refs[p].render(chroma=p > 0)
             got_b, got_s = to_host(outs_b[p], np.uint16), to_host(outs_s[p], np.uint16)
             assert np.array_equal(got_b, got_s), p
+            assert np.array_equal(to_host(outs_f[p], np.uint16), got_s), p
             assert np.array_equal(got_b, ref), (p, compare(got_b, ref))
             lb, cb, qb = batched[p].download()
             ls, cs, qs = single[p].download()
+            lf, cf, qf = fused[p].download()
             assert np.array_equal(lb, ls) and np.array_equal(cb, cs) and np.array_equal(qb, qs)
+            assert np.array_equal(lf, ls) and np.array_equal(cf, cs) and np.array_equal(qf, qs)
