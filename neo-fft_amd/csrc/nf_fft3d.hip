@@ -9,7 +9,7 @@
 // (spatial Wiener + sharpen/dehalo, temporal T-point DFT Wiener, Kalman
 // update or Kalman render), the inverse FFT and the horizontal synthesis
 // window, and writes each block's windowed output to a contribution buffer.
-// Stage 2 (fft3d_synthesis_kernel): one lane per visible pixel sums the
+// Stage 2 (fft3d_synthesis_kernel): one lane per four visible pixels sums the
 // contributions in the reference's two-level canonical order -- block
 // columns left to right into a row value, times the vertical synthesis
 // weight, block rows top to bottom (plan.cpp:486-490, 520-535) -- and
@@ -942,26 +942,7 @@ hipError_t launch_fft3d_generic_block(const GenericFft3dArgs& A, size_t lds, hip
   return hipGetLastError();
 }
 
-// Vertical synthesis of one cover pixel (x, y): block columns left to right
-// into a row value, times the vertical synthesis weight, block rows top to
-// bottom.
-__device__ __forceinline__ float synthesize_pixel(const Fft3dJob& J, int x, int y) {
-  const int BW = J.gx.block, BH = J.gy.block, Sx = J.gx.step, Sy = J.gy.step;
-  const int by_first = max(0, ceil_div_int(y - BH + 1, Sy)), by_last = min(J.gy.count - 1, y / Sy);
-  const int bx_first = max(0, ceil_div_int(x - BW + 1, Sx)), bx_last = min(J.gx.count - 1, x / Sx);
-  float acc = 0.0f;
-  for (int by = by_first; by <= by_last; ++by) {
-    const int yy = y - by * Sy;
-    float r = 0.0f;
-    for (int bx = bx_first; bx <= bx_last; ++bx) {
-      const long long blk = (long long)by * J.gx.count + bx;
-      r = r + J.contrib[(blk * BH + yy) * BW + (x - bx * Sx)];
-    }
-    acc = acc + r * J.wys[yy];
-  }
-  return acc;
-}
-
+// One output sample: FFT3D midpoint rounding and clamp, or the float clamp.
 __device__ __forceinline__ void store_fft3d_output(const Fft3dJob& J, int oy, int ox, float acc) {
   char* drow = static_cast<char*>(J.dst) + (long long)oy * J.dst_stride;
   if (J.storage == kF32) {
@@ -988,13 +969,6 @@ __global__ __launch_bounds__(256) void fft3d_synthesis_kernel(const Fft3dLaunch 
   const Fft3dJob& J = L.jobs[jid];
   const int u = (int)(gid - unit_start[jid]);  // units per job < 2^31
   const int w = J.gx.length;
-  if (!synthesis_vector(J)) {
-    const int oy = u / w, ox = u - oy * w;
-    const float acc = synthesize_pixel(J, ox + J.gx.offset, oy + J.gy.offset);
-    if (!isfinite(acc)) atomicOr(L.flags, NF_FLAG_NONFINITE);
-    store_fft3d_output(J, oy, ox, acc);
-    return;
-  }
   const int gpr = synthesis_groups(J);
   const int BW = J.gx.block, BH = J.gy.block, Sx = J.gx.step, Sy = J.gy.step;
   // integer divisions by job constants through float estimates (exact after
@@ -1004,22 +978,48 @@ __global__ __launch_bounds__(256) void fft3d_synthesis_kernel(const Fft3dLaunch 
   const int x0 = (J.gx.offset / 4 + gi) * 4;
   const int y = oy + J.gy.offset;
   const int by_first = max(0, ceil_div_est(y - BH + 1, Sy, inv_sy)), by_last = min(J.gy.count - 1, floor_div_est(y, Sy, inv_sy));
-  const int bx_first = max(0, ceil_div_est(x0 - BW + 1, Sx, inv_sx)), bx_last = min(J.gx.count - 1, floor_div_est(x0, Sx, inv_sx));
   float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-  for (int by = by_first; by <= by_last; ++by) {
-    const int yy = y - by * Sy;
-    float r[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int bx = bx_first; bx <= bx_last; ++bx) {
-      const long long blk = (long long)by * J.gx.count + bx;
-      const float4 c = *reinterpret_cast<const float4*>(J.contrib + (blk * BH + yy) * BW + (x0 - bx * Sx));
-      r[0] = r[0] + c.x;
-      r[1] = r[1] + c.y;
-      r[2] = r[2] + c.z;
-      r[3] = r[3] + c.w;
-    }
-    const float wy = J.wys[yy];
+  if (synthesis_vector(J)) {
+    // the four pixels share their covering blocks; one float4 per block
+    const int bx_first = max(0, ceil_div_est(x0 - BW + 1, Sx, inv_sx)), bx_last = min(J.gx.count - 1, floor_div_est(x0, Sx, inv_sx));
+    for (int by = by_first; by <= by_last; ++by) {
+      const int yy = y - by * Sy;
+      float r[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      for (int bx = bx_first; bx <= bx_last; ++bx) {
+        const long long blk = (long long)by * J.gx.count + bx;
+        const float4 c = *reinterpret_cast<const float4*>(J.contrib + (blk * BH + yy) * BW + (x0 - bx * Sx));
+        r[0] = r[0] + c.x;
+        r[1] = r[1] + c.y;
+        r[2] = r[2] + c.z;
+        r[3] = r[3] + c.w;
+      }
+      const float wy = J.wys[yy];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) acc[e] = acc[e] + r[e] * wy;
+      for (int e = 0; e < 4; ++e) acc[e] = acc[e] + r[e] * wy;
+    }
+  } else {
+    // block step or width not a multiple of four: each pixel keeps its own
+    // covering-block range (the same adds in the same order as one pixel
+    // per lane), the four pixels' loads issued side by side
+    int bf[4], bl[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      bf[e] = max(0, ceil_div_est(x0 + e - BW + 1, Sx, inv_sx));
+      bl[e] = min(J.gx.count - 1, floor_div_est(x0 + e, Sx, inv_sx));
+    }
+    for (int by = by_first; by <= by_last; ++by) {
+      const int yy = y - by * Sy;
+      float r[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      for (int bx = bf[0]; bx <= bl[3]; ++bx) {
+        const float* row = J.contrib + ((long long)(by * J.gx.count + bx) * BH + yy) * BW - bx * Sx;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (bx >= bf[e] && bx <= bl[e]) r[e] = r[e] + row[x0 + e];
+      }
+      const float wy = J.wys[yy];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc[e] = acc[e] + r[e] * wy;
+    }
   }
   bool bad = false;
   const int ox0 = x0 - J.gx.offset;
@@ -1060,7 +1060,7 @@ __global__ __launch_bounds__(256) void fft3d_synthesis_kernel(const Fft3dLaunch 
 }
 
 long long fft3d_synthesis_units(const Fft3dJob& J) {
-  return synthesis_vector(J) ? (long long)J.gy.length * synthesis_groups(J) : (long long)J.gy.length * J.gx.length;
+  return (long long)J.gy.length * synthesis_groups(J);
 }
 
 size_t fft3d_block_lds_bytes(int BW, int BH, int T, int NB, int Sx, int storage) {
