@@ -340,9 +340,10 @@ void choose_dft_tiling(nf_plan& p) {
     p.dft_tiles.push_back(dft_tile(p.B, p.S, tw ? std::max(8, std::atoi(tw)) : 64, th ? std::max(8, std::atoi(th)) : 128));
   } else {
     // widths 96 and 48 give sbsize 16 / sosize 12 row and column passes that
-    // fill the 256 lanes of a workgroup closely (27 and 15 blocks per row)
+    // fill the 256 lanes of a workgroup closely (27 and 15 blocks per row);
+    // the small tiles give small planes more workgroups than CUs
     static const int cand[][2] = {{64, 128}, {64, 64}, {32, 64}, {32, 32}, {128, 64}, {128, 128},
-                                  {96, 128}, {96, 64},  {48, 128}};
+                                  {96, 128}, {96, 64},  {48, 128}, {32, 16}, {16, 32}, {16, 16}};
     for (const auto& c : cand) {
       const auto t = dft_tile(p.B, p.S, c[0], c[1]);
       if (t.lds <= 160 * 1024) p.dft_tiles.push_back(t);
